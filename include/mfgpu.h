@@ -1,0 +1,173 @@
+/*
+ * mfgpu.h -- C ABI of the MI355X numeric phase for PureUMFPACK.jl's supernodal
+ * multifrontal LU.
+ *
+ * The reference has no FFI: its boundary is the Julia call surface
+ *   splu(A; method=:multifrontal, scale, check)      src/interface.jl:93-120
+ *   multifrontal_lu(A; q, tol, check)                src/multifrontal.jl:146-149
+ *   solve(F::PureLU, b; refine), F \ b               src/interface.jl:170-179
+ * Julia keeps ordering, scaling and symbolic analysis (src/amd.jl, src/scaling.jl,
+ * src/symbolic.jl) and hands their results to mf_plan_create; every numeric step of
+ * the supernode loop (src/multifrontal.jl:198-357), the triangular solves
+ * (src/solve.jl:10-39, src/interface.jl:123-138) and the refinement loop
+ * (src/interface.jl:170-177) runs on one GPU behind the calls below.
+ *
+ * Conventions
+ *   - index arrays are 1-based int64 (Julia's Vector{Int}); values are column-major;
+ *   - the caller owns every host array; the library never keeps a host pointer after a
+ *     call returns; device memory belongs to the plan and is freed by mf_plan_destroy;
+ *   - functions return MF_OK (0) or one of the error codes; mf_last_error() gives the
+ *     text of the calling thread's last error;
+ *   - a plan is not re-entrant: calls on one plan must be serialised by the caller.
+ *     Calls are synchronous from the caller's point of view.
+ */
+#ifndef MFGPU_H
+#define MFGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum mf_status {
+    MF_OK = 0,
+    MF_SINGULAR = 1, /* exactly-zero block pivot with check != 0: *info_col is the column */
+    MF_DIM = 2,      /* size mismatch (DimensionMismatch in the reference) */
+    MF_BADARG = 3,   /* bad option / argument (ArgumentError) */
+    MF_HIP = 4,      /* HIP runtime error */
+    MF_OOM = 5       /* device allocation failed */
+};
+
+enum mf_precision {
+    MF_FP64 = 0, /* parity path: fronts, getrf, trsm, Schur gemm in FP64 */
+    MF_FP32 = 1, /* fronts in FP32 (FP32 accumulate); FP64 residual and update */
+    MF_BF16 = 2  /* FP32 fronts/getrf/trsm, Schur gemm operands in BF16 with per-tile
+                    power-of-two scaling, FP32 accumulate; FP64 residual and update */
+};
+
+enum mf_ordering { MF_ORDER_AMD = 0, MF_ORDER_COLAMD = 1, MF_ORDER_GIVEN = 2 };
+
+typedef struct mf_options {
+    int32_t precision; /* enum mf_precision */
+    int32_t small_nf;  /* fronts with nf <= small_nf take the one-workgroup fused kernel;
+                          0 = library default */
+    int32_t use_graph; /* 1: capture the factor schedule in a hipGraph and replay it */
+    int32_t nbatch;    /* matrices sharing the pattern that one plan holds (>= 1) */
+    int32_t reserved[4];
+} mf_options;
+
+typedef struct mf_solve_report {
+    int32_t iterations;     /* refinement steps taken */
+    int32_t converged;      /* 1 if bwd_tol > 0 and the bound was met */
+    double backward_error;  /* max over RHS of ||b-Ax||_inf / (||A||_inf ||x||_inf + ||b||_inf) */
+    double residual_2;      /* max over RHS of ||b-Ax||_2 / ||b||_2 (src test metric) */
+    double seconds;         /* device time of the whole call */
+} mf_solve_report;
+
+/* ---- host symbolic analysis (the reference keeps this in Julia; offered for non-Julia
+ *      hosts and for tests).  Restates symbolic_mf (src/symbolic.jl:211-226) with AMD on
+ *      A+A^T (src/amd.jl:141-148) or on A^T A (colamd_order, src/amd.jl:183-188). ---- */
+typedef struct mf_symbolic mf_symbolic;
+
+/* Ap/Ai: CSC pattern of A (1-based).  ordering = MF_ORDER_GIVEN uses q (1-based, length n)
+ * as the pre-postorder column order, like symbolic_mf(A; q). */
+int mf_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, int32_t ordering,
+               const int64_t *q, mf_symbolic **out);
+/* Sizes: nsuper, total update-row entries (sum of |colstruct[lastcol(s)]|), total child
+ * entries, and sum over all columns of |colstruct[j]| (predicted_fill = 2(n + ncolstruct)). */
+int mf_symbolic_sizes(const mf_symbolic *S, int64_t *nsuper, int64_t *nupd, int64_t *nchild,
+                      int64_t *ncolstruct);
+/* Copies the analysis out (all 1-based): qf (n), parent (n, 0 = root), sstart (nsuper+1),
+ * upd_ptr (nsuper+1) / upd_idx (nupd): colstruct of each supernode's last column,
+ * child_ptr (nsuper+1) / child_idx (nchild): childsupers, ascending.  Any pointer may be
+ * NULL to skip that array. */
+int mf_symbolic_export(const mf_symbolic *S, int64_t *qf, int64_t *parent, int64_t *sstart,
+                       int64_t *upd_ptr, int64_t *upd_idx, int64_t *child_ptr,
+                       int64_t *child_idx);
+void mf_symbolic_destroy(mf_symbolic *S);
+
+/* ---- the numeric phase ---- */
+typedef struct mf_plan mf_plan;
+
+/* Builds the static device schedule from SymbolicMF-shaped arrays (src/symbolic.jl:203-209)
+ * and A's pattern.  Replaces the prologue of multifrontal_lu (src/multifrontal.jl:150-197)
+ * and _factor_colptrs (src/multifrontal.jl:73-104). */
+int mf_plan_create(int64_t n, const int64_t *qf, int64_t nsuper, const int64_t *sstart,
+                   const int64_t *upd_ptr, const int64_t *upd_idx, const int64_t *child_ptr,
+                   const int64_t *child_idx, const int64_t *Ap, const int64_t *Ai,
+                   const mf_options *opt, mf_plan **out);
+
+/* Numeric factorization of one matrix with A's pattern: values Ax (nnz, CSC order of the
+ * pattern given to mf_plan_create), row scaling Rs (n; NULL = no scaling).  Replaces the
+ * supernode loop src/multifrontal.jl:198-357.  With check != 0 an exactly-zero block
+ * pivot returns MF_SINGULAR and *info_col = its column (1-based, in factor order, as
+ * SingularException(c1+info-1) at src/multifrontal.jl:250-251). */
+int mf_factor(mf_plan *P, const double *Ax, const double *Rs, int32_t check,
+              int64_t *info_col);
+/* Same with Ax and Rs already in device memory (HBM-resident inputs). */
+int mf_factor_device(mf_plan *P, const double *d_Ax, const double *d_Rs, int32_t check,
+                     int64_t *info_col);
+
+/* Same-pattern batch: values of nmat matrices (nmat x nnz, row-major by matrix) and their
+ * row scalings (nmat x n, or NULL).  nmat <= opt.nbatch.  info_col[m] = first singular
+ * column of matrix m (1-based) or 0.  No reference counterpart (the reference refactors
+ * one matrix per call, src/multifrontal.jl:146). */
+int mf_refactor_batch(mf_plan *P, int64_t nmat, const double *Ax_batch, const double *Rs_batch,
+                      int64_t *info_col);
+
+/* Solve A X = B for nrhs right-hand sides (B: n x nrhs, leading dimension ldb; X: ldx).
+ * refine_steps >= 0 fixed refinement steps as solve(F, b; refine=k)
+ * (src/interface.jl:170-177); with bwd_tol > 0 the loop stops early once the normwise
+ * backward error is <= bwd_tol (refine_steps is then the maximum).  rep may be NULL. */
+int mf_solve(mf_plan *P, int64_t nrhs, const double *B, int64_t ldb, double *X, int64_t ldx,
+             int32_t refine_steps, double bwd_tol, mf_solve_report *rep);
+/* Same with B and X in device memory. */
+int mf_solve_device(mf_plan *P, int64_t nrhs, const double *d_B, int64_t ldb, double *d_X,
+                    int64_t ldx, int32_t refine_steps, double bwd_tol, mf_solve_report *rep);
+/* Batched solve after mf_refactor_batch: matrix m solves B[m] (n x nrhs, contiguous). */
+int mf_solve_batch(mf_plan *P, int64_t nmat, int64_t nrhs, const double *B, double *X,
+                   int32_t refine_steps, double bwd_tol, mf_solve_report *rep);
+
+/* nnz(L) and nnz(U) of the factor (each = predicted_fill/2, src/symbolic.jl:230-237). */
+int mf_nnz(const mf_plan *P, int64_t *nnzL, int64_t *nnzU);
+
+/* Exports L, U (CSC, 1-based, exactly the reference's arrays: src/multifrontal.jl:277-350),
+ * p = qf[prow] and q = qf (src/multifrontal.jl:355).  Lnzval/Unzval are FP64 (converted
+ * from the front precision).  Any pointer may be NULL. */
+int mf_export_factors(mf_plan *P, int64_t *Lcolptr, int64_t *Lrowval, double *Lnzval,
+                      int64_t *Ucolptr, int64_t *Urowval, double *Unzval, int64_t *p,
+                      int64_t *q);
+
+/* Statistics of the plan: factor flops, Schur-gemm flops, flops in fronts with nf >= 1024,
+ * front storage bytes, number of levels, kernel launches per factorization. */
+typedef struct mf_plan_stats {
+    double factor_flops;
+    double gemm_flops;
+    double gemm_flops_big;   /* Schur gemm flops of fronts with nf >= 1024 */
+    double front_bytes;
+    int64_t nsuper;
+    int64_t nlevels;
+    int64_t launches;
+    int64_t max_nf;
+} mf_plan_stats;
+int mf_plan_get_stats(const mf_plan *P, mf_plan_stats *st);
+
+/* Per-kernel-class timing with HIP events on the plan's stream (instrumentation; no
+ * reference counterpart).  While profiling is on, mf_factor* and mf_solve* record an event
+ * pair around every launch and add the elapsed time to its class:
+ *   0 fused small fronts, 1 column assembly, 2 panel getrf, 3 panel-update gemm,
+ *   4 Schur gemm, 5 forward solve, 6 backward solve, 7 residual/norms/vector kernels.
+ * flops[k] / bytes[k] are the algorithmic flops and HBM bytes of class k accumulated over the
+ * same launches.  Profiling disables graph replay. */
+#define MF_PROF_CLASSES 8
+int mf_plan_set_profiling(mf_plan *P, int32_t on);
+int mf_plan_get_profile(const mf_plan *P, double *ms, int64_t *launches, double *flops, double *bytes);
+
+const char *mf_last_error(void);
+void mf_plan_destroy(mf_plan *P);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

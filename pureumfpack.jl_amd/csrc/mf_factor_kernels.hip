@@ -1,0 +1,466 @@
+// mf_factor_kernels.hip -- numeric factorization kernels (assembly, extend-add, getrf,
+// trsm, Schur gemm) for gfx950.  See mf_kernels.h for the front storage layout.
+#include "mf_kernels.h"
+
+namespace mf {
+
+// ------------------------------------------------------------------------------------------
+// Small fronts (nf <= small_nf): one workgroup per front, the whole front in LDS.
+// Steps: zero the front, place the (Rs.*A)[qf,qf] entries (src/multifrontal.jl:216-231),
+// extend-add the children's Schur blocks in reverse child order (:235-243, :45-57), then
+// right-looking elimination over all nf rows with the pivot search confined to the np pivot
+// rows (:247-252).  Eliminating the L21 rows and the R12 columns in the same sweep gives
+// L21 = A21 U11^-1, U12 = L11^-1 P A12 and F22 - L21 U12 (:254-268).
+// ------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_fused_small(FactorArgs a)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T *F = reinterpret_cast<T *>(smem_raw);
+    const int mat = blockIdx.y;
+    const SnDev d = a.sn[a.list[blockIdx.x]];
+    const int nf = d.nf, np = d.np;
+    int *orig = reinterpret_cast<int *>(F + nf * nf);
+    __shared__ int s_piv;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    T *pool = reinterpret_cast<T *>(a.pool) + mat * a.pool_stride;
+    const double *ax = a.ax + mat * a.ax_stride;
+    const double *rs = a.rs ? a.rs + mat * a.rs_stride : nullptr;
+
+    for (int i = tid; i < nf * nf; i += nt) F[i] = T(0);
+    for (int i = tid; i < np; i += nt) orig[i] = d.c1 + i;
+    __syncthreads();
+    for (int e = tid; e < d.norig; e += nt) {
+        int q = d.orig_off + e;
+        F[a.orig_row[q] + a.orig_col[q] * nf] = scaled_value<T>(ax, rs, a.arow, a.orig_src[q]);
+    }
+    __syncthreads();
+    for (int ci = 0; ci < d.nchild; ci++) {
+        const SnDev c = a.sn[a.child_list[d.child_off + ci]];
+        const int m = c.nu;
+        const T *cb = pool + c.foff + c.np + (int64_t)c.np * c.nf;
+        const int32_t *rm = a.rmap + c.upd_off;
+        for (int idx = tid; idx < m * m; idx += nt) {
+            int r = idx % m, s = idx / m;
+            F[rm[r] + rm[s] * nf] += cb[r + (int64_t)s * c.nf];
+        }
+        __syncthreads();
+    }
+
+    for (int k = 0; k < np; k++) {
+        if (tid < 64) {
+            T best = T(-1);
+            int bi = 0x7fffffff;
+            for (int r = k + tid; r < np; r += 64) {
+                T v = fabs(F[r + k * nf]);
+                if (v > best) { best = v; bi = r; }
+            }
+            wave_argmax(best, bi);
+            if (tid == 0) s_piv = bi;
+        }
+        __syncthreads();
+        const int p = s_piv;
+        if (p != k) {
+            for (int c = tid; c < nf; c += nt) {
+                T t = F[k + c * nf];
+                F[k + c * nf] = F[p + c * nf];
+                F[p + c * nf] = t;
+            }
+            if (tid == 0) { int t = orig[k]; orig[k] = orig[p]; orig[p] = t; }
+        }
+        __syncthreads();
+        const T piv = F[k + k * nf];
+        if (piv == T(0) && tid == 0) atomicMin(a.info + mat, d.c1 + k);
+        const T rcp = T(1) / piv;
+        const bool small_piv = fabs(piv) < Tiny<T>::v();
+        for (int r = k + 1 + tid; r < nf; r += nt) {
+            T v = F[r + k * nf];
+            if (r < np) {
+                // LAPACK getf2: scale by the reciprocal unless the pivot is zero or tiny
+                if (piv != T(0)) v = small_piv ? v / piv : v * rcp;
+            } else {
+                v = v * rcp;  // trsm R/U/N/N multiplies by 1/U(j,j)
+            }
+            F[r + k * nf] = v;
+        }
+        __syncthreads();
+        const int w = nf - k - 1;
+        for (int idx = tid; idx < w * w; idx += nt) {
+            int r = k + 1 + idx % w, c = k + 1 + idx / w;
+            F[r + c * nf] -= F[r + k * nf] * F[k + c * nf];
+        }
+        __syncthreads();
+    }
+    T *Fg = pool + d.foff;
+    for (int i = tid; i < nf * nf; i += nt) Fg[i] = F[i];
+    int32_t *prow = a.prow + mat * a.n_stride;
+    for (int i = tid; i < np; i += nt) prow[d.c1 + i] = orig[i];
+}
+
+// ------------------------------------------------------------------------------------------
+// Large fronts, step 1: assemble one front column per workgroup in LDS, then store it.
+// The column receives its original entries, then each contributing child column in reverse
+// child order (the extend-add of src/multifrontal.jl:45-57 split by parent column, so no two
+// workgroups ever write the same element and the result does not depend on scheduling).
+// ------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_asm_cols(FactorArgs a)
+{
+    extern __shared__ unsigned char smem_raw[];
+    T *col = reinterpret_cast<T *>(smem_raw);
+    const int mat = blockIdx.y;
+    const int2 w = reinterpret_cast<const int2 *>(a.list)[blockIdx.x];
+    const SnDev d = a.sn[w.x];
+    const int c = w.y, nf = d.nf;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    T *pool = reinterpret_cast<T *>(a.pool) + mat * a.pool_stride;
+    const double *ax = a.ax + mat * a.ax_stride;
+    const double *rs = a.rs ? a.rs + mat * a.rs_stride : nullptr;
+
+    for (int i = tid; i < nf; i += nt) col[i] = T(0);
+    __syncthreads();
+    const int fc = d.fcol + c;
+    for (int q = a.orig_colptr[fc] + tid; q < a.orig_colptr[fc + 1]; q += nt)
+        col[a.orig_row[q]] = scaled_value<T>(ax, rs, a.arow, a.orig_src[q]);
+    __syncthreads();
+    for (int q = a.ext_colptr[fc]; q < a.ext_colptr[fc + 1]; q++) {
+        const int2 e = a.ext[q];
+        const SnDev ch = a.sn[e.x];
+        const T *src = pool + ch.foff + ch.np + (int64_t)(ch.np + e.y) * ch.nf;
+        const int32_t *rm = a.rmap + ch.upd_off;
+        for (int r = tid; r < ch.nu; r += nt) col[rm[r]] += src[r];
+        __syncthreads();
+    }
+    T *dst = pool + d.foff + (int64_t)c * nf;
+    for (int i = tid; i < nf; i += nt) dst[i] = col[i];
+    if (c < d.np && tid == 0) a.prow[mat * a.n_stride + d.c1 + c] = d.c1 + c;
+}
+
+// ------------------------------------------------------------------------------------------
+// Large fronts, step 2: factor the panel of pivot columns [k, k+kb) over the candidate rows
+// [k, np) held in registers (RPT rows per thread), pivot by workgroup argmax (first maximum),
+// then
+//   * apply the kb row swaps to every column outside the panel (LAPACK dlaswp on A11 and the
+//     R12 swaps of src/multifrontal.jl:256-263),
+//   * U rows:  A[k:k+kb, c] <- L_kk^-1 A[k:k+kb, c] for c >= k+kb     (trsm L/L/N/U),
+//   * L21 cols: A[r, k:k+kb] <- A[r, k:k+kb] U_kk^-1 for r >= np        (trsm R/U/N/N).
+// The trailing update is a grouped gemm launched next.
+// ------------------------------------------------------------------------------------------
+template <typename T, int NT, int RPT>
+__global__ __launch_bounds__(NT) void k_panel(FactorArgs a, int k)
+{
+    constexpr int KB = PANEL_KB;
+    constexpr int NW = NT / 64;
+    const int mat = blockIdx.y;
+    const SnDev d = a.sn[a.list[blockIdx.x]];
+    const int np = d.np, nf = d.nf;
+    const int kb = min(KB, np - k);
+    const int m = np - k;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wid = tid >> 6;
+    T *F = reinterpret_cast<T *>(a.pool) + mat * a.pool_stride + d.foff;
+
+    __shared__ T s_redv[2][NW];
+    __shared__ int s_redi[2][NW];
+    __shared__ T s_prow[2][KB];
+    __shared__ T s_jrow[2][KB];
+    __shared__ T s_blk[KB * KB];
+    __shared__ int s_ipiv[KB];
+
+    T x[RPT][KB];
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+        const int i = tid + NT * r;
+#pragma unroll
+        for (int j = 0; j < KB; j++)
+            x[r][j] = (i < m && j < kb) ? F[(k + i) + (int64_t)(k + j) * nf] : T(0);
+    }
+
+#pragma unroll
+    for (int j = 0; j < KB; j++) {
+        if (j >= kb) continue;  // kb is uniform over the workgroup
+        const int par = j & 1;
+        T best = T(-1);
+        int bi = 0x7fffffff;
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+            const int i = tid + NT * r;
+            if (i >= j && i < m) {
+                T v = fabs(x[r][j]);
+                if (v > best) { best = v; bi = i; }
+            }
+        }
+        wave_argmax(best, bi);
+        if (lane == 0) { s_redv[par][wid] = best; s_redi[par][wid] = bi; }
+        __syncthreads();
+        best = s_redv[par][0];
+        bi = s_redi[par][0];
+#pragma unroll
+        for (int w2 = 1; w2 < NW; w2++) {
+            T ob = s_redv[par][w2];
+            int oi = s_redi[par][w2];
+            if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+        }
+        const int piv = bi;
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+            const int i = tid + NT * r;
+            if (i == piv) {
+#pragma unroll
+                for (int jj = 0; jj < KB; jj++) s_prow[par][jj] = x[r][jj];
+            }
+            if (i == j) {
+#pragma unroll
+                for (int jj = 0; jj < KB; jj++) s_jrow[par][jj] = x[r][jj];
+            }
+        }
+        __syncthreads();
+        if (piv != j) {
+#pragma unroll
+            for (int r = 0; r < RPT; r++) {
+                const int i = tid + NT * r;
+                if (i == j) {
+#pragma unroll
+                    for (int jj = 0; jj < KB; jj++) x[r][jj] = s_prow[par][jj];
+                }
+                if (i == piv) {
+#pragma unroll
+                    for (int jj = 0; jj < KB; jj++) x[r][jj] = s_jrow[par][jj];
+                }
+            }
+        }
+        const T pv = s_prow[par][j];
+        if (tid == 0) {
+            s_ipiv[j] = piv;
+            if (pv == T(0)) atomicMin(a.info + mat, d.c1 + k + j);
+        }
+        const T rcp = T(1) / pv;
+        const bool small_piv = fabs(pv) < Tiny<T>::v();
+#pragma unroll
+        for (int r = 0; r < RPT; r++) {
+            const int i = tid + NT * r;
+            if (i > j && i < m) {
+                T l = x[r][j];
+                if (pv != T(0)) l = small_piv ? l / pv : l * rcp;
+                x[r][j] = l;
+#pragma unroll
+                for (int jj = 0; jj < KB; jj++)
+                    if (jj > j) x[r][jj] -= l * s_prow[par][jj];
+            }
+        }
+    }
+
+    // store the panel, keep the kb x kb diagonal block in LDS
+#pragma unroll
+    for (int r = 0; r < RPT; r++) {
+        const int i = tid + NT * r;
+        if (i < m) {
+#pragma unroll
+            for (int j = 0; j < KB; j++)
+                if (j < kb) F[(k + i) + (int64_t)(k + j) * nf] = x[r][j];
+        }
+        if (i < kb) {
+#pragma unroll
+            for (int j = 0; j < KB; j++) s_blk[i + j * KB] = x[r][j];
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int32_t *prow = a.prow + mat * a.n_stride + d.c1;
+        for (int j = 0; j < kb; j++) {
+            int p = s_ipiv[j];
+            if (p != j) { int t = prow[k + j]; prow[k + j] = prow[k + p]; prow[k + p] = t; }
+        }
+    }
+    // swaps outside the panel, then the U-row solve for columns right of the panel
+    for (int c = tid; c < nf; c += NT) {
+        if (c >= k && c < k + kb) continue;
+        T *colp = F + (int64_t)c * nf + k;
+        for (int j = 0; j < kb; j++) {
+            int p = s_ipiv[j];
+            if (p != j) { T t = colp[j]; colp[j] = colp[p]; colp[p] = t; }
+        }
+        if (c >= k + kb) {
+            T y[KB];
+#pragma unroll
+            for (int j = 0; j < KB; j++) y[j] = j < kb ? colp[j] : T(0);
+#pragma unroll
+            for (int j = 0; j < KB; j++)
+#pragma unroll
+                for (int i = 0; i < KB; i++)
+                    if (i > j) y[i] -= s_blk[i + j * KB] * y[j];
+#pragma unroll
+            for (int j = 0; j < KB; j++)
+                if (j < kb) colp[j] = y[j];
+        }
+    }
+    // L21 columns of the panel: rows np..nf-1
+    for (int r = np + tid; r < nf; r += NT) {
+        T y[KB];
+#pragma unroll
+        for (int j = 0; j < KB; j++) y[j] = j < kb ? F[r + (int64_t)(k + j) * nf] : T(0);
+#pragma unroll
+        for (int j = 0; j < KB; j++) {
+            if (j < kb) {
+                T v = y[j];
+#pragma unroll
+                for (int i = 0; i < KB; i++)
+                    if (i < j) v -= y[i] * s_blk[i + j * KB];
+                y[j] = v * (T(1) / s_blk[j + j * KB]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < KB; j++)
+            if (j < kb) F[r + (int64_t)(k + j) * nf] = y[j];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Grouped gemm C -= A B on MFMA (v_mfma_*_16x16x4): 64x64 tile per 256-thread workgroup,
+// four waves in a 2x2 arrangement, each owning a 32x32 quadrant (2x2 MFMA blocks).  A and B
+// tiles are staged through LDS k-major so the MFMA operand reads are conflict free.
+// ------------------------------------------------------------------------------------------
+template <typename T> struct Mfma;
+// Output register q of lane l holds C[row(l, q)][l % 16].  The FP64 instruction interleaves the
+// four lane groups row by row (row = 4q + l/16); the FP32 one gives each lane group four
+// consecutive rows (row = 4(l/16) + q).
+template <> struct Mfma<double> {
+    typedef v4d acc_t;
+    static __device__ __forceinline__ acc_t mma(double x, double y, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int q) { return 4 * q + (lane >> 4); }
+};
+template <> struct Mfma<float> {
+    typedef v4f acc_t;
+    static __device__ __forceinline__ acc_t mma(float x, float y, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int q) { return 4 * (lane >> 4) + q; }
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs,
+                                              const GemmTile *__restrict__ tiles, void *pool_v,
+                                              int64_t pool_stride)
+{
+    typedef typename Mfma<T>::acc_t acc_t;
+    constexpr int BM = GEMM_BM, BN = GEMM_BN, BK = GEMM_BK;
+    __shared__ T As[BK][BM + 1];
+    __shared__ T Bs[BK][BN + 1];
+    const GemmTile tl = tiles[blockIdx.x];
+    const GemmProb pr = probs[tl.prob];
+    T *pool = reinterpret_cast<T *>(pool_v) + blockIdx.y * pool_stride;
+    const T *A = pool + pr.aoff;
+    const T *B = pool + pr.boff;
+    T *C = pool + pr.coff;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = (wid & 1) * 32, wn = (wid >> 1) * 32;
+    const int m0 = tl.tm, n0 = tl.tn;
+    acc_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    for (int k0 = 0; k0 < pr.k; k0 += BK) {
+        // A tile: BM x BK, thread -> row tid%64, k (tid/64) + 4i
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            int r = tid & 63, kk = (tid >> 6) + 4 * i;
+            int gr = m0 + r, gk = k0 + kk;
+            As[kk][r] = (gr < pr.m && gk < pr.k) ? A[gr + (int64_t)gk * pr.lda] : T(0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            int kk = tid & 15, cc = (tid >> 4) + 16 * i;
+            int gc = n0 + cc, gk = k0 + kk;
+            Bs[kk][cc] = (gc < pr.n && gk < pr.k) ? B[gk + (int64_t)gc * pr.ldb] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kq = 0; kq < BK; kq += 4) {
+            T av[2], bv[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) av[i] = As[kq + (lane >> 4)][wm + 16 * i + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < 2; j++) bv[j] = Bs[kq + (lane >> 4)][wn + 16 * j + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) acc[i][j] = Mfma<T>::mma(av[i], bv[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            int gc = n0 + wn + 16 * j + (lane & 15);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                int gr = m0 + wm + 16 * i + Mfma<T>::row(lane, q);
+                if (gr < pr.m && gc < pr.n) C[gr + (int64_t)gc * pr.ldc] -= acc[i][j][q];
+            }
+        }
+}
+
+// ------------------------------------------------------------------------------------------
+// Host launchers
+// ------------------------------------------------------------------------------------------
+template <typename T>
+hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int32_t *lists,
+                              const GemmProb *probs, const GemmTile *tiles, int nmat,
+                              hipStream_t st)
+{
+    FactorArgs a = base;
+    a.list = lists + L.list_off;
+    dim3 grid(L.count, nmat);
+    switch (L.kind) {
+    case L_FUSED_SMALL:
+        hipLaunchKernelGGL(k_fused_small<T>, grid, dim3(256), L.shmem, st, a);
+        break;
+    case L_ASM_COLS:
+        hipLaunchKernelGGL(k_asm_cols<T>, grid, dim3(256), L.shmem, st, a);
+        break;
+    case L_PANEL:
+        switch (L.variant) {
+        case 0: hipLaunchKernelGGL((k_panel<T, 256, 1>), grid, dim3(256), 0, st, a, L.k); break;
+        case 1: hipLaunchKernelGGL((k_panel<T, 256, 2>), grid, dim3(256), 0, st, a, L.k); break;
+        case 2: hipLaunchKernelGGL((k_panel<T, 256, 4>), grid, dim3(256), 0, st, a, L.k); break;
+        case 3: hipLaunchKernelGGL((k_panel<T, 1024, 2>), grid, dim3(1024), 0, st, a, L.k); break;
+        case 4: hipLaunchKernelGGL((k_panel<T, 1024, 3>), grid, dim3(1024), 0, st, a, L.k); break;
+        default: return hipErrorInvalidValue;
+        }
+        break;
+    case L_GEMM:
+        hipLaunchKernelGGL(k_gemm<T>, grid, dim3(256), 0, st, probs + L.prob_off,
+                           tiles + L.list_off, base.pool, base.pool_stride);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template hipError_t launch_factor_step<double>(const Launch &, const FactorArgs &, const int32_t *,
+                                               const GemmProb *, const GemmTile *, int, hipStream_t);
+template hipError_t launch_factor_step<float>(const Launch &, const FactorArgs &, const int32_t *,
+                                              const GemmProb *, const GemmTile *, int, hipStream_t);
+
+// rows that one panel variant holds
+int panel_variant_rows(int v)
+{
+    switch (v) {
+    case 0: return 256;
+    case 1: return 512;
+    case 2: return 1024;
+    case 3: return 2048;
+    case 4: return 3072;
+    default: return 0;
+    }
+}
+
+}  // namespace mf

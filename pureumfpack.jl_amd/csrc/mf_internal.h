@@ -1,0 +1,69 @@
+// mf_internal.h -- data structures shared by the host plan builder and the HIP kernels.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace mf {
+
+using i64 = int64_t;
+
+// Result of the host symbolic analysis (0-based), the arrays of SymbolicMF
+// (src/symbolic.jl:203-209) with colstruct kept only for each supernode's last column.
+struct Symbolic {
+    i64 n = 0;
+    std::vector<i64> qf, parent, sstart;
+    std::vector<std::vector<i64>> upd;       // update rows of each supernode (ascending)
+    std::vector<std::vector<i64>> children;  // child supernodes (ascending)
+    i64 ncolstruct = 0;                      // sum over columns of |colstruct[j]|
+};
+
+bool analyze(i64 n, const i64 *Ap, const i64 *Ai, int ordering, const i64 *q_in, Symbolic &out,
+             std::string &err);
+
+// Per-supernode record in device memory.
+struct SnDev {
+    i64 foff;         // element offset of the front (column-major, ld = nf) in one matrix's pool
+    int32_t c1, np, nu, nf;
+    int32_t upd_off;  // offset of this supernode's rows in upd[] / rmap[] / uoff[]
+    int32_t orig_off; // offset of this front's original entries (sorted by column)
+    int32_t norig;
+    int32_t child_off; // offset in child list (children stored in REVERSE order)
+    int32_t nchild;
+    int32_t fcol;     // index of this front's first column in the per-front-column arrays
+    int32_t pad;
+};
+
+// Grouped gemm problem: C -= A * B, column-major, offsets in elements into one matrix's pool.
+struct GemmProb {
+    i64 aoff, boff, coff;
+    int32_t m, n, k, lda, ldb, ldc;
+    int32_t pad[2];
+};
+
+// Grouped gemm tile: problem index and tile origin.
+struct GemmTile {
+    int32_t prob, tm, tn, pad;
+};
+
+enum LaunchKind : int32_t {
+    L_FUSED_SMALL = 0,  // one workgroup per small front: assemble + factor in LDS
+    L_ASM_COLS = 1,     // one workgroup per large-front column: assemble in LDS
+    L_PANEL = 2,        // one workgroup per large front: factor a panel of the pivot block
+    L_GEMM = 3,         // grouped C -= A B (panel updates and the Schur update)
+};
+
+struct Launch {
+    int32_t kind;
+    int32_t count;      // workgroups (fronts / columns / tiles)
+    int32_t list_off;   // offset into the int32 work-list array (fronts or columns)
+    int32_t k;          // panel start column (L_PANEL)
+    int32_t variant;    // kernel variant (panel rows-per-thread / LDS size selector)
+    int32_t prob_off;   // L_GEMM: offset into problems; tiles at list_off in tile array
+    int32_t shmem;      // dynamic LDS bytes
+    int32_t level;
+    double flops;       // algorithmic flops of the launch (host-side bookkeeping)
+    double bytes;       // algorithmic HBM bytes of the launch
+};
+
+}  // namespace mf

@@ -1,0 +1,112 @@
+// mf_kernels.h -- device kernels of the multifrontal numeric phase (gfx950, wave64).
+//
+// Front storage: every supernode s owns a dense nf x nf column-major block at pool + foff(s)
+// (ld = nf) for the whole life of the plan; rows/cols are [pivot rows c1..c2 | update rows],
+// the pre-pivoting order of src/multifrontal.jl:206-211.  After factorization the block holds
+// L11\U11 (rows permuted by the in-block pivoting), L21, U12 and the Schur complement F22,
+// which is the contribution block its parent reads directly (no CB copy, no LIFO arena).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "mf_internal.h"
+
+namespace mf {
+
+constexpr int PANEL_KB = 16;   // panel width of the large-front getrf
+constexpr int GEMM_BM = 64, GEMM_BN = 64, GEMM_BK = 16;
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Tiny;
+template <> struct Tiny<double> { static __device__ __forceinline__ double v() { return DBL_MIN; } };
+template <> struct Tiny<float> { static __device__ __forceinline__ float v() { return FLT_MIN; } };
+
+// Value of an original entry: (Rs .* A)[i, j] as apply_row_scaling forms it
+// (src/scaling.jl:118-126): one FP64 product, then rounded to the front type.
+template <typename T>
+__device__ __forceinline__ T scaled_value(const double *ax, const double *rs, const int32_t *arow, int src)
+{
+    double v = ax[src];
+    if (rs) v *= rs[arow[src]];
+    return (T)v;
+}
+
+// Wave-level argmax of |v| with the first (smallest) row index winning ties, the rule of
+// BLAS idamax that LAPACK getrf uses (src/multifrontal.jl:248 lu!(A11, RowMaximum())).
+template <typename T>
+__device__ __forceinline__ void wave_argmax(T &best, int &bi)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        T ob = __shfl_xor(best, off);
+        int oi = __shfl_xor(bi, off);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+    }
+}
+
+struct FactorArgs {
+    const SnDev *sn;
+    const int32_t *list;        // fronts of this launch (or columns as int2 for L_ASM_COLS)
+    const int32_t *orig_row, *orig_col, *orig_src, *orig_colptr;
+    const int32_t *child_list, *rmap;
+    const int32_t *ext_colptr;
+    const int2 *ext;
+    const int32_t *arow;
+    const double *ax;
+    const double *rs;
+    void *pool;
+    int32_t *prow;
+    int32_t *info;
+    int64_t pool_stride, ax_stride, rs_stride, n_stride;
+};
+
+struct SolveArgs {
+    const SnDev *sn;
+    const int32_t *list;
+    const int32_t *child_list, *rmap, *upd;
+    const void *pool;
+    const int32_t *prow;
+    void *w0;      // permuted, scaled RHS in V-row order (n x nrhs)
+    void *y;       // forward result in factor-row order (n x nrhs)
+    void *x;       // backward result in factor-column order (n x nrhs)
+    void *uv;      // update vectors (nupd x nrhs)
+    int64_t n, nupd, nrhs;
+    int64_t pool_stride, n_stride;
+};
+
+// ---- launchers (mf_factor_kernels.hip, mf_solve_kernels.hip) ----
+template <typename T>
+hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int32_t *lists,
+                              const GemmProb *probs, const GemmTile *tiles, int nmat, hipStream_t st);
+int panel_variant_rows(int v);
+template <typename T>
+hipError_t launch_fwd(const SolveArgs &a, const int32_t *lists, int list_off, int count, int shmem, int nmat,
+                      hipStream_t st);
+template <typename T>
+hipError_t launch_bwd(const SolveArgs &a, const int32_t *lists, int list_off, int count, int shmem, int nmat,
+                      hipStream_t st);
+template <typename T>
+hipError_t launch_prep_rhs(const double *B, int64_t ldb, const double *rs, int64_t rs_stride, const int32_t *qf,
+                           T *w0, int64_t n, int64_t nrhs, int64_t b_mat_stride, int nmat, hipStream_t st);
+template <typename T>
+hipError_t launch_finish_x(const T *x, const int32_t *qf, double *X, int64_t ldx, int64_t n, int64_t nrhs,
+                           int accumulate, int64_t x_mat_stride, int nmat, hipStream_t st);
+hipError_t launch_residual(const int32_t *rp, const int32_t *ci, const int32_t *vmap, const double *ax,
+                           int64_t ax_stride, const double *X, int64_t ldx, const double *B, int64_t ldb, double *R,
+                           int64_t n, int64_t nrhs, int64_t msx, int64_t msb, int nmat, hipStream_t st);
+hipError_t launch_norms(const double *R, const double *X, int64_t ldx, const double *B, int64_t ldb, int64_t n,
+                        int64_t nrhs, int64_t msx, int64_t msb, double *out, int nmat, hipStream_t st);
+hipError_t launch_anorm(const int32_t *rp, const int32_t *vmap, const double *ax, int64_t ax_stride, int64_t n,
+                        double *out, int nmat, hipStream_t st);
+hipError_t launch_rowfac(const int32_t *prow, int32_t *rowfac, int64_t n, hipStream_t st);
+template <typename T>
+hipError_t launch_export(const SnDev *sn, int64_t nsuper, const T *pool, const int32_t *upd, const int32_t *rowfac,
+                         const int64_t *Lp, const int64_t *Up, const int64_t *uoff, int64_t *Li, double *Lx,
+                         int64_t *Ui, double *Ux, hipStream_t st);
+hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st);
+
+}  // namespace mf

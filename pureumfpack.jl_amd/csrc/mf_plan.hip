@@ -1,0 +1,1131 @@
+// mf_plan.hip -- plan construction (the static device schedule) and the C ABI of mfgpu.h.
+//
+// The symbolic structure fixes every size of the numeric phase: front shapes, where each
+// original entry and each child column lands, the L/U CSC offsets, the etree levels.  The
+// plan turns it, once per pattern, into device arrays and a flat list of kernel launches
+// (level by level: small fronts fused in LDS, large fronts as column assembly -> panel
+// getrf -> grouped gemm updates -> Schur gemm).  mf_factor only replays that list.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/mfgpu.h"
+#include "mf_internal.h"
+#include "mf_kernels.h"
+
+using namespace mf;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) {                                                                   \
+            return fail(e_ == hipErrorOutOfMemory ? MF_OOM : MF_HIP,                              \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                       \
+        }                                                                                         \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t b)
+    {
+        if (p && bytes >= b) return hipSuccess;
+        if (p) { (void)hipFree(p); p = nullptr; bytes = 0; }
+        hipError_t e = hipMalloc(&p, b > 0 ? b : 16);
+        if (e == hipSuccess) bytes = b;
+        return e;
+    }
+    template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+template <typename T>
+static hipError_t upload(DevBuf &b, const std::vector<T> &v)
+{
+    hipError_t e = b.alloc(v.size() * sizeof(T));
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+struct mf_symbolic {
+    Symbolic S;
+};
+
+struct mf_plan {
+    // options
+    int prec = MF_FP64;
+    int small_nf = 64;
+    int use_graph = 0;
+    int nbatch = 1;
+    // sizes
+    int64_t n = 0, nnz = 0, nsuper = 0, nupd = 0, nlevels = 0, maxnf = 0;
+    int64_t pool_elems = 0;
+    int64_t nnzL = 0, nnzU = 0;
+    double flops = 0, gemm_flops = 0, gemm_flops_big = 0;
+    // host copies
+    std::vector<int64_t> qf, Lp, Up, uoff;
+    std::vector<SnDev> sn;
+    std::vector<Launch> launches;
+    std::vector<int32_t> lvl_off, lvl_cnt;  // solve lists per level
+    std::vector<int32_t> lvl_shfwd, lvl_shbwd;
+    std::vector<double> lvl_lflops, lvl_uflops, lvl_lbytes, lvl_ubytes, lvl_vec;
+    // device
+    DevBuf d_sn, d_upd, d_rmap, d_child, d_orow, d_ocol, d_osrc, d_ocolptr, d_ecolptr, d_ext, d_arow;
+    DevBuf d_lists, d_probs, d_tiles, d_solvelist, d_qf;
+    DevBuf d_csr_rp, d_csr_ci, d_csr_vmap;
+    DevBuf d_pool, d_prow, d_info, d_ax, d_rs;
+    DevBuf d_w0, d_y, d_x, d_uv, d_B, d_X, d_R, d_norms, d_anorm;
+    DevBuf d_Lp, d_Up, d_uoff, d_Li, d_Lx, d_Ui, d_Ux, d_rowfac;
+    int64_t solve_nrhs = 0;
+    bool factored = false;
+    int nmat_factored = 0;
+    int rs_given = 0;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_nmat = 0;
+    int graph_rs = 0;
+    // profiling (mf_plan_set_profiling)
+    int prof = 0;
+    double prof_ms[MF_PROF_CLASSES] = {0}, prof_flops[MF_PROF_CLASSES] = {0}, prof_bytes[MF_PROF_CLASSES] = {0};
+    int64_t prof_n[MF_PROF_CLASSES] = {0};
+    std::vector<hipEvent_t> ev;
+    struct EvRec { int cls; size_t a, b; double flops, bytes; };
+    std::vector<EvRec> evrec;
+    size_t evused = 0;
+
+    ~mf_plan()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    size_t esize() const { return prec == MF_FP64 ? 8 : 4; }
+
+    hipError_t ev_mark(size_t &idx)
+    {
+        if (evused == ev.size()) {
+            hipEvent_t e;
+            hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+            ev.push_back(e);
+        }
+        idx = evused++;
+        return hipEventRecord(ev[idx], stream);
+    }
+    // collect the recorded pairs after the stream has been synchronised
+    hipError_t ev_collect()
+    {
+        for (const EvRec &r : evrec) {
+            float ms = 0;
+            hipError_t e = hipEventElapsedTime(&ms, ev[r.a], ev[r.b]);
+            if (e != hipSuccess) return e;
+            prof_ms[r.cls] += ms;
+            prof_n[r.cls] += 1;
+            prof_flops[r.cls] += r.flops;
+            prof_bytes[r.cls] += r.bytes;
+        }
+        evrec.clear();
+        evused = 0;
+        return hipSuccess;
+    }
+};
+
+// Runs `launch` between two profiling events when profiling is on.
+template <typename F>
+static hipError_t profiled(mf_plan &P, int cls, double flops, double bytes, F &&launch)
+{
+    if (!P.prof) return launch();
+    size_t a, b;
+    hipError_t e = P.ev_mark(a);
+    if (e != hipSuccess) return e;
+    e = launch();
+    if (e != hipSuccess) return e;
+    e = P.ev_mark(b);
+    if (e != hipSuccess) return e;
+    P.evrec.push_back({cls, a, b, flops, bytes});
+    return hipSuccess;
+}
+
+static int launch_class(const Launch &L)
+{
+    switch (L.kind) {
+    case L_FUSED_SMALL: return 0;
+    case L_ASM_COLS: return 1;
+    case L_PANEL: return 2;
+    default: return L.variant == 1 ? 4 : 3;
+    }
+}
+
+extern "C" const char *mf_last_error(void) { return g_err.c_str(); }
+
+// ------------------------------------------------------------------ symbolic API
+extern "C" int mf_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, int32_t ordering, const int64_t *q,
+                          mf_symbolic **out)
+{
+    if (!out) return fail(MF_BADARG, "out is NULL");
+    *out = nullptr;
+    if (n < 0 || !Ap || (n > 0 && !Ai)) return fail(MF_BADARG, "bad pattern arguments");
+    if (ordering < 0 || ordering > 2) return fail(MF_BADARG, "unknown ordering");
+    for (int64_t j = 0; j < n; j++)
+        if (Ap[j + 1] < Ap[j]) return fail(MF_BADARG, "column pointers decrease");
+    for (int64_t k = 0; k < Ap[n] - 1; k++)
+        if (Ai[k] < 1 || Ai[k] > n) return fail(MF_BADARG, "row index out of range");
+    std::unique_ptr<mf_symbolic> s(new (std::nothrow) mf_symbolic);
+    if (!s) return fail(MF_OOM, "host allocation failed");
+    std::string err;
+    try {
+        if (!analyze(n, Ap, Ai, ordering, q, s->S, err)) return fail(MF_BADARG, err);
+    } catch (const std::bad_alloc &) {
+        return fail(MF_OOM, "host allocation failed in analysis");
+    }
+    *out = s.release();
+    return MF_OK;
+}
+
+extern "C" int mf_symbolic_sizes(const mf_symbolic *S, int64_t *nsuper, int64_t *nupd, int64_t *nchild,
+                                 int64_t *ncolstruct)
+{
+    if (!S) return fail(MF_BADARG, "NULL symbolic");
+    int64_t ns = (int64_t)S->S.sstart.size() - 1, nu = 0, nc = 0;
+    for (auto &u : S->S.upd) nu += (int64_t)u.size();
+    for (auto &c : S->S.children) nc += (int64_t)c.size();
+    if (nsuper) *nsuper = ns;
+    if (nupd) *nupd = nu;
+    if (nchild) *nchild = nc;
+    if (ncolstruct) *ncolstruct = S->S.ncolstruct;
+    return MF_OK;
+}
+
+extern "C" int mf_symbolic_export(const mf_symbolic *S, int64_t *qf, int64_t *parent, int64_t *sstart,
+                                  int64_t *upd_ptr, int64_t *upd_idx, int64_t *child_ptr, int64_t *child_idx)
+{
+    if (!S) return fail(MF_BADARG, "NULL symbolic");
+    const Symbolic &s = S->S;
+    int64_t n = s.n, ns = (int64_t)s.sstart.size() - 1;
+    if (qf)
+        for (int64_t j = 0; j < n; j++) qf[j] = s.qf[j] + 1;
+    if (parent)
+        for (int64_t j = 0; j < n; j++) parent[j] = s.parent[j] + 1;
+    if (sstart)
+        for (int64_t k = 0; k <= ns; k++) sstart[k] = s.sstart[k] + 1;
+    int64_t o = 0;
+    for (int64_t k = 0; k < ns; k++) {
+        if (upd_ptr) upd_ptr[k] = o + 1;
+        for (int64_t v : s.upd[k]) {
+            if (upd_idx) upd_idx[o] = v + 1;
+            o++;
+        }
+    }
+    if (upd_ptr) upd_ptr[ns] = o + 1;
+    o = 0;
+    for (int64_t k = 0; k < ns; k++) {
+        if (child_ptr) child_ptr[k] = o + 1;
+        for (int64_t v : s.children[k]) {
+            if (child_idx) child_idx[o] = v + 1;
+            o++;
+        }
+    }
+    if (child_ptr) child_ptr[ns] = o + 1;
+    return MF_OK;
+}
+
+extern "C" void mf_symbolic_destroy(mf_symbolic *S) { delete S; }
+
+// ------------------------------------------------------------------ plan
+static int choose_panel_variant(int rows)
+{
+    for (int v = 0; v < 5; v++)
+        if (panel_variant_rows(v) >= rows) return v;
+    return -1;
+}
+
+static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper, const int64_t *sstart1,
+                      const int64_t *upd_ptr1, const int64_t *upd_idx1, const int64_t *child_ptr1,
+                      const int64_t *child_idx1, const int64_t *Ap1, const int64_t *Ai1)
+{
+    P.n = n;
+    P.nsuper = nsuper;
+    P.nnz = Ap1[n] - 1;
+    if (n >= INT32_MAX / 2 || P.nnz >= INT32_MAX) return fail(MF_BADARG, "matrix too large for int32 indexing");
+    // ---- validate and convert
+    P.qf.resize(n);
+    {
+        std::vector<char> seen(n, 0);
+        for (int64_t j = 0; j < n; j++) {
+            int64_t v = qf1[j] - 1;
+            if (v < 0 || v >= n || seen[v]) return fail(MF_BADARG, "qf is not a permutation");
+            seen[v] = 1;
+            P.qf[j] = v;
+        }
+    }
+    if (sstart1[0] != 1 || sstart1[nsuper] != n + 1) return fail(MF_BADARG, "sstart must span 1..n+1");
+    std::vector<int64_t> c1(nsuper), np(nsuper), nu(nsuper), nf(nsuper), uoff0(nsuper + 1, 0);
+    std::vector<int64_t> super_of(n);
+    for (int64_t s = 0; s < nsuper; s++) {
+        c1[s] = sstart1[s] - 1;
+        np[s] = sstart1[s + 1] - sstart1[s];
+        if (np[s] <= 0) return fail(MF_BADARG, "empty supernode");
+        nu[s] = upd_ptr1[s + 1] - upd_ptr1[s];
+        if (nu[s] < 0) return fail(MF_BADARG, "bad upd_ptr");
+        nf[s] = np[s] + nu[s];
+        uoff0[s + 1] = uoff0[s] + nu[s];
+        for (int64_t j = c1[s]; j < c1[s] + np[s]; j++) super_of[j] = s;
+    }
+    P.nupd = uoff0[nsuper];
+    std::vector<int32_t> upd(P.nupd);
+    for (int64_t s = 0; s < nsuper; s++) {
+        int64_t last = c1[s] + np[s] - 1;
+        for (int64_t t = 0; t < nu[s]; t++) {
+            int64_t v = upd_idx1[upd_ptr1[s] - 1 + t] - 1;
+            if (v <= last || v >= n || (t > 0 && v <= upd[uoff0[s] + t - 1]))
+                return fail(MF_BADARG, "update rows must be ascending and beyond the supernode");
+            upd[uoff0[s] + t] = (int32_t)v;
+        }
+    }
+    std::vector<int64_t> parent_sn(nsuper, -1);
+    std::vector<std::vector<int64_t>> kids(nsuper);
+    for (int64_t s = 0; s < nsuper; s++)
+        for (int64_t q = child_ptr1[s] - 1; q < child_ptr1[s + 1] - 1; q++) {
+            int64_t c = child_idx1[q] - 1;
+            if (c < 0 || c >= s || parent_sn[c] != -1) return fail(MF_BADARG, "bad child list");
+            parent_sn[c] = s;
+            kids[s].push_back(c);
+        }
+    // levels
+    std::vector<int64_t> level(nsuper, 0);
+    int64_t nlev = 0;
+    for (int64_t s = 0; s < nsuper; s++) {
+        for (int64_t c : kids[s]) level[s] = std::max(level[s], level[c] + 1);
+        nlev = std::max(nlev, level[s] + 1);
+    }
+    P.nlevels = nlev;
+    // rmap: position of each child update row in the parent front
+    std::vector<int32_t> loc(n, -1), rmap(P.nupd);
+    for (int64_t s = 0; s < nsuper; s++) {
+        for (int64_t a = 0; a < np[s]; a++) loc[c1[s] + a] = (int32_t)a;
+        for (int64_t t = 0; t < nu[s]; t++) loc[upd[uoff0[s] + t]] = (int32_t)(np[s] + t);
+        for (int64_t c : kids[s])
+            for (int64_t t = 0; t < nu[c]; t++) {
+                int32_t l = loc[upd[uoff0[c] + t]];
+                if (l < 0) return fail(MF_BADARG, "child update row missing from parent front");
+                rmap[uoff0[c] + t] = l;
+            }
+        for (int64_t a = 0; a < np[s]; a++) loc[c1[s] + a] = -1;
+        for (int64_t t = 0; t < nu[s]; t++) loc[upd[uoff0[s] + t]] = -1;
+    }
+    // front offsets and per-front column bases
+    std::vector<int64_t> foff(nsuper), fcol(nsuper + 1, 0);
+    int64_t pe = 0;
+    for (int64_t s = 0; s < nsuper; s++) {
+        foff[s] = pe;
+        pe += nf[s] * nf[s];
+        fcol[s + 1] = fcol[s] + nf[s];
+        P.maxnf = std::max(P.maxnf, nf[s]);
+    }
+    P.pool_elems = pe;
+    if (fcol[nsuper] >= INT32_MAX) return fail(MF_BADARG, "too many front columns");
+    // original entries: V = As[qf,qf]; entry (iV, jV) goes to the front of column min(iV,jV)
+    // (src/multifrontal.jl:216-231)
+    std::vector<int64_t> qinv(n);
+    for (int64_t j = 0; j < n; j++) qinv[P.qf[j]] = j;
+    std::vector<int64_t> ocount(nsuper + 1, 0);
+    struct OE { int32_t row, col, src; };
+    std::vector<OE> oe(P.nnz);
+    std::vector<int32_t> ofront(P.nnz);
+    auto front_pos = [&](int64_t s, int64_t g) -> int64_t {
+        if (g >= c1[s] && g < c1[s] + np[s]) return g - c1[s];
+        const int32_t *b = upd.data() + uoff0[s], *e = b + nu[s];
+        const int32_t *it = std::lower_bound(b, e, (int32_t)g);
+        if (it == e || *it != g) return -1;
+        return np[s] + (it - b);
+    };
+    for (int64_t jA = 0; jA < n; jA++) {
+        for (int64_t p = Ap1[jA] - 1; p < Ap1[jA + 1] - 1; p++) {
+            int64_t iA = Ai1[p] - 1;
+            if (iA < 0 || iA >= n) return fail(MF_BADARG, "row index out of range");
+            int64_t iV = qinv[iA], jV = qinv[jA];
+            int64_t s, lr, lc;
+            if (iV >= jV) {
+                s = super_of[jV];
+                lr = front_pos(s, iV);
+                lc = jV - c1[s];
+            } else {
+                s = super_of[iV];
+                lr = iV - c1[s];
+                lc = front_pos(s, jV);
+            }
+            if (lr < 0 || lc < 0) return fail(MF_BADARG, "entry outside the symbolic structure");
+            oe[p] = {(int32_t)lr, (int32_t)lc, (int32_t)p};
+            ofront[p] = (int32_t)s;
+            ocount[s + 1]++;
+        }
+    }
+    for (int64_t s = 0; s < nsuper; s++) ocount[s + 1] += ocount[s];
+    std::vector<OE> oes(P.nnz);
+    {
+        std::vector<int64_t> cur(ocount.begin(), ocount.end() - 1);
+        for (int64_t p = 0; p < P.nnz; p++) oes[cur[ofront[p]]++] = oe[p];
+    }
+    std::vector<int32_t> orow(P.nnz), ocol(P.nnz), osrc(P.nnz), ocolptr(fcol[nsuper] + 1, 0);
+    for (int64_t s = 0; s < nsuper; s++) {
+        std::sort(oes.begin() + ocount[s], oes.begin() + ocount[s + 1], [](const OE &a, const OE &b) {
+            return a.col != b.col ? a.col < b.col : a.row < b.row;
+        });
+        for (int64_t q = ocount[s]; q < ocount[s + 1]; q++) {
+            orow[q] = oes[q].row;
+            ocol[q] = oes[q].col;
+            osrc[q] = oes[q].src;
+            ocolptr[fcol[s] + oes[q].col + 1]++;
+        }
+    }
+    // ocolptr: entries of front s are contiguous from ocount[s]; make per-column pointers global
+    for (int64_t s = 0; s < nsuper; s++) {
+        int64_t base = ocount[s];
+        int64_t run = base;
+        for (int64_t c = 0; c < nf[s]; c++) {
+            int64_t cnt = ocolptr[fcol[s] + c + 1];
+            ocolptr[fcol[s] + c] = (int32_t)run;
+            run += cnt;
+        }
+    }
+    ocolptr[fcol[nsuper]] = (int32_t)P.nnz;
+    // child contributions per parent column, reverse child order within a column
+    std::vector<int32_t> ecolptr(fcol[nsuper] + 1, 0);
+    for (int64_t s = 0; s < nsuper; s++)
+        for (int64_t c : kids[s])
+            for (int64_t t = 0; t < nu[c]; t++) ecolptr[fcol[s] + rmap[uoff0[c] + t] + 1]++;
+    for (int64_t q = 0; q < fcol[nsuper]; q++) ecolptr[q + 1] += ecolptr[q];
+    std::vector<int2> ext(ecolptr[fcol[nsuper]]);
+    {
+        std::vector<int32_t> cur(ecolptr.begin(), ecolptr.end() - 1);
+        for (int64_t s = 0; s < nsuper; s++)
+            for (auto it = kids[s].rbegin(); it != kids[s].rend(); ++it) {
+                int64_t c = *it;
+                for (int64_t t = 0; t < nu[c]; t++) {
+                    int64_t col = fcol[s] + rmap[uoff0[c] + t];
+                    ext[cur[col]++] = make_int2((int)c, (int)t);
+                }
+            }
+    }
+    // children in reverse order (the order src/multifrontal.jl:236 absorbs them)
+    std::vector<int32_t> child_rev;
+    std::vector<int64_t> child_off(nsuper);
+    for (int64_t s = 0; s < nsuper; s++) {
+        child_off[s] = (int64_t)child_rev.size();
+        for (auto it = kids[s].rbegin(); it != kids[s].rend(); ++it) child_rev.push_back((int32_t)*it);
+    }
+    // per-supernode device records
+    P.sn.resize(nsuper);
+    for (int64_t s = 0; s < nsuper; s++) {
+        SnDev &d = P.sn[s];
+        d.foff = foff[s];
+        d.c1 = (int32_t)c1[s];
+        d.np = (int32_t)np[s];
+        d.nu = (int32_t)nu[s];
+        d.nf = (int32_t)nf[s];
+        d.upd_off = (int32_t)uoff0[s];
+        d.orig_off = (int32_t)ocount[s];
+        d.norig = (int32_t)(ocount[s + 1] - ocount[s]);
+        d.child_off = (int32_t)child_off[s];
+        d.nchild = (int32_t)kids[s].size();
+        d.fcol = (int32_t)fcol[s];
+        d.pad = 0;
+    }
+    // flops
+    for (int64_t s = 0; s < nsuper; s++) {
+        double p = (double)np[s], u = (double)nu[s];
+        double getrf = 0;
+        for (int64_t k = 1; k <= np[s]; k++) getrf += (double)(np[s] - k) + 2.0 * (double)(np[s] - k) * (double)(np[s] - k);
+        double g = 2.0 * p * u * u;
+        P.flops += getrf + u * p * p + u * p * (p - 1) + g;
+        P.gemm_flops += g;
+        if (nf[s] >= 1024) P.gemm_flops_big += g;
+    }
+    // ---- launch schedule
+    std::vector<std::vector<int64_t>> bylevel(nlev);
+    for (int64_t s = 0; s < nsuper; s++) bylevel[level[s]].push_back(s);
+    std::vector<int32_t> lists;
+    std::vector<GemmProb> probs;
+    std::vector<GemmTile> tiles;
+    const double ebytes = (double)P.esize();
+    auto add_gemm = [&](std::vector<GemmProb> &pv, int lvl, int schur) {
+        if (pv.empty()) return;
+        Launch L{};
+        L.kind = L_GEMM;
+        L.variant = schur;
+        for (const GemmProb &g : pv) {
+            L.flops += 2.0 * g.m * (double)g.n * g.k;
+            L.bytes += ((double)g.m * g.k + (double)g.k * g.n + 2.0 * g.m * (double)g.n) * ebytes;
+        }
+        L.prob_off = (int32_t)probs.size();
+        L.list_off = (int32_t)tiles.size();
+        for (size_t q = 0; q < pv.size(); q++) {
+            const GemmProb &g = pv[q];
+            for (int tn = 0; tn < g.n; tn += GEMM_BN)
+                for (int tm = 0; tm < g.m; tm += GEMM_BM) tiles.push_back({(int32_t)q, tm, tn, 0});
+            probs.push_back(g);
+        }
+        L.count = (int32_t)(tiles.size() - L.list_off);
+        L.level = lvl;
+        if (L.count > 0) P.launches.push_back(L);
+    };
+    const int elem = (int)P.esize();
+    // algorithmic bytes of assembling front s: the front written once, every child
+    // contribution block read once, each original entry (value, scale, 3 int32 indices)
+    auto asm_bytes = [&](int64_t s) {
+        double b = (double)nf[s] * nf[s] * elem + 28.0 * (double)(ocount[s + 1] - ocount[s]);
+        for (int64_t c : kids[s]) b += (double)nu[c] * nu[c] * elem + 4.0 * nu[c];
+        return b;
+    };
+    auto front_flops = [&](int64_t s) {
+        double p = (double)np[s], u = (double)nu[s], g = 0;
+        for (int64_t k = 1; k <= np[s]; k++) g += (double)(np[s] - k) + 2.0 * (double)(np[s] - k) * (double)(np[s] - k);
+        return g + u * p * p + u * p * (p - 1) + 2.0 * p * u * u;
+    };
+    for (int64_t lv = 0; lv < nlev; lv++) {
+        std::vector<int64_t> small, large;
+        for (int64_t s : bylevel[lv]) (nf[s] <= P.small_nf ? small : large).push_back(s);
+        if (!large.empty()) {
+            // column assembly (int2 entries: keep 8-byte alignment)
+            if (lists.size() & 1) lists.push_back(0);
+            Launch L{};
+            L.kind = L_ASM_COLS;
+            L.list_off = (int32_t)lists.size();
+            int64_t mnf = 0;
+            for (int64_t s : large) {
+                for (int64_t c = 0; c < nf[s]; c++) { lists.push_back((int32_t)s); lists.push_back((int32_t)c); }
+                mnf = std::max(mnf, nf[s]);
+                L.bytes += asm_bytes(s);
+            }
+            L.count = (int32_t)((lists.size() - L.list_off) / 2);
+            L.shmem = (int32_t)(mnf * elem);
+            L.level = (int32_t)lv;
+            if (L.shmem > 160 * 1024) return fail(MF_BADARG, "front too large for column assembly");
+            P.launches.push_back(L);
+            int64_t maxnp = 0;
+            for (int64_t s : large) maxnp = std::max(maxnp, np[s]);
+            for (int64_t k = 0; k < maxnp; k += PANEL_KB) {
+                Launch Pn{};
+                Pn.kind = L_PANEL;
+                Pn.k = (int32_t)k;
+                Pn.list_off = (int32_t)lists.size();
+                int64_t rows = 0;
+                std::vector<GemmProb> pv;
+                for (int64_t s : large) {
+                    if (np[s] <= k) continue;
+                    lists.push_back((int32_t)s);
+                    rows = std::max(rows, np[s] - k);
+                    int64_t kb = std::min<int64_t>(PANEL_KB, np[s] - k), k2 = k + kb;
+                    {
+                        double m = (double)(np[s] - k), b = (double)kb, right = (double)(nf[s] - k2);
+                        for (int64_t j = 0; j < kb; j++)
+                            Pn.flops += (m - j - 1) + 2.0 * (m - j - 1) * (b - j - 1);
+                        Pn.flops += right * b * (b - 1) + (double)nu[s] * b * b;
+                        Pn.bytes += (2.0 * m * b + 2.0 * right * b + 2.0 * (double)nu[s] * b) * elem;
+                    }
+                    if (k2 < np[s]) {
+                        // rows [k2, nf) x cols [k2, np): pivot-column block incl. L21 rows
+                        GemmProb g{};
+                        g.aoff = foff[s] + k2 + k * nf[s];
+                        g.boff = foff[s] + k + k2 * nf[s];
+                        g.coff = foff[s] + k2 + k2 * nf[s];
+                        g.m = (int32_t)(nf[s] - k2);
+                        g.n = (int32_t)(np[s] - k2);
+                        g.k = (int32_t)kb;
+                        g.lda = g.ldb = g.ldc = (int32_t)nf[s];
+                        pv.push_back(g);
+                        if (nu[s] > 0) {  // rows [k2, np) x cols [np, nf): the U12 block
+                            GemmProb h{};
+                            h.aoff = foff[s] + k2 + k * nf[s];
+                            h.boff = foff[s] + k + np[s] * nf[s];
+                            h.coff = foff[s] + k2 + np[s] * nf[s];
+                            h.m = (int32_t)(np[s] - k2);
+                            h.n = (int32_t)nu[s];
+                            h.k = (int32_t)kb;
+                            h.lda = h.ldb = h.ldc = (int32_t)nf[s];
+                            pv.push_back(h);
+                        }
+                    }
+                }
+                Pn.count = (int32_t)(lists.size() - Pn.list_off);
+                Pn.variant = choose_panel_variant((int)rows);
+                if (Pn.variant < 0) return fail(MF_BADARG, "pivot block too large for the panel kernel");
+                Pn.level = (int32_t)lv;
+                P.launches.push_back(Pn);
+                add_gemm(pv, (int)lv, 0);
+            }
+            std::vector<GemmProb> sv;
+            for (int64_t s : large) {
+                if (nu[s] == 0) continue;
+                GemmProb g{};
+                g.aoff = foff[s] + np[s];
+                g.boff = foff[s] + np[s] * nf[s];
+                g.coff = foff[s] + np[s] + np[s] * nf[s];
+                g.m = g.n = (int32_t)nu[s];
+                g.k = (int32_t)np[s];
+                g.lda = g.ldb = g.ldc = (int32_t)nf[s];
+                sv.push_back(g);
+            }
+            add_gemm(sv, (int)lv, 1);
+        }
+        if (!small.empty()) {
+            Launch L{};
+            L.kind = L_FUSED_SMALL;
+            L.list_off = (int32_t)lists.size();
+            int64_t mnf = 0, mnp = 0;
+            for (int64_t s : small) {
+                lists.push_back((int32_t)s);
+                mnf = std::max(mnf, nf[s]);
+                mnp = std::max(mnp, np[s]);
+                L.flops += front_flops(s);
+                L.bytes += asm_bytes(s);
+            }
+            L.count = (int32_t)small.size();
+            L.shmem = (int32_t)(mnf * mnf * elem + mnp * 4);
+            L.level = (int32_t)lv;
+            if (L.shmem > 160 * 1024) return fail(MF_BADARG, "small_nf too large for LDS");
+            P.launches.push_back(L);
+        }
+    }
+    // solve lists: all fronts per level
+    std::vector<int32_t> slist;
+    P.lvl_off.resize(nlev);
+    P.lvl_cnt.resize(nlev);
+    P.lvl_shfwd.resize(nlev);
+    P.lvl_shbwd.resize(nlev);
+    for (int64_t lv = 0; lv < nlev; lv++) {
+        P.lvl_off[lv] = (int32_t)slist.size();
+        int64_t mf = 0, mb = 0;
+        for (int64_t s : bylevel[lv]) {
+            slist.push_back((int32_t)s);
+            mf = std::max(mf, nf[s] + np[s]);
+            mb = std::max(mb, nf[s]);
+        }
+        P.lvl_cnt[lv] = (int32_t)bylevel[lv].size();
+        double lf = 0, uf = 0, lb = 0, ub = 0, vec = 0;
+        for (int64_t s : bylevel[lv]) {
+            double p = (double)np[s], f = (double)nf[s];
+            lf += 2.0 * (p * (p - 1) / 2 + (f - p) * p);   // unit-lower L11 + L21
+            uf += 2.0 * (p * (p + 1) / 2 + p * (f - p));   // U11 + U12
+            lb += (p * (p - 1) / 2 + (f - p) * p) * elem;
+            ub += (p * (p + 1) / 2 + p * (f - p)) * elem;
+            vec += f;
+        }
+        P.lvl_lflops.push_back(lf);
+        P.lvl_uflops.push_back(uf);
+        P.lvl_lbytes.push_back(lb);
+        P.lvl_ubytes.push_back(ub);
+        P.lvl_vec.push_back(vec);
+        P.lvl_shfwd[lv] = (int32_t)(mf * elem);
+        P.lvl_shbwd[lv] = (int32_t)(mb * elem);
+        if (P.lvl_shfwd[lv] > 160 * 1024) return fail(MF_BADARG, "front too large for the solve kernels");
+    }
+    // CSC export offsets (src/multifrontal.jl:73-104, :277-317)
+    P.Lp.assign(n + 1, 0);
+    P.Up.assign(n + 1, 0);
+    {
+        std::vector<int64_t> ucount(n, 0);
+        for (int64_t s = 0; s < nsuper; s++) {
+            for (int64_t a = 0; a < np[s]; a++) {
+                int64_t j = c1[s] + a;
+                P.Lp[j + 1] = 1 + (np[s] - 1 - a) + nu[s];
+                ucount[j] += a + 1;
+            }
+            for (int64_t t = 0; t < nu[s]; t++) ucount[upd[uoff0[s] + t]] += np[s];
+        }
+        for (int64_t j = 0; j < n; j++) P.Lp[j + 1] += P.Lp[j];
+        for (int64_t j = 0; j < n; j++) P.Up[j + 1] = P.Up[j] + ucount[j];
+        P.uoff.assign(P.nupd, 0);
+        std::vector<int64_t> cur(P.Up.begin(), P.Up.end() - 1);
+        for (int64_t s = 0; s < nsuper; s++)
+            for (int64_t t = 0; t < nu[s]; t++) {
+                int64_t J = upd[uoff0[s] + t];
+                P.uoff[uoff0[s] + t] = cur[J];
+                cur[J] += np[s];
+            }
+        P.nnzL = P.Lp[n];
+        P.nnzU = P.Up[n];
+    }
+    // CSR of A for the residual: rows ascending within, column order kept per row
+    std::vector<int32_t> rp(n + 1, 0), ci(P.nnz), vmap(P.nnz), arow(P.nnz);
+    for (int64_t p = 0; p < P.nnz; p++) { arow[p] = (int32_t)(Ai1[p] - 1); rp[arow[p] + 1]++; }
+    for (int64_t i = 0; i < n; i++) rp[i + 1] += rp[i];
+    {
+        std::vector<int32_t> cur(rp.begin(), rp.end() - 1);
+        for (int64_t j = 0; j < n; j++)
+            for (int64_t p = Ap1[j] - 1; p < Ap1[j + 1] - 1; p++) {
+                int32_t q = cur[arow[p]]++;
+                ci[q] = (int32_t)j;
+                vmap[q] = (int32_t)p;
+            }
+    }
+    std::vector<int32_t> qf32(n);
+    for (int64_t j = 0; j < n; j++) qf32[j] = (int32_t)P.qf[j];
+
+    // ---- device uploads
+    HIPCHK(upload(P.d_sn, P.sn));
+    HIPCHK(upload(P.d_upd, upd));
+    HIPCHK(upload(P.d_rmap, rmap));
+    HIPCHK(upload(P.d_child, child_rev));
+    HIPCHK(upload(P.d_orow, orow));
+    HIPCHK(upload(P.d_ocol, ocol));
+    HIPCHK(upload(P.d_osrc, osrc));
+    HIPCHK(upload(P.d_ocolptr, ocolptr));
+    HIPCHK(upload(P.d_ecolptr, ecolptr));
+    HIPCHK(upload(P.d_ext, ext));
+    HIPCHK(upload(P.d_arow, arow));
+    HIPCHK(upload(P.d_lists, lists));
+    HIPCHK(upload(P.d_probs, probs));
+    HIPCHK(upload(P.d_tiles, tiles));
+    HIPCHK(upload(P.d_solvelist, slist));
+    HIPCHK(upload(P.d_qf, qf32));
+    HIPCHK(upload(P.d_csr_rp, rp));
+    HIPCHK(upload(P.d_csr_ci, ci));
+    HIPCHK(upload(P.d_csr_vmap, vmap));
+    HIPCHK(upload(P.d_uoff, P.uoff));
+    HIPCHK(P.d_pool.alloc((size_t)P.pool_elems * P.esize() * P.nbatch));
+    HIPCHK(P.d_prow.alloc((size_t)n * 4 * P.nbatch));
+    HIPCHK(P.d_info.alloc((size_t)4 * P.nbatch));
+    HIPCHK(P.d_ax.alloc((size_t)P.nnz * 8 * P.nbatch));
+    HIPCHK(P.d_rs.alloc((size_t)n * 8 * P.nbatch));
+    HIPCHK(P.d_anorm.alloc((size_t)8 * P.nbatch));
+    return MF_OK;
+}
+
+extern "C" int mf_plan_create(int64_t n, const int64_t *qf, int64_t nsuper, const int64_t *sstart,
+                              const int64_t *upd_ptr, const int64_t *upd_idx, const int64_t *child_ptr,
+                              const int64_t *child_idx, const int64_t *Ap, const int64_t *Ai, const mf_options *opt,
+                              mf_plan **out)
+{
+    if (!out) return fail(MF_BADARG, "out is NULL");
+    *out = nullptr;
+    if (n <= 0) return fail(MF_DIM, "matrix must have n >= 1");
+    if (!qf || !sstart || !upd_ptr || !child_ptr || !Ap || nsuper <= 0 || nsuper > n)
+        return fail(MF_BADARG, "missing plan arguments");
+    std::unique_ptr<mf_plan> P(new (std::nothrow) mf_plan);
+    if (!P) return fail(MF_OOM, "host allocation failed");
+    if (opt) {
+        if (opt->precision < MF_FP64 || opt->precision > MF_BF16) return fail(MF_BADARG, "unknown precision");
+        P->prec = opt->precision;
+        if (opt->small_nf > 0) P->small_nf = opt->small_nf;
+        P->use_graph = opt->use_graph;
+        P->nbatch = opt->nbatch > 0 ? opt->nbatch : 1;
+    }
+    if (P->small_nf > 128) return fail(MF_BADARG, "small_nf must be <= 128");
+    HIPCHK(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+    int rc;
+    try {
+        rc = build_plan(*P, n, qf, nsuper, sstart, upd_ptr, upd_idx, child_ptr, child_idx, Ap, Ai);
+    } catch (const std::bad_alloc &) {
+        return fail(MF_OOM, "host allocation failed while building the plan");
+    }
+    if (rc != MF_OK) return rc;
+    *out = P.release();
+    return MF_OK;
+}
+
+extern "C" void mf_plan_destroy(mf_plan *P) { delete P; }
+
+extern "C" int mf_nnz(const mf_plan *P, int64_t *nnzL, int64_t *nnzU)
+{
+    if (!P) return fail(MF_BADARG, "NULL plan");
+    if (nnzL) *nnzL = P->nnzL;
+    if (nnzU) *nnzU = P->nnzU;
+    return MF_OK;
+}
+
+extern "C" int mf_plan_get_stats(const mf_plan *P, mf_plan_stats *st)
+{
+    if (!P || !st) return fail(MF_BADARG, "NULL argument");
+    st->factor_flops = P->flops;
+    st->gemm_flops = P->gemm_flops;
+    st->gemm_flops_big = P->gemm_flops_big;
+    st->front_bytes = (double)P->pool_elems * (double)P->esize();
+    st->nsuper = P->nsuper;
+    st->nlevels = P->nlevels;
+    st->launches = (int64_t)P->launches.size();
+    st->max_nf = P->maxnf;
+    return MF_OK;
+}
+
+// ------------------------------------------------------------------ factor
+template <typename T>
+static int run_factor(mf_plan &P, const double *d_ax, const double *d_rs, int nmat)
+{
+    FactorArgs a{};
+    a.sn = P.d_sn.as<SnDev>();
+    a.orig_row = P.d_orow.as<int32_t>();
+    a.orig_col = P.d_ocol.as<int32_t>();
+    a.orig_src = P.d_osrc.as<int32_t>();
+    a.orig_colptr = P.d_ocolptr.as<int32_t>();
+    a.child_list = P.d_child.as<int32_t>();
+    a.rmap = P.d_rmap.as<int32_t>();
+    a.ext_colptr = P.d_ecolptr.as<int32_t>();
+    a.ext = P.d_ext.as<int2>();
+    a.arow = P.d_arow.as<int32_t>();
+    a.ax = d_ax;
+    a.rs = d_rs;
+    a.pool = P.d_pool.p;
+    a.prow = P.d_prow.as<int32_t>();
+    a.info = P.d_info.as<int32_t>();
+    a.pool_stride = P.pool_elems;
+    a.ax_stride = P.nnz;
+    a.rs_stride = P.n;
+    a.n_stride = P.n;
+    HIPCHK(launch_fill_i32(P.d_info.as<int32_t>(), INT32_MAX, nmat, P.stream));
+    for (const Launch &L : P.launches)
+        HIPCHK(profiled(P, launch_class(L), L.flops * nmat, L.bytes * nmat, [&] {
+            return launch_factor_step<T>(L, a, P.d_lists.as<int32_t>(), P.d_probs.as<GemmProb>(),
+                                         P.d_tiles.as<GemmTile>(), nmat, P.stream);
+        }));
+    HIPCHK(launch_anorm(P.d_csr_rp.as<int32_t>(), P.d_csr_vmap.as<int32_t>(), d_ax, P.nnz, P.n,
+                        P.d_anorm.as<double>(), nmat, P.stream));
+    return MF_OK;
+}
+
+static int factor_common(mf_plan *P, const double *d_ax, const double *d_rs, int nmat, int32_t check,
+                         int64_t *info_col)
+{
+    int rc;
+    const int rsg = d_rs != nullptr;
+    if (P->use_graph && !P->prof) {
+        // inputs are always read from the plan's own buffers, so the captured graph stays valid
+        if (!P->graph || P->graph_nmat != nmat || rsg != P->graph_rs) {
+            if (P->graph) { (void)hipGraphExecDestroy(P->graph); P->graph = nullptr; }
+            hipGraph_t g;
+            HIPCHK(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
+            rc = P->prec == MF_FP64 ? run_factor<double>(*P, d_ax, d_rs, nmat) : run_factor<float>(*P, d_ax, d_rs, nmat);
+            hipError_t ec = hipStreamEndCapture(P->stream, &g);
+            if (rc != MF_OK) return rc;
+            HIPCHK(ec);
+            hipError_t ei = hipGraphInstantiate(&P->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPCHK(ei);
+            P->graph_nmat = nmat;
+            P->graph_rs = rsg;
+        }
+        HIPCHK(hipGraphLaunch(P->graph, P->stream));
+    } else {
+        rc = P->prec == MF_FP64 ? run_factor<double>(*P, d_ax, d_rs, nmat) : run_factor<float>(*P, d_ax, d_rs, nmat);
+        if (rc != MF_OK) return rc;
+    }
+    P->rs_given = rsg;
+    std::vector<int32_t> info(nmat);
+    HIPCHK(hipMemcpyAsync(info.data(), P->d_info.p, (size_t)nmat * 4, hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    if (P->prof) HIPCHK(P->ev_collect());
+    P->factored = true;
+    P->nmat_factored = nmat;
+    bool sing = false;
+    for (int m = 0; m < nmat; m++) {
+        int64_t c = info[m] == INT32_MAX ? 0 : (int64_t)info[m] + 1;
+        if (info_col) info_col[m] = c;
+        if (c) sing = true;
+    }
+    if (check && sing) {
+        for (int m = 0; m < nmat; m++)
+            if (info[m] != INT32_MAX) {
+                g_err = "SingularException(" + std::to_string((long long)info[m] + 1) + ")";
+                break;
+            }
+        return MF_SINGULAR;
+    }
+    return MF_OK;
+}
+
+extern "C" int mf_factor_device(mf_plan *P, const double *d_Ax, const double *d_Rs, int32_t check, int64_t *info_col)
+{
+    if (!P || !d_Ax) return fail(MF_BADARG, "NULL argument");
+    // the plan keeps its own copy: the refinement residual reads A and Rs again later
+    HIPCHK(hipMemcpyAsync(P->d_ax.p, d_Ax, (size_t)P->nnz * 8, hipMemcpyDeviceToDevice, P->stream));
+    if (d_Rs) HIPCHK(hipMemcpyAsync(P->d_rs.p, d_Rs, (size_t)P->n * 8, hipMemcpyDeviceToDevice, P->stream));
+    return factor_common(P, P->d_ax.as<double>(), d_Rs ? P->d_rs.as<double>() : nullptr, 1, check, info_col);
+}
+
+extern "C" int mf_factor(mf_plan *P, const double *Ax, const double *Rs, int32_t check, int64_t *info_col)
+{
+    if (!P || !Ax) return fail(MF_BADARG, "NULL argument");
+    HIPCHK(hipMemcpyAsync(P->d_ax.p, Ax, (size_t)P->nnz * 8, hipMemcpyHostToDevice, P->stream));
+    if (Rs) HIPCHK(hipMemcpyAsync(P->d_rs.p, Rs, (size_t)P->n * 8, hipMemcpyHostToDevice, P->stream));
+    return factor_common(P, P->d_ax.as<double>(), Rs ? P->d_rs.as<double>() : nullptr, 1, check, info_col);
+}
+
+extern "C" int mf_refactor_batch(mf_plan *P, int64_t nmat, const double *Ax_batch, const double *Rs_batch,
+                                 int64_t *info_col)
+{
+    if (!P || !Ax_batch) return fail(MF_BADARG, "NULL argument");
+    if (nmat < 1 || nmat > P->nbatch) return fail(MF_BADARG, "nmat exceeds the plan's nbatch");
+    HIPCHK(hipMemcpyAsync(P->d_ax.p, Ax_batch, (size_t)P->nnz * 8 * nmat, hipMemcpyHostToDevice, P->stream));
+    if (Rs_batch)
+        HIPCHK(hipMemcpyAsync(P->d_rs.p, Rs_batch, (size_t)P->n * 8 * nmat, hipMemcpyHostToDevice, P->stream));
+    return factor_common(P, P->d_ax.as<double>(), Rs_batch ? P->d_rs.as<double>() : nullptr, (int)nmat, 0, info_col);
+}
+
+// ------------------------------------------------------------------ solve
+template <typename T>
+static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumulate, int nmat)
+{
+    // d_rhs: n x nrhs per matrix (ld n); result into P.d_X (ld n), overwritten or accumulated
+    const int64_t n = P.n;
+    const double *rs = P.rs_given ? P.d_rs.as<double>() : nullptr;
+    HIPCHK(launch_prep_rhs<T>(d_rhs, n, rs, n, P.d_qf.as<int32_t>(), P.d_w0.as<T>(), n, nrhs, n * nrhs, nmat,
+                              P.stream));
+    SolveArgs a{};
+    a.sn = P.d_sn.as<SnDev>();
+    a.child_list = P.d_child.as<int32_t>();
+    a.rmap = P.d_rmap.as<int32_t>();
+    a.upd = P.d_upd.as<int32_t>();
+    a.pool = P.d_pool.p;
+    a.prow = P.d_prow.as<int32_t>();
+    a.w0 = P.d_w0.p;
+    a.y = P.d_y.p;
+    a.x = P.d_x.p;
+    a.uv = P.d_uv.p;
+    a.n = n;
+    a.nupd = P.nupd;
+    a.nrhs = nrhs;
+    a.pool_stride = P.pool_elems;
+    a.n_stride = n;
+    const double e = (double)P.esize();
+    for (int64_t lv = 0; lv < P.nlevels; lv++)
+        HIPCHK(profiled(P, 5, P.lvl_lflops[lv] * nrhs * nmat, (P.lvl_lbytes[lv] + 3.0 * e * P.lvl_vec[lv] * nrhs) * nmat,
+                        [&] {
+                            return launch_fwd<T>(a, P.d_solvelist.as<int32_t>(), P.lvl_off[lv], P.lvl_cnt[lv],
+                                                 P.lvl_shfwd[lv], nmat, P.stream);
+                        }));
+    for (int64_t lv = P.nlevels - 1; lv >= 0; lv--)
+        HIPCHK(profiled(P, 6, P.lvl_uflops[lv] * nrhs * nmat, (P.lvl_ubytes[lv] + 3.0 * e * P.lvl_vec[lv] * nrhs) * nmat,
+                        [&] {
+                            return launch_bwd<T>(a, P.d_solvelist.as<int32_t>(), P.lvl_off[lv], P.lvl_cnt[lv],
+                                                 P.lvl_shbwd[lv], nmat, P.stream);
+                        }));
+    HIPCHK(launch_finish_x<T>(P.d_x.as<T>(), P.d_qf.as<int32_t>(), P.d_X.as<double>(), n, n, nrhs, accumulate,
+                              n * nrhs, nmat, P.stream));
+    return MF_OK;
+}
+
+static int solve_core(mf_plan *P, int64_t nrhs, int nmat, int32_t refine_steps, double bwd_tol,
+                      mf_solve_report *rep)
+{
+    // B already in P->d_B (n x nrhs x nmat); result left in P->d_X
+    const int64_t n = P->n;
+    auto t0 = std::chrono::steady_clock::now();
+    int rc;
+    auto once = [&](const double *rhs, int acc) {
+        return P->prec == MF_FP64 ? solve_once<double>(*P, rhs, nrhs, acc, nmat)
+                                  : solve_once<float>(*P, rhs, nrhs, acc, nmat);
+    };
+    auto norms = [&](std::vector<double> &h) -> int {
+        HIPCHK(launch_residual(P->d_csr_rp.as<int32_t>(), P->d_csr_ci.as<int32_t>(), P->d_csr_vmap.as<int32_t>(),
+                               P->d_ax.as<double>(), P->nnz, P->d_X.as<double>(), n, P->d_B.as<double>(), n,
+                               P->d_R.as<double>(), n, nrhs, n * nrhs, n * nrhs, nmat, P->stream));
+        HIPCHK(launch_norms(P->d_R.as<double>(), P->d_X.as<double>(), n, P->d_B.as<double>(), n, n, nrhs, n * nrhs,
+                            n * nrhs, P->d_norms.as<double>(), nmat, P->stream));
+        h.resize((size_t)5 * nrhs * nmat + nmat);
+        HIPCHK(hipMemcpyAsync(h.data(), P->d_norms.p, (size_t)5 * nrhs * nmat * 8, hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipMemcpyAsync(h.data() + 5 * nrhs * nmat, P->d_anorm.p, (size_t)nmat * 8, hipMemcpyDeviceToHost,
+                              P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+        return MF_OK;
+    };
+    auto bwd = [&](const std::vector<double> &h, double &be, double &r2) {
+        be = 0;
+        r2 = 0;
+        for (int m = 0; m < nmat; m++) {
+            double an = h[5 * nrhs * nmat + m];
+            for (int64_t c = 0; c < nrhs; c++) {
+                const double *o = &h[(m * nrhs + c) * 5];
+                double den = an * o[1] + o[2];
+                be = std::max(be, den > 0 ? o[0] / den : o[0]);
+                r2 = std::max(r2, o[4] > 0 ? std::sqrt(o[3] / o[4]) : std::sqrt(o[3]));
+            }
+        }
+    };
+    if ((rc = once(P->d_B.as<double>(), 0)) != MF_OK) return rc;
+    int it = 0, conv = 0;
+    std::vector<double> h;
+    double be = 0, r2 = 0;
+    for (; it < refine_steps; it++) {
+        if (bwd_tol > 0) {
+            if ((rc = norms(h)) != MF_OK) return rc;
+            bwd(h, be, r2);
+            if (be <= bwd_tol) { conv = 1; break; }
+        } else {
+            HIPCHK(launch_residual(P->d_csr_rp.as<int32_t>(), P->d_csr_ci.as<int32_t>(),
+                                   P->d_csr_vmap.as<int32_t>(), P->d_ax.as<double>(), P->nnz, P->d_X.as<double>(), n,
+                                   P->d_B.as<double>(), n, P->d_R.as<double>(), n, nrhs, n * nrhs, n * nrhs, nmat,
+                                   P->stream));
+        }
+        if ((rc = once(P->d_R.as<double>(), 1)) != MF_OK) return rc;
+    }
+    if (rep) {
+        if ((rc = norms(h)) != MF_OK) return rc;
+        bwd(h, be, r2);
+        if (bwd_tol > 0 && be <= bwd_tol) conv = 1;
+        rep->iterations = it;
+        rep->converged = conv;
+        rep->backward_error = be;
+        rep->residual_2 = r2;
+    } else {
+        HIPCHK(hipStreamSynchronize(P->stream));
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (rep) rep->seconds = std::chrono::duration<double>(t1 - t0).count();
+    if (P->prof) {
+        HIPCHK(hipStreamSynchronize(P->stream));
+        HIPCHK(P->ev_collect());
+    }
+    return MF_OK;
+}
+
+static int ensure_solve_buffers(mf_plan *P, int64_t nrhs, int nmat)
+{
+    size_t e = P->esize();
+    size_t nv = (size_t)P->n * nrhs * nmat;
+    HIPCHK(P->d_w0.alloc(nv * e));
+    HIPCHK(P->d_y.alloc(nv * e));
+    HIPCHK(P->d_x.alloc(nv * e));
+    HIPCHK(P->d_uv.alloc((size_t)std::max<int64_t>(P->nupd, 1) * nrhs * nmat * e));
+    HIPCHK(P->d_B.alloc(nv * 8));
+    HIPCHK(P->d_X.alloc(nv * 8));
+    HIPCHK(P->d_R.alloc(nv * 8));
+    HIPCHK(P->d_norms.alloc((size_t)5 * nrhs * nmat * 8));
+    return MF_OK;
+}
+
+extern "C" int mf_solve_device(mf_plan *P, int64_t nrhs, const double *d_B, int64_t ldb, double *d_X, int64_t ldx,
+                               int32_t refine_steps, double bwd_tol, mf_solve_report *rep)
+{
+    if (!P || !d_B || !d_X) return fail(MF_BADARG, "NULL argument");
+    if (!P->factored) return fail(MF_BADARG, "plan has no factorization");
+    if (nrhs < 1 || ldb < P->n || ldx < P->n || refine_steps < 0) return fail(MF_DIM, "bad RHS dimensions");
+    int rc = ensure_solve_buffers(P, nrhs, 1);
+    if (rc != MF_OK) return rc;
+    const size_t n8 = (size_t)P->n * 8;
+    HIPCHK(hipMemcpy2DAsync(P->d_B.p, n8, d_B, (size_t)ldb * 8, n8, nrhs, hipMemcpyDeviceToDevice, P->stream));
+    if ((rc = solve_core(P, nrhs, 1, refine_steps, bwd_tol, rep)) != MF_OK) return rc;
+    HIPCHK(hipMemcpy2DAsync(d_X, (size_t)ldx * 8, P->d_X.p, n8, n8, nrhs, hipMemcpyDeviceToDevice, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    return MF_OK;
+}
+
+extern "C" int mf_solve(mf_plan *P, int64_t nrhs, const double *B, int64_t ldb, double *X, int64_t ldx,
+                        int32_t refine_steps, double bwd_tol, mf_solve_report *rep)
+{
+    if (!P || !B || !X) return fail(MF_BADARG, "NULL argument");
+    if (!P->factored) return fail(MF_BADARG, "plan has no factorization");
+    if (nrhs < 1 || ldb < P->n || ldx < P->n || refine_steps < 0) return fail(MF_DIM, "bad RHS dimensions");
+    int rc = ensure_solve_buffers(P, nrhs, 1);
+    if (rc != MF_OK) return rc;
+    const size_t n8 = (size_t)P->n * 8;
+    HIPCHK(hipMemcpy2DAsync(P->d_B.p, n8, B, (size_t)ldb * 8, n8, nrhs, hipMemcpyHostToDevice, P->stream));
+    if ((rc = solve_core(P, nrhs, 1, refine_steps, bwd_tol, rep)) != MF_OK) return rc;
+    HIPCHK(hipMemcpy2DAsync(X, (size_t)ldx * 8, P->d_X.p, n8, n8, nrhs, hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    return MF_OK;
+}
+
+extern "C" int mf_solve_batch(mf_plan *P, int64_t nmat, int64_t nrhs, const double *B, double *X,
+                              int32_t refine_steps, double bwd_tol, mf_solve_report *rep)
+{
+    if (!P || !B || !X) return fail(MF_BADARG, "NULL argument");
+    if (!P->factored || nmat != P->nmat_factored) return fail(MF_BADARG, "solve_batch needs a matching refactor_batch");
+    if (nrhs < 1 || refine_steps < 0) return fail(MF_DIM, "bad RHS dimensions");
+    int rc = ensure_solve_buffers(P, nrhs, (int)nmat);
+    if (rc != MF_OK) return rc;
+    size_t bytes = (size_t)P->n * nrhs * nmat * 8;
+    HIPCHK(hipMemcpyAsync(P->d_B.p, B, bytes, hipMemcpyHostToDevice, P->stream));
+    if ((rc = solve_core(P, nrhs, (int)nmat, refine_steps, bwd_tol, rep)) != MF_OK) return rc;
+    HIPCHK(hipMemcpyAsync(X, P->d_X.p, bytes, hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    return MF_OK;
+}
+
+// ------------------------------------------------------------------ export
+extern "C" int mf_export_factors(mf_plan *P, int64_t *Lcolptr, int64_t *Lrowval, double *Lnzval, int64_t *Ucolptr,
+                                 int64_t *Urowval, double *Unzval, int64_t *p, int64_t *q)
+{
+    if (!P) return fail(MF_BADARG, "NULL plan");
+    if (!P->factored) return fail(MF_BADARG, "plan has no factorization");
+    const int64_t n = P->n;
+    if (Lcolptr)
+        for (int64_t j = 0; j <= n; j++) Lcolptr[j] = P->Lp[j] + 1;
+    if (Ucolptr)
+        for (int64_t j = 0; j <= n; j++) Ucolptr[j] = P->Up[j] + 1;
+    std::vector<int32_t> prow(n);
+    HIPCHK(hipMemcpy(prow.data(), P->d_prow.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (p)
+        for (int64_t k = 0; k < n; k++) p[k] = P->qf[prow[k]] + 1;
+    if (q)
+        for (int64_t k = 0; k < n; k++) q[k] = P->qf[k] + 1;
+    if (Lrowval || Lnzval || Urowval || Unzval) {
+        HIPCHK(P->d_rowfac.alloc((size_t)n * 4));
+        HIPCHK(upload(P->d_Lp, P->Lp));
+        HIPCHK(upload(P->d_Up, P->Up));
+        HIPCHK(P->d_Li.alloc((size_t)P->nnzL * 8));
+        HIPCHK(P->d_Lx.alloc((size_t)P->nnzL * 8));
+        HIPCHK(P->d_Ui.alloc((size_t)P->nnzU * 8));
+        HIPCHK(P->d_Ux.alloc((size_t)P->nnzU * 8));
+        HIPCHK(launch_rowfac(P->d_prow.as<int32_t>(), P->d_rowfac.as<int32_t>(), n, P->stream));
+        if (P->prec == MF_FP64)
+            HIPCHK(launch_export<double>(P->d_sn.as<SnDev>(), P->nsuper, P->d_pool.as<double>(), P->d_upd.as<int32_t>(),
+                                         P->d_rowfac.as<int32_t>(), P->d_Lp.as<int64_t>(), P->d_Up.as<int64_t>(),
+                                         P->d_uoff.as<int64_t>(), P->d_Li.as<int64_t>(), P->d_Lx.as<double>(),
+                                         P->d_Ui.as<int64_t>(), P->d_Ux.as<double>(), P->stream));
+        else
+            HIPCHK(launch_export<float>(P->d_sn.as<SnDev>(), P->nsuper, P->d_pool.as<float>(), P->d_upd.as<int32_t>(),
+                                        P->d_rowfac.as<int32_t>(), P->d_Lp.as<int64_t>(), P->d_Up.as<int64_t>(),
+                                        P->d_uoff.as<int64_t>(), P->d_Li.as<int64_t>(), P->d_Lx.as<double>(),
+                                        P->d_Ui.as<int64_t>(), P->d_Ux.as<double>(), P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+        if (Lrowval) HIPCHK(hipMemcpy(Lrowval, P->d_Li.p, (size_t)P->nnzL * 8, hipMemcpyDeviceToHost));
+        if (Lnzval) HIPCHK(hipMemcpy(Lnzval, P->d_Lx.p, (size_t)P->nnzL * 8, hipMemcpyDeviceToHost));
+        if (Urowval) HIPCHK(hipMemcpy(Urowval, P->d_Ui.p, (size_t)P->nnzU * 8, hipMemcpyDeviceToHost));
+        if (Unzval) HIPCHK(hipMemcpy(Unzval, P->d_Ux.p, (size_t)P->nnzU * 8, hipMemcpyDeviceToHost));
+    }
+    return MF_OK;
+}
+
+extern "C" int mf_plan_set_profiling(mf_plan *P, int32_t on)
+{
+    if (!P) return fail(MF_BADARG, "NULL plan");
+    P->prof = on ? 1 : 0;
+    for (int c = 0; c < MF_PROF_CLASSES; c++) {
+        P->prof_ms[c] = P->prof_flops[c] = P->prof_bytes[c] = 0;
+        P->prof_n[c] = 0;
+    }
+    P->evrec.clear();
+    P->evused = 0;
+    return MF_OK;
+}
+
+extern "C" int mf_plan_get_profile(const mf_plan *P, double *ms, int64_t *launches, double *flops, double *bytes)
+{
+    if (!P) return fail(MF_BADARG, "NULL plan");
+    for (int c = 0; c < MF_PROF_CLASSES; c++) {
+        if (ms) ms[c] = P->prof_ms[c];
+        if (launches) launches[c] = P->prof_n[c];
+        if (flops) flops[c] = P->prof_flops[c];
+        if (bytes) bytes[c] = P->prof_bytes[c];
+    }
+    return MF_OK;
+}

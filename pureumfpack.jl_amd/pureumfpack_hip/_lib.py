@@ -1,0 +1,86 @@
+"""ctypes binding of libmfgpu.so (the C ABI declared in include/mfgpu.h).
+
+Loading fails loudly: there is no CPU fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+from ._build import LIB_PATH
+
+MF_OK, MF_SINGULAR, MF_DIM, MF_BADARG, MF_HIP, MF_OOM = range(6)
+MF_FP64, MF_FP32, MF_BF16 = 0, 1, 2
+MF_ORDER_AMD, MF_ORDER_COLAMD, MF_ORDER_GIVEN = 0, 1, 2
+
+c_i64p = ctypes.POINTER(ctypes.c_int64)
+c_f64p = ctypes.POINTER(ctypes.c_double)
+c_void_pp = ctypes.POINTER(ctypes.c_void_p)
+
+
+class mf_options(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("small_nf", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+                ("nbatch", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
+
+
+class mf_solve_report(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("converged", ctypes.c_int32), ("backward_error", ctypes.c_double),
+                ("residual_2", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+class mf_plan_stats(ctypes.Structure):
+    _fields_ = [("factor_flops", ctypes.c_double), ("gemm_flops", ctypes.c_double),
+                ("gemm_flops_big", ctypes.c_double), ("front_bytes", ctypes.c_double),
+                ("nsuper", ctypes.c_int64), ("nlevels", ctypes.c_int64), ("launches", ctypes.c_int64),
+                ("max_nf", ctypes.c_int64)]
+
+
+# exported symbol -> (restype, argtypes)
+SIGNATURES = {
+    "mf_analyze": (ctypes.c_int, [ctypes.c_int64, c_i64p, c_i64p, ctypes.c_int32, c_i64p, c_void_pp]),
+    "mf_symbolic_sizes": (ctypes.c_int, [ctypes.c_void_p, c_i64p, c_i64p, c_i64p, c_i64p]),
+    "mf_symbolic_export": (ctypes.c_int, [ctypes.c_void_p, c_i64p, c_i64p, c_i64p, c_i64p, c_i64p, c_i64p, c_i64p]),
+    "mf_symbolic_destroy": (None, [ctypes.c_void_p]),
+    "mf_plan_create": (ctypes.c_int, [ctypes.c_int64, c_i64p, ctypes.c_int64, c_i64p, c_i64p, c_i64p, c_i64p,
+                                      c_i64p, c_i64p, c_i64p, ctypes.POINTER(mf_options), c_void_pp]),
+    "mf_factor": (ctypes.c_int, [ctypes.c_void_p, c_f64p, c_f64p, ctypes.c_int32, c_i64p]),
+    "mf_factor_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, c_i64p]),
+    "mf_refactor_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, c_f64p, c_f64p, c_i64p]),
+    "mf_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, c_f64p, ctypes.c_int64, c_f64p, ctypes.c_int64,
+                                ctypes.c_int32, ctypes.c_double, ctypes.POINTER(mf_solve_report)]),
+    "mf_solve_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
+                                       ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double,
+                                       ctypes.POINTER(mf_solve_report)]),
+    "mf_solve_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, c_f64p, c_f64p,
+                                      ctypes.c_int32, ctypes.c_double, ctypes.POINTER(mf_solve_report)]),
+    "mf_nnz": (ctypes.c_int, [ctypes.c_void_p, c_i64p, c_i64p]),
+    "mf_export_factors": (ctypes.c_int, [ctypes.c_void_p, c_i64p, c_i64p, c_f64p, c_i64p, c_i64p, c_f64p, c_i64p,
+                                         c_i64p]),
+    "mf_plan_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(mf_plan_stats)]),
+    "mf_plan_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
+    "mf_plan_get_profile": (ctypes.c_int, [ctypes.c_void_p, c_f64p, c_i64p, c_f64p, c_f64p]),
+    "mf_last_error": (ctypes.c_char_p, []),
+    "mf_plan_destroy": (None, [ctypes.c_void_p]),
+}
+
+_LIB = None
+
+
+def load():
+    """Load libmfgpu.so built in-tree; raise if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libmfgpu.so not built ({LIB_PATH}); run __graft_entry__.build() first")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def ptr(a, kind=c_i64p):
+    return a.ctypes.data_as(kind) if a is not None else None
