@@ -130,24 +130,32 @@ def run_gpu(hip, A, S, Rs, B, precision, steps, warmup, tol, max_refine):
 
 
 def cpu_baseline(A, S, Rs, B, refine):
-    """The oracle's C restatement of the reference multifrontal path (FP64, OpenBLAS on all host
-    cores), timed on the same matrix: numeric factor + solve(F, b; refine) per RHS column."""
+    """The oracle's C restatement of the reference multifrontal path (FP64, dense fronts through
+    OpenBLAS), timed on the same matrix: numeric factor + solve(F, b; refine) per RHS column.
+    Timed with 1 BLAS thread (the reference's published setting) and with every host core the
+    process may use; the faster run is reported, both are kept."""
     from oracle import numeric as ON
-    cores = len(os.sched_getaffinity(0))
-    ON.use_openblas(threads=cores)
 
     class _Q:
         qf = S.qf
 
     flat = (S.sstart, S.upd_ptr, S.upd_idx, S.child_ptr, S.child_idx)
-    t0 = time.perf_counter()
-    F = ON.splu_mf(A, S=_Q, flat=flat, backend=1)
-    t1 = time.perf_counter()
-    for j in range(B.shape[1]):
-        ON.solve(F, B[:, j], refine=refine)
-    t2 = time.perf_counter()
-    return {"value": t2 - t0, "unit": "s", "cores": cores, "kind": "port",
-            "factor_s": t1 - t0, "solve_s": t2 - t1,
+    runs = {}
+    for threads in sorted({1, len(os.sched_getaffinity(0))}):
+        ON.use_openblas(threads=threads)
+        t0 = time.perf_counter()
+        F = ON.splu_mf(A, S=_Q, flat=flat, backend=1)
+        t1 = time.perf_counter()
+        for j in range(B.shape[1]):
+            ON.solve(F, B[:, j], refine=refine)
+        t2 = time.perf_counter()
+        runs[threads] = {"value": t2 - t0, "factor_s": t1 - t0, "solve_s": t2 - t1}
+        print(f"# cpu baseline {threads} thread(s): {t2 - t0:.2f} s", file=sys.stderr, flush=True)
+        del F
+    best = min(runs, key=lambda t: runs[t]["value"])
+    return {"value": runs[best]["value"], "unit": "s", "cores": best, "kind": "port",
+            "factor_s": runs[best]["factor_s"], "solve_s": runs[best]["solve_s"],
+            "by_threads": {str(t): r for t, r in runs.items()},
             "sample": f"full workload: FP64 factor + {B.shape[1]} x solve(F, b; refine={refine})"}
 
 

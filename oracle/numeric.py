@@ -80,9 +80,10 @@ def use_openblas(threads: int | None = None) -> bool:
     addr = lambda name: ctypes.cast(getattr(B, name), ctypes.c_void_p).value
     lib().mfo_set_blas(addr("scipy_dgetrf_64_"), addr("scipy_dtrsm_64_"), addr("scipy_dgemm_64_"))
     if threads is not None:
-        f = B.scipy_openblas_set_num_threads64_
-        f.argtypes = [ctypes.POINTER(ctypes.c_int64)]
-        f(ctypes.byref(ctypes.c_int64(threads)))
+        f = B.scipy_openblas_set_num_threads64_      # C API: void (int)
+        f.argtypes = [ctypes.c_int]
+        f.restype = None
+        f(int(threads))
     _BLAS_SET = True
     return True
 

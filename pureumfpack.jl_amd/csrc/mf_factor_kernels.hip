@@ -164,7 +164,6 @@ __global__ __launch_bounds__(NT) void k_panel(FactorArgs a, int k)
     __shared__ int s_redi[2][NW];
     __shared__ T s_prow[2][KB];
     __shared__ T s_jrow[2][KB];
-    __shared__ T s_blk[KB * KB];
     __shared__ int s_ipiv[KB];
 
     T x[RPT][KB];
@@ -250,7 +249,7 @@ __global__ __launch_bounds__(NT) void k_panel(FactorArgs a, int k)
         }
     }
 
-    // store the panel, keep the kb x kb diagonal block in LDS
+    // store the factored panel
 #pragma unroll
     for (int r = 0; r < RPT; r++) {
         const int i = tid + NT * r;
@@ -259,43 +258,93 @@ __global__ __launch_bounds__(NT) void k_panel(FactorArgs a, int k)
             for (int j = 0; j < KB; j++)
                 if (j < kb) F[(k + i) + (int64_t)(k + j) * nf] = x[r][j];
         }
-        if (i < kb) {
-#pragma unroll
-            for (int j = 0; j < KB; j++) s_blk[i + j * KB] = x[r][j];
+    }
+    // The kb sequential swaps only move rows inside the set R = {0..kb-1} U {ipiv[j]} (panel
+    // coordinates).  Compose them once: row dst[q] receives the old contents of row src[q].
+    // k_panel_apply applies them to the other columns.
+    if (tid == 0) {
+        int dst[2 * KB], src[2 * KB], cont[2 * KB];
+        int cnt = kb;
+        for (int q = 0; q < kb; q++) dst[q] = q;
+        for (int j = 0; j < kb; j++) {
+            int p = s_ipiv[j], found = 0;
+            for (int q = 0; q < cnt; q++) found |= (dst[q] == p);
+            if (!found) dst[cnt++] = p;
         }
+        for (int q = 0; q < cnt; q++) cont[q] = q;
+        for (int j = 0; j < kb; j++) {
+            int p = s_ipiv[j], qp = 0;
+            for (int q = 0; q < cnt; q++)
+                if (dst[q] == p) qp = q;
+            int t = cont[j]; cont[j] = cont[qp]; cont[qp] = t;
+        }
+        for (int q = 0; q < cnt; q++) src[q] = dst[cont[q]];
+        int32_t *pi = a.pinfo + (int64_t)mat * a.pinfo_stride + (int64_t)a.list[blockIdx.x] * PINFO_STRIDE;
+        pi[0] = cnt;
+        for (int q = 0; q < cnt; q++) { pi[1 + q] = dst[q]; pi[1 + 2 * KB + q] = src[q]; }
+        int32_t *prow = a.prow + mat * a.n_stride + d.c1 + k;
+        int orig[2 * KB];
+        for (int q = 0; q < cnt; q++) orig[q] = prow[src[q]];
+        for (int q = 0; q < cnt; q++) prow[dst[q]] = orig[q];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Large fronts, step 2b: after k_panel, for every front of the launch
+//   * work item kind 0 (256 columns per workgroup, one column per thread): permute the rows
+//     touched by the panel's swaps in every column outside the panel (LAPACK dlaswp on A11 and
+//     the R12 swaps of src/multifrontal.jl:256-263), and for columns right of the panel solve
+//     the U rows A[k:k+kb, c] <- L_kk^-1 A[k:k+kb, c]                (trsm L/L/N/U);
+//   * work item kind 1 (256 rows per workgroup): L21 columns of the panel,
+//     A[r, k:k+kb] <- A[r, k:k+kb] U_kk^-1 for r >= np               (trsm R/U/N/N).
+// Items: int4 {front, kind, start, 0}.
+// ------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel_apply(FactorArgs a, int k)
+{
+    constexpr int KB = PANEL_KB;
+    __shared__ T s_blk[KB * KB];
+    __shared__ int s_dst[2 * KB], s_src[2 * KB], s_cnt;
+    const int mat = blockIdx.y;
+    const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
+    const SnDev d = a.sn[w.x];
+    const int np = d.np, nf = d.nf;
+    const int kb = min(KB, np - k);
+    const int tid = threadIdx.x;
+    T *F = reinterpret_cast<T *>(a.pool) + mat * a.pool_stride + d.foff;
+    for (int e = tid; e < KB * KB; e += blockDim.x) {
+        const int i = e % KB, j = e / KB;
+        s_blk[e] = (i < kb && j < kb) ? F[(k + i) + (int64_t)(k + j) * nf] : T(0);
+    }
+    if (tid == 0) {
+        const int32_t *pi = a.pinfo + (int64_t)mat * a.pinfo_stride + (int64_t)w.x * PINFO_STRIDE;
+        const int cnt = pi[0];
+        s_cnt = cnt;
+        for (int q = 0; q < cnt; q++) { s_dst[q] = pi[1 + q]; s_src[q] = pi[1 + 2 * KB + q]; }
     }
     __syncthreads();
-    if (tid == 0) {
-        int32_t *prow = a.prow + mat * a.n_stride + d.c1;
-        for (int j = 0; j < kb; j++) {
-            int p = s_ipiv[j];
-            if (p != j) { int t = prow[k + j]; prow[k + j] = prow[k + p]; prow[k + p] = t; }
-        }
-    }
-    // swaps outside the panel, then the U-row solve for columns right of the panel
-    for (int c = tid; c < nf; c += NT) {
-        if (c >= k && c < k + kb) continue;
+    if (w.y == 0) {
+        const int cnt = s_cnt;
+        int c = w.z + tid;
+        if (c >= k) c += kb;  // skip the panel columns
+        if (c >= nf) return;
         T *colp = F + (int64_t)c * nf + k;
-        for (int j = 0; j < kb; j++) {
-            int p = s_ipiv[j];
-            if (p != j) { T t = colp[j]; colp[j] = colp[p]; colp[p] = t; }
-        }
-        if (c >= k + kb) {
-            T y[KB];
+        T v[2 * KB];
 #pragma unroll
-            for (int j = 0; j < KB; j++) y[j] = j < kb ? colp[j] : T(0);
+        for (int q = 0; q < 2 * KB; q++) v[q] = q < cnt ? colp[s_src[q]] : T(0);
+        if (c >= k + kb) {
 #pragma unroll
             for (int j = 0; j < KB; j++)
 #pragma unroll
                 for (int i = 0; i < KB; i++)
-                    if (i > j) y[i] -= s_blk[i + j * KB] * y[j];
-#pragma unroll
-            for (int j = 0; j < KB; j++)
-                if (j < kb) colp[j] = y[j];
+                    if (i > j) v[i] -= s_blk[i + j * KB] * v[j];
         }
-    }
-    // L21 columns of the panel: rows np..nf-1
-    for (int r = np + tid; r < nf; r += NT) {
+#pragma unroll
+        for (int q = 0; q < 2 * KB; q++)
+            if (q < cnt) colp[s_dst[q]] = v[q];
+    } else {
+        const int r = np + w.z + tid;
+        if (r >= nf) return;
         T y[KB];
 #pragma unroll
         for (int j = 0; j < KB; j++) y[j] = j < kb ? F[r + (int64_t)(k + j) * nf] : T(0);
@@ -430,10 +479,13 @@ hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int
         case 0: hipLaunchKernelGGL((k_panel<T, 256, 1>), grid, dim3(256), 0, st, a, L.k); break;
         case 1: hipLaunchKernelGGL((k_panel<T, 256, 2>), grid, dim3(256), 0, st, a, L.k); break;
         case 2: hipLaunchKernelGGL((k_panel<T, 256, 4>), grid, dim3(256), 0, st, a, L.k); break;
-        case 3: hipLaunchKernelGGL((k_panel<T, 1024, 2>), grid, dim3(1024), 0, st, a, L.k); break;
-        case 4: hipLaunchKernelGGL((k_panel<T, 1024, 3>), grid, dim3(1024), 0, st, a, L.k); break;
+        case 3: hipLaunchKernelGGL((k_panel<T, 256, 8>), grid, dim3(256), 0, st, a, L.k); break;
+        case 4: hipLaunchKernelGGL((k_panel<T, 512, 6>), grid, dim3(512), 0, st, a, L.k); break;
         default: return hipErrorInvalidValue;
         }
+        break;
+    case L_PANEL_APPLY:
+        hipLaunchKernelGGL(k_panel_apply<T>, grid, dim3(256), 0, st, a, L.k);
         break;
     case L_GEMM:
         hipLaunchKernelGGL(k_gemm<T>, grid, dim3(256), 0, st, probs + L.prob_off,

@@ -51,6 +51,7 @@ enum LaunchKind : int32_t {
     L_ASM_COLS = 1,     // one workgroup per large-front column: assemble in LDS
     L_PANEL = 2,        // one workgroup per large front: factor a panel of the pivot block
     L_GEMM = 3,         // grouped C -= A B (panel updates and the Schur update)
+    L_PANEL_APPLY = 4,  // row swaps + U-row solve + L21 solve after a panel (many workgroups)
 };
 
 struct Launch {

@@ -16,6 +16,7 @@
 namespace mf {
 
 constexpr int PANEL_KB = 16;   // panel width of the large-front getrf
+constexpr int PINFO_STRIDE = 1 + 4 * PANEL_KB;  // per-front panel permutation record
 constexpr int GEMM_BM = 64, GEMM_BN = 64, GEMM_BK = 16;
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -61,7 +62,8 @@ struct FactorArgs {
     void *pool;
     int32_t *prow;
     int32_t *info;
-    int64_t pool_stride, ax_stride, rs_stride, n_stride;
+    int32_t *pinfo;             // per-front panel swap composition (PINFO_STRIDE ints each)
+    int64_t pool_stride, ax_stride, rs_stride, n_stride, pinfo_stride;
 };
 
 struct SolveArgs {
@@ -74,8 +76,9 @@ struct SolveArgs {
     void *y;       // forward result in factor-row order (n x nrhs)
     void *x;       // backward result in factor-column order (n x nrhs)
     void *uv;      // update vectors (nupd x nrhs)
+    void *part;    // split-K partial sums of the rectangular products
     int64_t n, nupd, nrhs;
-    int64_t pool_stride, n_stride;
+    int64_t pool_stride, n_stride, part_stride;
 };
 
 // ---- launchers (mf_factor_kernels.hip, mf_solve_kernels.hip) ----
@@ -83,12 +86,11 @@ template <typename T>
 hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int32_t *lists,
                               const GemmProb *probs, const GemmTile *tiles, int nmat, hipStream_t st);
 int panel_variant_rows(int v);
+// which: 0 forward triangular, 1/2 forward rectangular partials/sum, 3/4 backward rectangular
+// partials/sum, 5 backward triangular
 template <typename T>
-hipError_t launch_fwd(const SolveArgs &a, const int32_t *lists, int list_off, int count, int shmem, int nmat,
-                      hipStream_t st);
-template <typename T>
-hipError_t launch_bwd(const SolveArgs &a, const int32_t *lists, int list_off, int count, int shmem, int nmat,
-                      hipStream_t st);
+hipError_t launch_solve_level(int which, const SolveArgs &a, const int32_t *lists, int list_off, int count, int nmat,
+                              hipStream_t st);
 template <typename T>
 hipError_t launch_prep_rhs(const double *B, int64_t ldb, const double *rs, int64_t rs_stride, const int32_t *qf,
                            T *w0, int64_t n, int64_t nrhs, int64_t b_mat_stride, int nmat, hipStream_t st);

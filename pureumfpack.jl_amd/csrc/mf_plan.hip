@@ -23,6 +23,8 @@
 
 using namespace mf;
 
+static constexpr int64_t SOLVE_CK_HOST = 256;  // column chunk of the split-K solve products (k_rect_part)
+
 static thread_local std::string g_err;
 
 static int fail(int code, const std::string &msg)
@@ -86,14 +88,16 @@ struct mf_plan {
     std::vector<SnDev> sn;
     std::vector<Launch> launches;
     std::vector<int32_t> lvl_off, lvl_cnt;  // solve lists per level
-    std::vector<int32_t> lvl_shfwd, lvl_shbwd;
+    // per level: offsets/counts of the forward/backward split-K part and sum item lists
+    std::vector<int32_t> lvl_fpo, lvl_fpc, lvl_fso, lvl_fsc, lvl_bpo, lvl_bpc, lvl_bso, lvl_bsc;
+    int64_t part_slots = 0;  // partial-sum slots (256 rows x nrhs each) the solve needs
     std::vector<double> lvl_lflops, lvl_uflops, lvl_lbytes, lvl_ubytes, lvl_vec;
     // device
     DevBuf d_sn, d_upd, d_rmap, d_child, d_orow, d_ocol, d_osrc, d_ocolptr, d_ecolptr, d_ext, d_arow;
     DevBuf d_lists, d_probs, d_tiles, d_solvelist, d_qf;
     DevBuf d_csr_rp, d_csr_ci, d_csr_vmap;
-    DevBuf d_pool, d_prow, d_info, d_ax, d_rs;
-    DevBuf d_w0, d_y, d_x, d_uv, d_B, d_X, d_R, d_norms, d_anorm;
+    DevBuf d_pool, d_prow, d_info, d_ax, d_rs, d_pinfo;
+    DevBuf d_w0, d_y, d_x, d_uv, d_part, d_B, d_X, d_R, d_norms, d_anorm;
     DevBuf d_Lp, d_Up, d_uoff, d_Li, d_Lx, d_Ui, d_Ux, d_rowfac;
     int64_t solve_nrhs = 0;
     bool factored = false;
@@ -171,6 +175,7 @@ static int launch_class(const Launch &L)
     case L_FUSED_SMALL: return 0;
     case L_ASM_COLS: return 1;
     case L_PANEL: return 2;
+    case L_PANEL_APPLY: return 2;
     default: return L.variant == 1 ? 4 : 3;
     }
 }
@@ -540,7 +545,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                         for (int64_t j = 0; j < kb; j++)
                             Pn.flops += (m - j - 1) + 2.0 * (m - j - 1) * (b - j - 1);
                         Pn.flops += right * b * (b - 1) + (double)nu[s] * b * b;
-                        Pn.bytes += (2.0 * m * b + 2.0 * right * b + 2.0 * (double)nu[s] * b) * elem;
+                        Pn.bytes += 2.0 * m * b * elem;
                     }
                     if (k2 < np[s]) {
                         // rows [k2, nf) x cols [k2, np): pivot-column block incl. L21 rows
@@ -571,6 +576,26 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                 if (Pn.variant < 0) return fail(MF_BADARG, "pivot block too large for the panel kernel");
                 Pn.level = (int32_t)lv;
                 P.launches.push_back(Pn);
+                // swaps, U rows and L21 columns of this panel, spread over many workgroups
+                while (lists.size() & 3) lists.push_back(0);
+                Launch Ap{};
+                Ap.kind = L_PANEL_APPLY;
+                Ap.k = (int32_t)k;
+                Ap.list_off = (int32_t)lists.size();
+                Ap.level = (int32_t)lv;
+                for (int32_t q = Pn.list_off; q < Pn.list_off + Pn.count; q++) {
+                    int64_t s = lists[q];
+                    int64_t kb = std::min<int64_t>(PANEL_KB, np[s] - k);
+                    for (int64_t c0 = 0; c0 < nf[s] - kb; c0 += 256) {
+                        lists.push_back((int32_t)s); lists.push_back(0); lists.push_back((int32_t)c0); lists.push_back(0);
+                    }
+                    for (int64_t r0 = 0; r0 < nu[s]; r0 += 256) {
+                        lists.push_back((int32_t)s); lists.push_back(1); lists.push_back((int32_t)r0); lists.push_back(0);
+                    }
+                    Ap.bytes += 2.0 * ((double)kb * (nf[s] - kb) + (double)nu[s] * kb) * elem;
+                }
+                Ap.count = (int32_t)((lists.size() - Ap.list_off) / 4);
+                if (Ap.count > 0) P.launches.push_back(Ap);
                 add_gemm(pv, (int)lv, 0);
             }
             std::vector<GemmProb> sv;
@@ -608,19 +633,40 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
     }
     // solve lists: all fronts per level
     std::vector<int32_t> slist;
-    P.lvl_off.resize(nlev);
-    P.lvl_cnt.resize(nlev);
-    P.lvl_shfwd.resize(nlev);
-    P.lvl_shbwd.resize(nlev);
+    P.lvl_off.assign(nlev, 0);
+    P.lvl_cnt.assign(nlev, 0);
+    auto emit4 = [&](int32_t a0, int32_t a1, int32_t a2, int32_t a3) {
+        slist.push_back(a0); slist.push_back(a1); slist.push_back(a2); slist.push_back(a3);
+    };
     for (int64_t lv = 0; lv < nlev; lv++) {
         P.lvl_off[lv] = (int32_t)slist.size();
-        int64_t mf = 0, mb = 0;
-        for (int64_t s : bylevel[lv]) {
-            slist.push_back((int32_t)s);
-            mf = std::max(mf, nf[s] + np[s]);
-            mb = std::max(mb, nf[s]);
-        }
+        for (int64_t s : bylevel[lv]) slist.push_back((int32_t)s);
         P.lvl_cnt[lv] = (int32_t)bylevel[lv].size();
+        // forward: rows t < nu of L21 against columns a < np; backward: rows i < np of U12
+        // against columns t < nu.  Partial slots are reused level by level.
+        for (int dir = 0; dir < 2; dir++) {
+            int64_t slot = 0;
+            std::vector<int32_t> sums;
+            while (slist.size() & 3) slist.push_back(0);
+            int32_t po = (int32_t)slist.size();
+            for (int64_t s : bylevel[lv]) {
+                int64_t nrows = dir == 0 ? nu[s] : np[s], ncols = dir == 0 ? np[s] : nu[s];
+                for (int64_t r0 = 0; r0 < nrows; r0 += 256) {
+                    int64_t nch = (ncols + SOLVE_CK_HOST - 1) / SOLVE_CK_HOST;
+                    for (int64_t c = 0; c < nch; c++) emit4((int32_t)s, (int32_t)r0, (int32_t)(c * SOLVE_CK_HOST), (int32_t)(slot + c));
+                    sums.push_back((int32_t)s); sums.push_back((int32_t)r0); sums.push_back((int32_t)nch);
+                    sums.push_back((int32_t)slot);
+                    slot += nch;
+                }
+            }
+            int32_t pc = (int32_t)((slist.size() - po) / 4);
+            int32_t so = (int32_t)slist.size();
+            slist.insert(slist.end(), sums.begin(), sums.end());
+            int32_t sc = (int32_t)(sums.size() / 4);
+            if (dir == 0) { P.lvl_fpo.push_back(po); P.lvl_fpc.push_back(pc); P.lvl_fso.push_back(so); P.lvl_fsc.push_back(sc); }
+            else { P.lvl_bpo.push_back(po); P.lvl_bpc.push_back(pc); P.lvl_bso.push_back(so); P.lvl_bsc.push_back(sc); }
+            P.part_slots = std::max(P.part_slots, slot);
+        }
         double lf = 0, uf = 0, lb = 0, ub = 0, vec = 0;
         for (int64_t s : bylevel[lv]) {
             double p = (double)np[s], f = (double)nf[s];
@@ -635,9 +681,6 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
         P.lvl_lbytes.push_back(lb);
         P.lvl_ubytes.push_back(ub);
         P.lvl_vec.push_back(vec);
-        P.lvl_shfwd[lv] = (int32_t)(mf * elem);
-        P.lvl_shbwd[lv] = (int32_t)(mb * elem);
-        if (P.lvl_shfwd[lv] > 160 * 1024) return fail(MF_BADARG, "front too large for the solve kernels");
     }
     // CSC export offsets (src/multifrontal.jl:73-104, :277-317)
     P.Lp.assign(n + 1, 0);
@@ -705,6 +748,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
     HIPCHK(P.d_pool.alloc((size_t)P.pool_elems * P.esize() * P.nbatch));
     HIPCHK(P.d_prow.alloc((size_t)n * 4 * P.nbatch));
     HIPCHK(P.d_info.alloc((size_t)4 * P.nbatch));
+    HIPCHK(P.d_pinfo.alloc((size_t)4 * PINFO_STRIDE * nsuper * P.nbatch));
     HIPCHK(P.d_ax.alloc((size_t)P.nnz * 8 * P.nbatch));
     HIPCHK(P.d_rs.alloc((size_t)n * 8 * P.nbatch));
     HIPCHK(P.d_anorm.alloc((size_t)8 * P.nbatch));
@@ -787,6 +831,8 @@ static int run_factor(mf_plan &P, const double *d_ax, const double *d_rs, int nm
     a.pool = P.d_pool.p;
     a.prow = P.d_prow.as<int32_t>();
     a.info = P.d_info.as<int32_t>();
+    a.pinfo = P.d_pinfo.as<int32_t>();
+    a.pinfo_stride = (int64_t)PINFO_STRIDE * P.nsuper;
     a.pool_stride = P.pool_elems;
     a.ax_stride = P.nnz;
     a.rs_stride = P.n;
@@ -900,24 +946,42 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
     a.y = P.d_y.p;
     a.x = P.d_x.p;
     a.uv = P.d_uv.p;
+    a.part = P.d_part.p;
     a.n = n;
     a.nupd = P.nupd;
     a.nrhs = nrhs;
     a.pool_stride = P.pool_elems;
     a.n_stride = n;
+    a.part_stride = std::max<int64_t>(P.part_slots, 1) * 256 * nrhs;
     const double e = (double)P.esize();
-    for (int64_t lv = 0; lv < P.nlevels; lv++)
-        HIPCHK(profiled(P, 5, P.lvl_lflops[lv] * nrhs * nmat, (P.lvl_lbytes[lv] + 3.0 * e * P.lvl_vec[lv] * nrhs) * nmat,
-                        [&] {
-                            return launch_fwd<T>(a, P.d_solvelist.as<int32_t>(), P.lvl_off[lv], P.lvl_cnt[lv],
-                                                 P.lvl_shfwd[lv], nmat, P.stream);
-                        }));
-    for (int64_t lv = P.nlevels - 1; lv >= 0; lv--)
-        HIPCHK(profiled(P, 6, P.lvl_uflops[lv] * nrhs * nmat, (P.lvl_ubytes[lv] + 3.0 * e * P.lvl_vec[lv] * nrhs) * nmat,
-                        [&] {
-                            return launch_bwd<T>(a, P.d_solvelist.as<int32_t>(), P.lvl_off[lv], P.lvl_cnt[lv],
-                                                 P.lvl_shbwd[lv], nmat, P.stream);
-                        }));
+    const int32_t *sl = P.d_solvelist.as<int32_t>();
+    for (int64_t lv = 0; lv < P.nlevels; lv++) {
+        const double vb = 3.0 * e * P.lvl_vec[lv] * nrhs;
+        HIPCHK(profiled(P, 5, 0.0, vb * nmat, [&] {
+            return launch_solve_level<T>(0, a, sl, P.lvl_off[lv], P.lvl_cnt[lv], nmat, P.stream);
+        }));
+        if (P.lvl_fpc[lv] > 0) {
+            HIPCHK(profiled(P, 5, P.lvl_lflops[lv] * nrhs * nmat, P.lvl_lbytes[lv] * nmat, [&] {
+                return launch_solve_level<T>(1, a, sl, P.lvl_fpo[lv], P.lvl_fpc[lv], nmat, P.stream);
+            }));
+            HIPCHK(profiled(P, 5, 0.0, 0.0, [&] {
+                return launch_solve_level<T>(2, a, sl, P.lvl_fso[lv], P.lvl_fsc[lv], nmat, P.stream);
+            }));
+        }
+    }
+    for (int64_t lv = P.nlevels - 1; lv >= 0; lv--) {
+        const double vb = 3.0 * e * P.lvl_vec[lv] * nrhs;
+        if (P.lvl_bpc[lv] > 0)
+            HIPCHK(profiled(P, 6, P.lvl_uflops[lv] * nrhs * nmat, (P.lvl_ubytes[lv] + vb) * nmat, [&] {
+                return launch_solve_level<T>(3, a, sl, P.lvl_bpo[lv], P.lvl_bpc[lv], nmat, P.stream);
+            }));
+        HIPCHK(profiled(P, 6, 0.0, 0.0, [&] {
+            return launch_solve_level<T>(4, a, sl, P.lvl_bso[lv], P.lvl_bsc[lv], nmat, P.stream);
+        }));
+        HIPCHK(profiled(P, 6, 0.0, 0.0, [&] {
+            return launch_solve_level<T>(5, a, sl, P.lvl_off[lv], P.lvl_cnt[lv], nmat, P.stream);
+        }));
+    }
     HIPCHK(launch_finish_x<T>(P.d_x.as<T>(), P.d_qf.as<int32_t>(), P.d_X.as<double>(), n, n, nrhs, accumulate,
                               n * nrhs, nmat, P.stream));
     return MF_OK;
@@ -1005,6 +1069,7 @@ static int ensure_solve_buffers(mf_plan *P, int64_t nrhs, int nmat)
     HIPCHK(P->d_y.alloc(nv * e));
     HIPCHK(P->d_x.alloc(nv * e));
     HIPCHK(P->d_uv.alloc((size_t)std::max<int64_t>(P->nupd, 1) * nrhs * nmat * e));
+    HIPCHK(P->d_part.alloc((size_t)std::max<int64_t>(P->part_slots, 1) * 256 * nrhs * nmat * e));
     HIPCHK(P->d_B.alloc(nv * 8));
     HIPCHK(P->d_X.alloc(nv * 8));
     HIPCHK(P->d_R.alloc(nv * 8));

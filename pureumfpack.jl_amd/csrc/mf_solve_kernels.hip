@@ -24,102 +24,263 @@ hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st)
     return hipGetLastError();
 }
 
+constexpr int SOLVE_RC = 8;   // right-hand sides processed together per pass
+constexpr int SOLVE_NB = 64;  // triangular block size (one wave solves a diagonal block)
+
+// acc[r] += sum_{j < bs} row[j * ld] * s[r * SOLVE_NB + j]: one matrix row against a staged
+// block of SOLVE_RC vectors; 16 independent loads in flight per thread.
+// v[(r0 + r) * vld + i] -= sum_{j < bs} M[i + j * ld] * s[r * SOLVE_NB + j] for rows
+// i in [rbeg, rend), two rows per thread in flight.
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd(SolveArgs a)
+__device__ __forceinline__ void rows_update(const T *M, int ld, int bs, int rbeg, int rend, const T *s, T *v,
+                                            int64_t vld, int rc, int r0)
 {
-    extern __shared__ unsigned char smem_raw[];
-    const int mat = blockIdx.y;
-    const SnDev d = a.sn[a.list[blockIdx.x]];
-    const int nf = d.nf, np = d.np, nu = d.nu;
-    T *fv = reinterpret_cast<T *>(smem_raw);
-    T *ys = fv + nf;
-    const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
-    const int32_t *prow = a.prow + mat * a.n_stride;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int64_t vb = (int64_t)mat * a.nrhs;
-    for (int rhs = 0; rhs < a.nrhs; rhs++) {
-        const T *w0 = reinterpret_cast<const T *>(a.w0) + (vb + rhs) * a.n;
-        T *y = reinterpret_cast<T *>(a.y) + (vb + rhs) * a.n;
-        T *uv = reinterpret_cast<T *>(a.uv) + (vb + rhs) * a.nupd;
-        for (int i = tid; i < nf; i += nt) fv[i] = i < np ? w0[d.c1 + i] : T(0);
-        __syncthreads();
-        for (int ci = 0; ci < d.nchild; ci++) {
-            const SnDev c = a.sn[a.child_list[d.child_off + ci]];
-            const int32_t *rm = a.rmap + c.upd_off;
-            for (int t = tid; t < c.nu; t += nt) fv[rm[t]] += uv[c.upd_off + t];
-            __syncthreads();
-        }
-        for (int i = tid; i < np; i += nt) ys[i] = fv[prow[d.c1 + i] - d.c1];
-        __syncthreads();
-        for (int b0 = 0; b0 < np; b0 += 64) {
-            const int bs = min(64, np - b0);
-            if (tid < 64) {
-                T v = tid < bs ? ys[b0 + tid] : T(0);
-                for (int j = 0; j < bs; j++) {
-                    T vj = __shfl(v, j);
-                    if (tid > j && tid < bs) v -= F[(b0 + tid) + (int64_t)(b0 + j) * nf] * vj;
+    const int nt = blockDim.x;
+    for (int i0 = rbeg + threadIdx.x; i0 < rend; i0 += 2 * nt) {
+        const int i1 = i0 + nt;
+        T acc0[SOLVE_RC] = {}, acc1[SOLVE_RC] = {};
+        for (int j0 = 0; j0 < bs; j0 += 16) {
+            T l0[16], l1[16];
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const bool ok = j0 + q < bs;
+                l0[q] = ok ? M[i0 + (int64_t)(j0 + q) * ld] : T(0);
+                l1[q] = (ok && i1 < rend) ? M[i1 + (int64_t)(j0 + q) * ld] : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < 16; q++)
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) {
+                    const T sv = s[r * SOLVE_NB + j0 + q];
+                    acc0[r] += l0[q] * sv;
+                    acc1[r] += l1[q] * sv;
                 }
-                if (tid < bs) ys[b0 + tid] = v;
-            }
-            __syncthreads();
-            for (int r = b0 + bs + tid; r < nf; r += nt) {
-                T acc = T(0);
-                for (int j = 0; j < bs; j++) acc += F[r + (int64_t)(b0 + j) * nf] * ys[b0 + j];
-                if (r < np) ys[r] -= acc;
-                else fv[r] -= acc;
-            }
-            __syncthreads();
         }
-        for (int i = tid; i < np; i += nt) y[d.c1 + i] = ys[i];
-        for (int t = tid; t < nu; t += nt) uv[d.upd_off + t] = fv[np + t];
-        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < SOLVE_RC; r++)
+            if (r < rc) {
+                v[(r0 + r) * vld + i0] -= acc0[r];
+                if (i1 < rend) v[(r0 + r) * vld + i1] -= acc1[r];
+            }
     }
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd(SolveArgs a)
+__device__ __forceinline__ void panel_dot(const T *row, int ld, int bs, const T *s, T (&acc)[SOLVE_RC])
 {
-    extern __shared__ unsigned char smem_raw[];
+    int j = 0;
+    for (; j + 16 <= bs; j += 16) {
+        T l[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) l[q] = row[(int64_t)(j + q) * ld];
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) acc[r] += l[q] * s[r * SOLVE_NB + j + q];
+    }
+    for (; j < bs; j++) {
+        const T l = row[(int64_t)j * ld];
+#pragma unroll
+        for (int r = 0; r < SOLVE_RC; r++) acc[r] += l * s[r * SOLVE_NB + j];
+    }
+}
+
+// Forward sweep, part 1 (one workgroup per supernode of the level): gather the children's
+// update vectors into this supernode's pivot rows (kept in w0, V-row order) and update rows
+// (uv, this supernode's slot), permute the pivot part by the block pivoting into y (factor
+// order), then solve with the unit lower L11 block by block.  The L21 product that finishes
+// the update vector is k_fwd_rect.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_tri(SolveArgs a)
+{
+    __shared__ T sblk[SOLVE_NB * SOLVE_RC];
+    __shared__ T sdiag[SOLVE_NB * (SOLVE_NB + 1)];
     const int mat = blockIdx.y;
     const SnDev d = a.sn[a.list[blockIdx.x]];
-    const int nf = d.nf, np = d.np, nu = d.nu;
-    T *xs = reinterpret_cast<T *>(smem_raw);
-    T *xu = xs + np;
+    const int nf = d.nf, np = d.np, nu = d.nu, c1 = d.c1;
     const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
-    const int tid = threadIdx.x, nt = blockDim.x;
+    const int32_t *prow = a.prow + mat * a.n_stride;
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
     const int64_t vb = (int64_t)mat * a.nrhs;
-    for (int rhs = 0; rhs < a.nrhs; rhs++) {
-        const T *y = reinterpret_cast<const T *>(a.y) + (vb + rhs) * a.n;
-        T *x = reinterpret_cast<T *>(a.x) + (vb + rhs) * a.n;
-        for (int t = tid; t < nu; t += nt) xu[t] = x[a.upd[d.upd_off + t]];
-        __syncthreads();
-        for (int i = tid; i < np; i += nt) {
-            T acc = y[d.c1 + i];
-            for (int t = 0; t < nu; t++) acc -= F[i + (int64_t)(np + t) * nf] * xu[t];
-            xs[i] = acc;
+    T *w0 = reinterpret_cast<T *>(a.w0) + vb * a.n;
+    T *y = reinterpret_cast<T *>(a.y) + vb * a.n;
+    T *uv = reinterpret_cast<T *>(a.uv) + vb * a.nupd;
+    const int nrhs = (int)a.nrhs;
+    for (int e = tid; e < nu * nrhs; e += nt) uv[(e / nu) * a.nupd + d.upd_off + e % nu] = T(0);
+    __syncthreads();
+    for (int ci = 0; ci < d.nchild; ci++) {
+        const SnDev c = a.sn[a.child_list[d.child_off + ci]];
+        const int32_t *rm = a.rmap + c.upd_off;
+        for (int e = tid; e < c.nu * nrhs; e += nt) {
+            const int t = e % c.nu, r = e / c.nu;
+            const T v = uv[r * a.nupd + c.upd_off + t];
+            const int pos = rm[t];
+            if (pos < np) w0[r * a.n + c1 + pos] += v;
+            else uv[r * a.nupd + d.upd_off + pos - np] += v;
         }
         __syncthreads();
-        for (int b0 = ((np - 1) / 64) * 64; b0 >= 0; b0 -= 64) {
-            const int bs = min(64, np - b0);
+    }
+    for (int e = tid; e < np * nrhs; e += nt) {
+        const int i = e % np, r = e / np;
+        y[r * a.n + c1 + i] = w0[r * a.n + prow[c1 + i]];
+    }
+    __syncthreads();
+    for (int r0 = 0; r0 < nrhs; r0 += SOLVE_RC) {
+        const int rc = min(SOLVE_RC, nrhs - r0);
+        for (int b0 = 0; b0 < np; b0 += SOLVE_NB) {
+            const int bs = min(SOLVE_NB, np - b0);
+            for (int e = tid; e < SOLVE_NB * SOLVE_NB; e += nt) {
+                const int i = e % SOLVE_NB, j = e / SOLVE_NB;
+                sdiag[i + j * (SOLVE_NB + 1)] = (i < bs && j < bs) ? F[(b0 + i) + (int64_t)(b0 + j) * nf] : T(0);
+            }
+            __syncthreads();
             if (tid < 64) {
-                T v = tid < bs ? xs[b0 + tid] : T(0);
-                for (int j = bs - 1; j >= 0; j--) {
-                    if (tid == j) v = v / F[(b0 + j) + (int64_t)(b0 + j) * nf];
-                    T vj = __shfl(v, j);
-                    if (tid < j) v -= F[(b0 + tid) + (int64_t)(b0 + j) * nf] * vj;
+                T v[SOLVE_RC];
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++)
+                    v[r] = (r < rc && lane < bs) ? y[(r0 + r) * a.n + c1 + b0 + lane] : T(0);
+                for (int j = 0; j < bs; j++) {
+                    const T l = (lane > j) ? sdiag[lane + j * (SOLVE_NB + 1)] : T(0);
+#pragma unroll
+                    for (int r = 0; r < SOLVE_RC; r++) v[r] -= l * __shfl(v[r], j);
                 }
-                if (tid < bs) xs[b0 + tid] = v;
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) {
+                    sblk[r * SOLVE_NB + lane] = v[r];
+                    if (r < rc && lane < bs) y[(r0 + r) * a.n + c1 + b0 + lane] = v[r];
+                }
             }
             __syncthreads();
-            for (int r = tid; r < b0; r += nt) {
-                T acc = T(0);
-                for (int j = 0; j < bs; j++) acc += F[r + (int64_t)(b0 + j) * nf] * xs[b0 + j];
-                xs[r] -= acc;
-            }
+            rows_update<T>(F + (int64_t)b0 * nf, nf, bs, b0 + bs, np, sblk, y + c1, a.n, rc, r0);
             __syncthreads();
         }
-        for (int i = tid; i < np; i += nt) x[d.c1 + i] = xs[i];
-        __syncthreads();
+    }
+}
+
+// Rectangular products of the sweeps, split over (256-row block) x (SOLVE_CK-column chunk)
+// workgroups.  Each item writes its partial sums for SOLVE_RC right-hand sides at a time to
+// part[slot ...]; k_rect_sum then adds the chunks of a row block in chunk order (the result
+// does not depend on scheduling) and applies them:
+//   forward  (kind 0): uv[t] -= sum_a L21[t, a] y[c1 + a]        rows t < nu, columns a < np
+//   backward (kind 1): x[c1+i] = y[c1+i] - sum_t U12[i, t] x[upd[t]]   rows i < np, columns t < nu
+// Items are int4 {front, row0, col0, slot}; a partial slot holds 256 * nrhs values.
+constexpr int SOLVE_CK = 256;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_rect_part(SolveArgs a, int kind)
+{
+    __shared__ T sv[SOLVE_NB * SOLVE_RC];
+    const int mat = blockIdx.y;
+    const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
+    const SnDev d = a.sn[w.x];
+    const int nf = d.nf, np = d.np, nu = d.nu, c1 = d.c1;
+    const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
+    const int tid = threadIdx.x;
+    const int row = w.y + tid;
+    const int nrows = kind == 0 ? nu : np, ncols = kind == 0 ? np : nu;
+    const int cend = min(w.z + SOLVE_CK, ncols);
+    const int64_t vb = (int64_t)mat * a.nrhs;
+    const T *y = reinterpret_cast<const T *>(a.y) + vb * a.n;
+    const T *x = reinterpret_cast<const T *>(a.x) + vb * a.n;
+    T *part = reinterpret_cast<T *>(a.part) + mat * a.part_stride + (int64_t)w.w * 256 * a.nrhs;
+    const int32_t *upd = a.upd + d.upd_off;
+    // row r of the operand: forward reads L21 row (np + r) over columns a; backward U12 row r
+    // over columns np + t
+    const T *rowp = kind == 0 ? F + (np + row) : F + row + (int64_t)np * nf;
+    const int nrhs = (int)a.nrhs;
+    for (int r0 = 0; r0 < nrhs; r0 += SOLVE_RC) {
+        const int rc = min(SOLVE_RC, nrhs - r0);
+        T acc[SOLVE_RC] = {};
+        for (int j0 = w.z; j0 < cend; j0 += SOLVE_NB) {
+            const int bs = min(SOLVE_NB, cend - j0);
+            __syncthreads();
+            for (int e = tid; e < SOLVE_NB * SOLVE_RC; e += blockDim.x) {
+                const int j = e % SOLVE_NB, r = e / SOLVE_NB;
+                T v = T(0);
+                if (j < bs && r < rc) v = kind == 0 ? y[(r0 + r) * a.n + c1 + j0 + j] : x[(r0 + r) * a.n + upd[j0 + j]];
+                sv[e] = v;
+            }
+            __syncthreads();
+            if (row < nrows) panel_dot<T>(rowp + (int64_t)j0 * nf, nf, bs, sv, acc);
+        }
+#pragma unroll
+        for (int r = 0; r < SOLVE_RC; r++)
+            if (r < rc) part[(int64_t)(r0 + r) * 256 + tid] = acc[r];
+    }
+}
+
+// Items int4 {front, row0, nchunks, slot0}; chunk c of the row block is partial slot slot0 + c.
+template <typename T>
+__global__ __launch_bounds__(256) void k_rect_sum(SolveArgs a, int kind)
+{
+    const int mat = blockIdx.y;
+    const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
+    const SnDev d = a.sn[w.x];
+    const int np = d.np, nu = d.nu, c1 = d.c1;
+    const int row = w.y + threadIdx.x;
+    const int nrows = kind == 0 ? nu : np;
+    if (row >= nrows) return;
+    const int64_t vb = (int64_t)mat * a.nrhs;
+    const T *y = reinterpret_cast<const T *>(a.y) + vb * a.n;
+    T *x = reinterpret_cast<T *>(a.x) + vb * a.n;
+    T *uv = reinterpret_cast<T *>(a.uv) + vb * a.nupd;
+    const T *part = reinterpret_cast<const T *>(a.part) + mat * a.part_stride + (int64_t)w.w * 256 * a.nrhs;
+    const int64_t cs = (int64_t)256 * a.nrhs;
+    for (int r = 0; r < (int)a.nrhs; r++) {
+        T s = T(0);
+        for (int c = 0; c < w.z; c++) s += part[c * cs + (int64_t)r * 256 + threadIdx.x];
+        if (kind == 0) uv[r * a.nupd + d.upd_off + row] -= s;
+        else x[r * a.n + c1 + row] = y[r * a.n + c1 + row] - s;
+    }
+}
+
+// Backward sweep, part 2 (one workgroup per supernode): x_s <- U11^-1 x_s, bottom block first.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_tri(SolveArgs a)
+{
+    __shared__ T sblk[SOLVE_NB * SOLVE_RC];
+    __shared__ T sdiag[SOLVE_NB * (SOLVE_NB + 1)];
+    const int mat = blockIdx.y;
+    const SnDev d = a.sn[a.list[blockIdx.x]];
+    const int nf = d.nf, np = d.np, c1 = d.c1;
+    const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
+    const int64_t vb = (int64_t)mat * a.nrhs;
+    T *x = reinterpret_cast<T *>(a.x) + vb * a.n;
+    const int nrhs = (int)a.nrhs;
+    for (int r0 = 0; r0 < nrhs; r0 += SOLVE_RC) {
+        const int rc = min(SOLVE_RC, nrhs - r0);
+        for (int b0 = ((np - 1) / SOLVE_NB) * SOLVE_NB; b0 >= 0; b0 -= SOLVE_NB) {
+            const int bs = min(SOLVE_NB, np - b0);
+            for (int e = tid; e < SOLVE_NB * SOLVE_NB; e += nt) {
+                const int i = e % SOLVE_NB, j = e / SOLVE_NB;
+                sdiag[i + j * (SOLVE_NB + 1)] = (i < bs && j < bs) ? F[(b0 + i) + (int64_t)(b0 + j) * nf] : T(0);
+            }
+            __syncthreads();
+            if (tid < 64) {
+                T v[SOLVE_RC];
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++)
+                    v[r] = (r < rc && lane < bs) ? x[(r0 + r) * a.n + c1 + b0 + lane] : T(0);
+                for (int j = bs - 1; j >= 0; j--) {
+                    const T ujj = sdiag[j + j * (SOLVE_NB + 1)];
+                    const T u = (lane < j) ? sdiag[lane + j * (SOLVE_NB + 1)] : T(0);
+#pragma unroll
+                    for (int r = 0; r < SOLVE_RC; r++) {
+                        if (lane == j) v[r] = v[r] / ujj;
+                        v[r] -= u * __shfl(v[r], j);
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) {
+                    sblk[r * SOLVE_NB + lane] = v[r];
+                    if (r < rc && lane < bs) x[(r0 + r) * a.n + c1 + b0 + lane] = v[r];
+                }
+            }
+            __syncthreads();
+            rows_update<T>(F + (int64_t)b0 * nf, nf, bs, 0, b0, sblk, x + c1, a.n, rc, r0);
+            __syncthreads();
+        }
     }
 }
 
@@ -279,21 +440,21 @@ __global__ __launch_bounds__(256) void k_export(const SnDev *__restrict__ sn, co
 
 // ---- host launchers ----
 template <typename T>
-hipError_t launch_fwd(const SolveArgs &a, const int32_t *lists, int list_off, int count, int shmem, int nmat,
-                      hipStream_t st)
+hipError_t launch_solve_level(int which, const SolveArgs &a, const int32_t *lists, int list_off, int count, int nmat,
+                              hipStream_t st)
 {
     SolveArgs b = a;
     b.list = lists + list_off;
-    hipLaunchKernelGGL(k_fwd<T>, dim3(count, nmat), dim3(256), shmem, st, b);
-    return hipGetLastError();
-}
-template <typename T>
-hipError_t launch_bwd(const SolveArgs &a, const int32_t *lists, int list_off, int count, int shmem, int nmat,
-                      hipStream_t st)
-{
-    SolveArgs b = a;
-    b.list = lists + list_off;
-    hipLaunchKernelGGL(k_bwd<T>, dim3(count, nmat), dim3(256), shmem, st, b);
+    dim3 g(count, nmat), blk(256);
+    switch (which) {
+    case 0: hipLaunchKernelGGL(k_fwd_tri<T>, g, blk, 0, st, b); break;
+    case 1: hipLaunchKernelGGL(k_rect_part<T>, g, blk, 0, st, b, 0); break;
+    case 2: hipLaunchKernelGGL(k_rect_sum<T>, g, blk, 0, st, b, 0); break;
+    case 3: hipLaunchKernelGGL(k_rect_part<T>, g, blk, 0, st, b, 1); break;
+    case 4: hipLaunchKernelGGL(k_rect_sum<T>, g, blk, 0, st, b, 1); break;
+    case 5: hipLaunchKernelGGL(k_bwd_tri<T>, g, blk, 0, st, b); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 template <typename T>
@@ -353,8 +514,7 @@ hipError_t launch_export(const SnDev *sn, int64_t nsuper, const T *pool, const i
 }
 
 #define MF_INST(T)                                                                                                \
-    template hipError_t launch_fwd<T>(const SolveArgs &, const int32_t *, int, int, int, int, hipStream_t);     \
-    template hipError_t launch_bwd<T>(const SolveArgs &, const int32_t *, int, int, int, int, hipStream_t);     \
+    template hipError_t launch_solve_level<T>(int, const SolveArgs &, const int32_t *, int, int, int, hipStream_t); \
     template hipError_t launch_prep_rhs<T>(const double *, int64_t, const double *, int64_t, const int32_t *,   \
                                            T *, int64_t, int64_t, int64_t, int, hipStream_t);                   \
     template hipError_t launch_finish_x<T>(const T *, const int32_t *, double *, int64_t, int64_t, int64_t,     \
