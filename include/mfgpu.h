@@ -48,13 +48,24 @@ enum mf_precision {
 
 enum mf_ordering { MF_ORDER_AMD = 0, MF_ORDER_COLAMD = 1, MF_ORDER_GIVEN = 2 };
 
+enum mf_pivoting {
+    MF_PIVOT_AUTO = 0,    /* large fronts: interchange-free blocks with a multiplier check,
+                             redone with partial pivoting if the check fails */
+    MF_PIVOT_PARTIAL = 1  /* large fronts: always the partial-pivoting panel schedule */
+};
+
 typedef struct mf_options {
     int32_t precision; /* enum mf_precision */
     int32_t small_nf;  /* fronts with nf <= small_nf take the one-workgroup fused kernel;
                           0 = library default */
     int32_t use_graph; /* 1: capture the factor schedule in a hipGraph and replay it */
     int32_t nbatch;    /* matrices sharing the pattern that one plan holds (>= 1) */
-    int32_t reserved[4];
+    int32_t pivoting;  /* enum mf_pivoting */
+    float pivot_tol;   /* MF_PIVOT_AUTO accepts the diagonal while every multiplier satisfies
+                          |l_ij| <= 1/pivot_tol.  0 = default: 1 for MF_FP64 (then the result
+                          is exactly that of partial pivoting, src/multifrontal.jl:248), 0.1 for
+                          the mixed precisions (threshold pivoting) */
+    int32_t reserved[2];
 } mf_options;
 
 typedef struct mf_solve_report {
@@ -142,6 +153,7 @@ int mf_export_factors(mf_plan *P, int64_t *Lcolptr, int64_t *Lrowval, double *Ln
 /* Statistics of the plan: factor flops, Schur-gemm flops, flops in fronts with nf >= 1024,
  * front storage bytes, number of levels, kernel launches per factorization. */
 typedef struct mf_plan_stats {
+    int64_t last_schedule;   /* 0: pivoted panels ran last, 1: interchange-free blocks held */
     double factor_flops;
     double gemm_flops;
     double gemm_flops_big;   /* Schur gemm flops of fronts with nf >= 1024 */

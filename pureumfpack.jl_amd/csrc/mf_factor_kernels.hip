@@ -365,6 +365,125 @@ __global__ __launch_bounds__(256) void k_panel_apply(FactorArgs a, int k)
 }
 
 // ------------------------------------------------------------------------------------------
+// Large fronts, speculative path.  Partial pivoting restricted to the pivot block chooses, at
+// step j, the first row of maximum magnitude among rows j..np-1.  If for every j the updated
+// diagonal a_jj satisfies |a_ij| <= |a_jj| / tau for all candidate rows i > j (tau = 1: the
+// diagonal IS that first maximum), the factorization without row interchanges is the
+// pivoted one.  These kernels factor a 64-column block at a time without interchanges and
+// check that condition on every multiplier; a violation sets a per-matrix flag and the
+// factorization is redone on the pivoted schedule (k_panel / k_panel_apply).
+//   k_spec_diag: one workgroup per front factors the kb x kb diagonal block in LDS.
+//   k_spec_trsm: many workgroups; item int4 {front, kind, start, 0}:
+//     kind 0: rows r in [start, start+256) of [k+kb, nf): L[r, k:k+kb] = A[r, k:k+kb] U_kk^-1,
+//             checking the multiplier bound on the candidate rows r < np;
+//     kind 1: columns c in [start, start+256) of [k+kb, nf): U[k:k+kb, c] = L_kk^-1 A[k:k+kb, c].
+// ------------------------------------------------------------------------------------------
+template <typename T, int KB>
+__global__ __launch_bounds__(64) void k_spec_diag(FactorArgs a, int k, float tau)
+{
+    // one wave: lane i owns row i of the block; the pivot row goes through LDS (a single
+    // wave's LDS accesses complete in issue order, so no barrier is needed)
+    __shared__ T srow[KB];
+    const int mat = blockIdx.y;
+    const SnDev d = a.sn[a.list[blockIdx.x]];
+    const int np = d.np, nf = d.nf;
+    const int kb = min(KB, np - k);
+    const int lane = threadIdx.x;
+    T *F = reinterpret_cast<T *>(a.pool) + mat * a.pool_stride + d.foff;
+    T x[KB];
+#pragma unroll
+    for (int c = 0; c < KB; c++) x[c] = (lane < kb && c < kb) ? F[(k + lane) + (int64_t)(k + c) * nf] : T(0);
+    const T itau = T(1) / T(tau);
+    int bad = 0;
+#pragma unroll
+    for (int j = 0; j < KB; j++) {
+        if (j >= kb) continue;
+        if (lane == j) {
+#pragma unroll
+            for (int c = 0; c < KB; c++)
+                if (c >= j) srow[c] = x[c];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const T piv = srow[j];
+        if (lane > j && lane < kb) {
+            T v = x[j];
+            if (fabs(v) > fabs(piv) * itau) bad = 1;
+            if (piv != T(0)) v = fabs(piv) < Tiny<T>::v() ? v / piv : v * (T(1) / piv);
+            x[j] = v;
+#pragma unroll
+            for (int c = 0; c < KB; c++)
+                if (c > j) x[c] -= v * srow[c];
+        }
+        if (piv == T(0) && lane == 0) atomicMin(a.info + mat, d.c1 + k + j);
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (bad) atomicOr(a.flag + mat, 1);
+    if (lane < kb) {
+#pragma unroll
+        for (int c = 0; c < KB; c++)
+            if (c < kb) F[(k + lane) + (int64_t)(k + c) * nf] = x[c];
+    }
+}
+
+template <typename T, int KB>
+__global__ __launch_bounds__(256) void k_spec_trsm(FactorArgs a, int k, float tau)
+{
+    // M[q + j*KB]: kind 0 uses U_kk (M[q][j] = U(q, j)); kind 1 uses L_kk transposed
+    // (M[q][j] = L(j, q)), so both solve x_j <- (x_j - sum_{q<j} x_q M[q][j]) [/ M[j][j]].
+    __shared__ T sd[KB * KB];
+    const int mat = blockIdx.y;
+    const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
+    const SnDev d = a.sn[w.x];
+    const int np = d.np, nf = d.nf;
+    const int kb = min(KB, np - k);
+    const int tid = threadIdx.x;
+    const bool lrows = w.y == 0;
+    T *F = reinterpret_cast<T *>(a.pool) + mat * a.pool_stride + d.foff;
+    for (int e = tid; e < KB * KB; e += blockDim.x) {
+        const int i = e % KB, j = e / KB;
+        const int r = lrows ? i : j, c = lrows ? j : i;
+        sd[e] = (i < kb && j < kb) ? F[(k + r) + (int64_t)(k + c) * nf] : (i == j ? T(1) : T(0));
+    }
+    __syncthreads();
+    const int idx = k + kb + w.z + tid;   // row (kind 0) or column (kind 1) index in the front
+    if (idx >= nf) return;
+    const bool cand = lrows && idx < np;
+    const T itau = T(1) / T(tau);
+    // kind 0 reads row idx across the block columns (stride nf); kind 1 reads column idx
+    // down the block rows (contiguous)
+    T *base = lrows ? F + idx + (int64_t)k * nf : F + (int64_t)idx * nf + k;
+    const int64_t stride = lrows ? nf : 1;
+    T x[KB];
+#pragma unroll
+    for (int j = 0; j < KB; j++) x[j] = j < kb ? base[j * stride] : T(0);
+    int bad = 0;
+#pragma unroll
+    for (int j = 0; j < KB; j++) {
+        asm volatile("" ::: "memory");  // keep the block in LDS (no hoisting into registers)
+        T v = x[j];
+#pragma unroll
+        for (int q = 0; q < KB; q++)
+            if (q < j) v -= x[q] * sd[q + j * KB];
+        if (lrows) {
+            const T u = sd[j + j * KB];
+            if (cand) {
+                if (fabs(v) > fabs(u) * itau) bad = 1;
+                // getf2 scaling of a candidate row: reciprocal unless the pivot is zero or tiny
+                if (u != T(0)) v = fabs(u) < Tiny<T>::v() ? v / u : v * (T(1) / u);
+            } else {
+                v = v * (T(1) / u);  // trsm R/U/N/N
+            }
+        }
+        x[j] = v;
+    }
+    if (bad) atomicOr(a.flag + mat, 1);
+#pragma unroll
+    for (int j = 0; j < KB; j++)
+        if (j < kb) base[j * stride] = x[j];
+}
+
+// ------------------------------------------------------------------------------------------
 // Grouped gemm C -= A B on MFMA (v_mfma_*_16x16x4): 64x64 tile per 256-thread workgroup,
 // four waves in a 2x2 arrangement, each owning a 32x32 quadrant (2x2 MFMA blocks).  A and B
 // tiles are staged through LDS k-major so the MFMA operand reads are conflict free.
@@ -486,6 +605,12 @@ hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int
         break;
     case L_PANEL_APPLY:
         hipLaunchKernelGGL(k_panel_apply<T>, grid, dim3(256), 0, st, a, L.k);
+        break;
+    case L_SPEC_DIAG:
+        hipLaunchKernelGGL((k_spec_diag<T, SPEC_KB>), grid, dim3(64), 0, st, a, L.k, L.tau);
+        break;
+    case L_SPEC_TRSM:
+        hipLaunchKernelGGL((k_spec_trsm<T, SPEC_KB>), grid, dim3(256), 0, st, a, L.k, L.tau);
         break;
     case L_GEMM:
         hipLaunchKernelGGL(k_gemm<T>, grid, dim3(256), 0, st, probs + L.prob_off,

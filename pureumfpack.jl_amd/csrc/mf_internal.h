@@ -52,6 +52,8 @@ enum LaunchKind : int32_t {
     L_PANEL = 2,        // one workgroup per large front: factor a panel of the pivot block
     L_GEMM = 3,         // grouped C -= A B (panel updates and the Schur update)
     L_PANEL_APPLY = 4,  // row swaps + U-row solve + L21 solve after a panel (many workgroups)
+    L_SPEC_DIAG = 5,    // speculative schedule: factor the diagonal block without interchanges
+    L_SPEC_TRSM = 6,    // speculative schedule: L rows / U columns of the block + pivot check
 };
 
 struct Launch {
@@ -63,6 +65,7 @@ struct Launch {
     int32_t prob_off;   // L_GEMM: offset into problems; tiles at list_off in tile array
     int32_t shmem;      // dynamic LDS bytes
     int32_t level;
+    float tau;          // speculative schedule: multiplier bound 1/tau
     double flops;       // algorithmic flops of the launch (host-side bookkeeping)
     double bytes;       // algorithmic HBM bytes of the launch
 };

@@ -15,7 +15,8 @@
 
 namespace mf {
 
-constexpr int PANEL_KB = 16;   // panel width of the large-front getrf
+constexpr int PANEL_KB = 16;   // panel width of the large-front getrf (pivoted schedule)
+constexpr int SPEC_KB = 32;    // block width of the speculative (interchange-free) schedule
 constexpr int PINFO_STRIDE = 1 + 4 * PANEL_KB;  // per-front panel permutation record
 constexpr int GEMM_BM = 64, GEMM_BN = 64, GEMM_BK = 16;
 
@@ -63,6 +64,7 @@ struct FactorArgs {
     int32_t *prow;
     int32_t *info;
     int32_t *pinfo;             // per-front panel swap composition (PINFO_STRIDE ints each)
+    int32_t *flag;              // per-matrix: set when the speculative schedule needs pivoting
     int64_t pool_stride, ax_stride, rs_stride, n_stride, pinfo_stride;
 };
 

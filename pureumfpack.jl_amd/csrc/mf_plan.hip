@@ -86,7 +86,12 @@ struct mf_plan {
     // host copies
     std::vector<int64_t> qf, Lp, Up, uoff;
     std::vector<SnDev> sn;
-    std::vector<Launch> launches;
+    std::vector<Launch> sched[2];   // 0: partial-pivoting panels, 1: interchange-free blocks
+    bool has_large = false;
+    int pivoting = MF_PIVOT_AUTO;
+    float pivot_tol = 0.f;
+    bool spec_ok = true;            // the last interchange-free attempt held
+    int last_sched = 0;
     std::vector<int32_t> lvl_off, lvl_cnt;  // solve lists per level
     // per level: offsets/counts of the forward/backward split-K part and sum item lists
     std::vector<int32_t> lvl_fpo, lvl_fpc, lvl_fso, lvl_fsc, lvl_bpo, lvl_bpc, lvl_bso, lvl_bsc;
@@ -96,7 +101,7 @@ struct mf_plan {
     DevBuf d_sn, d_upd, d_rmap, d_child, d_orow, d_ocol, d_osrc, d_ocolptr, d_ecolptr, d_ext, d_arow;
     DevBuf d_lists, d_probs, d_tiles, d_solvelist, d_qf;
     DevBuf d_csr_rp, d_csr_ci, d_csr_vmap;
-    DevBuf d_pool, d_prow, d_info, d_ax, d_rs, d_pinfo;
+    DevBuf d_pool, d_prow, d_info, d_ax, d_rs, d_pinfo, d_flag;
     DevBuf d_w0, d_y, d_x, d_uv, d_part, d_B, d_X, d_R, d_norms, d_anorm;
     DevBuf d_Lp, d_Up, d_uoff, d_Li, d_Lx, d_Ui, d_Ux, d_rowfac;
     int64_t solve_nrhs = 0;
@@ -104,9 +109,9 @@ struct mf_plan {
     int nmat_factored = 0;
     int rs_given = 0;
     hipStream_t stream = nullptr;
-    hipGraphExec_t graph = nullptr;
-    int graph_nmat = 0;
-    int graph_rs = 0;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_nmat[2] = {0, 0};
+    int graph_rs[2] = {0, 0};
     // profiling (mf_plan_set_profiling)
     int prof = 0;
     double prof_ms[MF_PROF_CLASSES] = {0}, prof_flops[MF_PROF_CLASSES] = {0}, prof_bytes[MF_PROF_CLASSES] = {0};
@@ -119,7 +124,8 @@ struct mf_plan {
     ~mf_plan()
     {
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-        if (graph) (void)hipGraphExecDestroy(graph);
+        for (int g = 0; g < 2; g++)
+            if (graph[g]) (void)hipGraphExecDestroy(graph[g]);
         if (stream) (void)hipStreamDestroy(stream);
     }
     size_t esize() const { return prec == MF_FP64 ? 8 : 4; }
@@ -176,6 +182,8 @@ static int launch_class(const Launch &L)
     case L_ASM_COLS: return 1;
     case L_PANEL: return 2;
     case L_PANEL_APPLY: return 2;
+    case L_SPEC_DIAG: return 2;
+    case L_SPEC_TRSM: return 2;
     default: return L.variant == 1 ? 4 : 3;
     }
 }
@@ -472,6 +480,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
     std::vector<GemmProb> probs;
     std::vector<GemmTile> tiles;
     const double ebytes = (double)P.esize();
+    std::vector<Launch> *gemm_dst = nullptr;
     auto add_gemm = [&](std::vector<GemmProb> &pv, int lvl, int schur) {
         if (pv.empty()) return;
         Launch L{};
@@ -491,7 +500,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
         }
         L.count = (int32_t)(tiles.size() - L.list_off);
         L.level = lvl;
-        if (L.count > 0) P.launches.push_back(L);
+        if (L.count > 0) gemm_dst->push_back(L);
     };
     const int elem = (int)P.esize();
     // algorithmic bytes of assembling front s: the front written once, every child
@@ -506,9 +515,13 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
         for (int64_t k = 1; k <= np[s]; k++) g += (double)(np[s] - k) + 2.0 * (double)(np[s] - k) * (double)(np[s] - k);
         return g + u * p * p + u * p * (p - 1) + 2.0 * p * u * u;
     };
+    for (int sp = 0; sp < 2; sp++) {
+    std::vector<Launch> &sched = P.sched[sp];
+    gemm_dst = &sched;
     for (int64_t lv = 0; lv < nlev; lv++) {
         std::vector<int64_t> small, large;
         for (int64_t s : bylevel[lv]) (nf[s] <= P.small_nf ? small : large).push_back(s);
+        if (!large.empty()) P.has_large = true;
         if (!large.empty()) {
             // column assembly (int2 entries: keep 8-byte alignment)
             if (lists.size() & 1) lists.push_back(0);
@@ -525,10 +538,81 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             L.shmem = (int32_t)(mnf * elem);
             L.level = (int32_t)lv;
             if (L.shmem > 160 * 1024) return fail(MF_BADARG, "front too large for column assembly");
-            P.launches.push_back(L);
+            sched.push_back(L);
             int64_t maxnp = 0;
             for (int64_t s : large) maxnp = std::max(maxnp, np[s]);
-            for (int64_t k = 0; k < maxnp; k += PANEL_KB) {
+            if (sp == 1) {
+                // interchange-free blocks of SPEC_KB columns with the multiplier check
+                for (int64_t k = 0; k < maxnp; k += SPEC_KB) {
+                    Launch Dg{};
+                    Dg.kind = L_SPEC_DIAG;
+                    Dg.k = (int32_t)k;
+                    Dg.tau = P.pivot_tol;
+                    Dg.level = (int32_t)lv;
+                    Dg.list_off = (int32_t)lists.size();
+                    std::vector<GemmProb> pv;
+                    std::vector<int32_t> items;
+                    for (int64_t s : large) {
+                        if (np[s] <= k) continue;
+                        lists.push_back((int32_t)s);
+                        int64_t kb = std::min<int64_t>(SPEC_KB, np[s] - k), k2 = k + kb;
+                        double b = (double)kb;
+                        Dg.flops += 2.0 * b * b * b / 3.0;
+                        Dg.bytes += 2.0 * b * b * elem;
+                        for (int64_t r0 = 0; r0 < nf[s] - k2; r0 += 256) {
+                            items.push_back((int32_t)s); items.push_back(0); items.push_back((int32_t)r0); items.push_back(0);
+                        }
+                        for (int64_t c0 = 0; c0 < nf[s] - k2; c0 += 256) {
+                            items.push_back((int32_t)s); items.push_back(1); items.push_back((int32_t)c0); items.push_back(0);
+                        }
+                        if (k2 < np[s]) {
+                            GemmProb g{};
+                            g.aoff = foff[s] + k2 + k * nf[s];
+                            g.boff = foff[s] + k + k2 * nf[s];
+                            g.coff = foff[s] + k2 + k2 * nf[s];
+                            g.m = (int32_t)(nf[s] - k2);
+                            g.n = (int32_t)(np[s] - k2);
+                            g.k = (int32_t)kb;
+                            g.lda = g.ldb = g.ldc = (int32_t)nf[s];
+                            pv.push_back(g);
+                            if (nu[s] > 0) {
+                                GemmProb h{};
+                                h.aoff = foff[s] + k2 + k * nf[s];
+                                h.boff = foff[s] + k + np[s] * nf[s];
+                                h.coff = foff[s] + k2 + np[s] * nf[s];
+                                h.m = (int32_t)(np[s] - k2);
+                                h.n = (int32_t)nu[s];
+                                h.k = (int32_t)kb;
+                                h.lda = h.ldb = h.ldc = (int32_t)nf[s];
+                                pv.push_back(h);
+                            }
+                        }
+                    }
+                    Dg.count = (int32_t)(lists.size() - Dg.list_off);
+                    sched.push_back(Dg);
+                    if (!items.empty()) {
+                        while (lists.size() & 3) lists.push_back(0);
+                        Launch Tr{};
+                        Tr.kind = L_SPEC_TRSM;
+                        Tr.k = (int32_t)k;
+                        Tr.tau = P.pivot_tol;
+                        Tr.level = (int32_t)lv;
+                        Tr.list_off = (int32_t)lists.size();
+                        lists.insert(lists.end(), items.begin(), items.end());
+                        Tr.count = (int32_t)(items.size() / 4);
+                        for (size_t q = 0; q < items.size(); q += 4) {
+                            int64_t s = items[q];
+                            double b = (double)std::min<int64_t>(SPEC_KB, np[s] - k);
+                            double cnt = (double)std::min<int64_t>(256, nf[s] - (k + (int64_t)b) - items[q + 2]);
+                            Tr.flops += cnt * b * b;
+                            Tr.bytes += 2.0 * cnt * b * elem;
+                        }
+                        sched.push_back(Tr);
+                    }
+                    add_gemm(pv, (int)lv, 0);
+                }
+            }
+            for (int64_t k = 0; sp == 0 && k < maxnp; k += PANEL_KB) {
                 Launch Pn{};
                 Pn.kind = L_PANEL;
                 Pn.k = (int32_t)k;
@@ -575,7 +659,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                 Pn.variant = choose_panel_variant((int)rows);
                 if (Pn.variant < 0) return fail(MF_BADARG, "pivot block too large for the panel kernel");
                 Pn.level = (int32_t)lv;
-                P.launches.push_back(Pn);
+                sched.push_back(Pn);
                 // swaps, U rows and L21 columns of this panel, spread over many workgroups
                 while (lists.size() & 3) lists.push_back(0);
                 Launch Ap{};
@@ -595,7 +679,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                     Ap.bytes += 2.0 * ((double)kb * (nf[s] - kb) + (double)nu[s] * kb) * elem;
                 }
                 Ap.count = (int32_t)((lists.size() - Ap.list_off) / 4);
-                if (Ap.count > 0) P.launches.push_back(Ap);
+                if (Ap.count > 0) sched.push_back(Ap);
                 add_gemm(pv, (int)lv, 0);
             }
             std::vector<GemmProb> sv;
@@ -628,8 +712,9 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             L.shmem = (int32_t)(mnf * mnf * elem + mnp * 4);
             L.level = (int32_t)lv;
             if (L.shmem > 160 * 1024) return fail(MF_BADARG, "small_nf too large for LDS");
-            P.launches.push_back(L);
+            sched.push_back(L);
         }
+    }
     }
     // solve lists: all fronts per level
     std::vector<int32_t> slist;
@@ -749,6 +834,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
     HIPCHK(P.d_prow.alloc((size_t)n * 4 * P.nbatch));
     HIPCHK(P.d_info.alloc((size_t)4 * P.nbatch));
     HIPCHK(P.d_pinfo.alloc((size_t)4 * PINFO_STRIDE * nsuper * P.nbatch));
+    HIPCHK(P.d_flag.alloc((size_t)4 * P.nbatch));
     HIPCHK(P.d_ax.alloc((size_t)P.nnz * 8 * P.nbatch));
     HIPCHK(P.d_rs.alloc((size_t)n * 8 * P.nbatch));
     HIPCHK(P.d_anorm.alloc((size_t)8 * P.nbatch));
@@ -773,7 +859,13 @@ extern "C" int mf_plan_create(int64_t n, const int64_t *qf, int64_t nsuper, cons
         if (opt->small_nf > 0) P->small_nf = opt->small_nf;
         P->use_graph = opt->use_graph;
         P->nbatch = opt->nbatch > 0 ? opt->nbatch : 1;
+        if (opt->pivoting != MF_PIVOT_AUTO && opt->pivoting != MF_PIVOT_PARTIAL)
+            return fail(MF_BADARG, "unknown pivoting mode");
+        P->pivoting = opt->pivoting;
+        if (opt->pivot_tol < 0.f || opt->pivot_tol > 1.f) return fail(MF_BADARG, "pivot_tol must be in [0, 1]");
+        P->pivot_tol = opt->pivot_tol;
     }
+    if (P->pivot_tol == 0.f) P->pivot_tol = P->prec == MF_FP64 ? 1.f : 0.1f;
     if (P->small_nf > 128) return fail(MF_BADARG, "small_nf must be <= 128");
     HIPCHK(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
     int rc;
@@ -806,14 +898,15 @@ extern "C" int mf_plan_get_stats(const mf_plan *P, mf_plan_stats *st)
     st->front_bytes = (double)P->pool_elems * (double)P->esize();
     st->nsuper = P->nsuper;
     st->nlevels = P->nlevels;
-    st->launches = (int64_t)P->launches.size();
+    st->launches = (int64_t)P->sched[P->last_sched].size();
+    st->last_schedule = P->last_sched;
     st->max_nf = P->maxnf;
     return MF_OK;
 }
 
 // ------------------------------------------------------------------ factor
 template <typename T>
-static int run_factor(mf_plan &P, const double *d_ax, const double *d_rs, int nmat)
+static int run_factor(mf_plan &P, const double *d_ax, const double *d_rs, int nmat, int sp)
 {
     FactorArgs a{};
     a.sn = P.d_sn.as<SnDev>();
@@ -832,13 +925,15 @@ static int run_factor(mf_plan &P, const double *d_ax, const double *d_rs, int nm
     a.prow = P.d_prow.as<int32_t>();
     a.info = P.d_info.as<int32_t>();
     a.pinfo = P.d_pinfo.as<int32_t>();
+    a.flag = P.d_flag.as<int32_t>();
     a.pinfo_stride = (int64_t)PINFO_STRIDE * P.nsuper;
     a.pool_stride = P.pool_elems;
     a.ax_stride = P.nnz;
     a.rs_stride = P.n;
     a.n_stride = P.n;
     HIPCHK(launch_fill_i32(P.d_info.as<int32_t>(), INT32_MAX, nmat, P.stream));
-    for (const Launch &L : P.launches)
+    HIPCHK(launch_fill_i32(P.d_flag.as<int32_t>(), 0, nmat, P.stream));
+    for (const Launch &L : P.sched[sp])
         HIPCHK(profiled(P, launch_class(L), L.flops * nmat, L.bytes * nmat, [&] {
             return launch_factor_step<T>(L, a, P.d_lists.as<int32_t>(), P.d_probs.as<GemmProb>(),
                                          P.d_tiles.as<GemmTile>(), nmat, P.stream);
@@ -848,34 +943,58 @@ static int run_factor(mf_plan &P, const double *d_ax, const double *d_rs, int nm
     return MF_OK;
 }
 
+// Enqueues schedule sp (directly or through its captured graph).
+static int enqueue_schedule(mf_plan *P, const double *d_ax, const double *d_rs, int nmat, int sp)
+{
+    const int rsg = d_rs != nullptr;
+    if (P->use_graph && !P->prof) {
+        // inputs are always read from the plan's own buffers, so a captured graph stays valid
+        if (!P->graph[sp] || P->graph_nmat[sp] != nmat || rsg != P->graph_rs[sp]) {
+            if (P->graph[sp]) { (void)hipGraphExecDestroy(P->graph[sp]); P->graph[sp] = nullptr; }
+            hipGraph_t g;
+            HIPCHK(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
+            int rc = P->prec == MF_FP64 ? run_factor<double>(*P, d_ax, d_rs, nmat, sp)
+                                        : run_factor<float>(*P, d_ax, d_rs, nmat, sp);
+            hipError_t ec = hipStreamEndCapture(P->stream, &g);
+            if (rc != MF_OK) return rc;
+            HIPCHK(ec);
+            hipError_t ei = hipGraphInstantiate(&P->graph[sp], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPCHK(ei);
+            P->graph_nmat[sp] = nmat;
+            P->graph_rs[sp] = rsg;
+        }
+        HIPCHK(hipGraphLaunch(P->graph[sp], P->stream));
+        return MF_OK;
+    }
+    return P->prec == MF_FP64 ? run_factor<double>(*P, d_ax, d_rs, nmat, sp)
+                              : run_factor<float>(*P, d_ax, d_rs, nmat, sp);
+}
+
 static int factor_common(mf_plan *P, const double *d_ax, const double *d_rs, int nmat, int32_t check,
                          int64_t *info_col)
 {
     int rc;
-    const int rsg = d_rs != nullptr;
-    if (P->use_graph && !P->prof) {
-        // inputs are always read from the plan's own buffers, so the captured graph stays valid
-        if (!P->graph || P->graph_nmat != nmat || rsg != P->graph_rs) {
-            if (P->graph) { (void)hipGraphExecDestroy(P->graph); P->graph = nullptr; }
-            hipGraph_t g;
-            HIPCHK(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
-            rc = P->prec == MF_FP64 ? run_factor<double>(*P, d_ax, d_rs, nmat) : run_factor<float>(*P, d_ax, d_rs, nmat);
-            hipError_t ec = hipStreamEndCapture(P->stream, &g);
-            if (rc != MF_OK) return rc;
-            HIPCHK(ec);
-            hipError_t ei = hipGraphInstantiate(&P->graph, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            HIPCHK(ei);
-            P->graph_nmat = nmat;
-            P->graph_rs = rsg;
+    std::vector<int32_t> info(nmat), flag(nmat, 0);
+    // Interchange-free schedule first when it applies; if any multiplier broke the bound,
+    // the pivoted schedule reruns the whole factorization from the assembly on.
+    bool spec = P->has_large && P->pivoting == MF_PIVOT_AUTO && P->spec_ok;
+    if (spec) {
+        if ((rc = enqueue_schedule(P, d_ax, d_rs, nmat, 1)) != MF_OK) return rc;
+        HIPCHK(hipMemcpyAsync(flag.data(), P->d_flag.p, (size_t)nmat * 4, hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+        bool bad = false;
+        for (int m = 0; m < nmat; m++) bad |= flag[m] != 0;
+        if (bad) {
+            spec = false;
+            P->spec_ok = false;
         }
-        HIPCHK(hipGraphLaunch(P->graph, P->stream));
-    } else {
-        rc = P->prec == MF_FP64 ? run_factor<double>(*P, d_ax, d_rs, nmat) : run_factor<float>(*P, d_ax, d_rs, nmat);
-        if (rc != MF_OK) return rc;
     }
-    P->rs_given = rsg;
-    std::vector<int32_t> info(nmat);
+    if (!spec) {
+        if ((rc = enqueue_schedule(P, d_ax, d_rs, nmat, 0)) != MF_OK) return rc;
+    }
+    P->last_sched = spec ? 1 : 0;
+    P->rs_given = d_rs != nullptr;
     HIPCHK(hipMemcpyAsync(info.data(), P->d_info.p, (size_t)nmat * 4, hipMemcpyDeviceToHost, P->stream));
     HIPCHK(hipStreamSynchronize(P->stream));
     if (P->prof) HIPCHK(P->ev_collect());

@@ -12,6 +12,7 @@ from ._build import LIB_PATH
 MF_OK, MF_SINGULAR, MF_DIM, MF_BADARG, MF_HIP, MF_OOM = range(6)
 MF_FP64, MF_FP32, MF_BF16 = 0, 1, 2
 MF_ORDER_AMD, MF_ORDER_COLAMD, MF_ORDER_GIVEN = 0, 1, 2
+MF_PIVOT_AUTO, MF_PIVOT_PARTIAL = 0, 1
 
 c_i64p = ctypes.POINTER(ctypes.c_int64)
 c_f64p = ctypes.POINTER(ctypes.c_double)
@@ -20,7 +21,8 @@ c_void_pp = ctypes.POINTER(ctypes.c_void_p)
 
 class mf_options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("small_nf", ctypes.c_int32), ("use_graph", ctypes.c_int32),
-                ("nbatch", ctypes.c_int32), ("reserved", ctypes.c_int32 * 4)]
+                ("nbatch", ctypes.c_int32), ("pivoting", ctypes.c_int32), ("pivot_tol", ctypes.c_float),
+                ("reserved", ctypes.c_int32 * 2)]
 
 
 class mf_solve_report(ctypes.Structure):
@@ -29,7 +31,7 @@ class mf_solve_report(ctypes.Structure):
 
 
 class mf_plan_stats(ctypes.Structure):
-    _fields_ = [("factor_flops", ctypes.c_double), ("gemm_flops", ctypes.c_double),
+    _fields_ = [("last_schedule", ctypes.c_int64), ("factor_flops", ctypes.c_double), ("gemm_flops", ctypes.c_double),
                 ("gemm_flops_big", ctypes.c_double), ("front_bytes", ctypes.c_double),
                 ("nsuper", ctypes.c_int64), ("nlevels", ctypes.c_int64), ("launches", ctypes.c_int64),
                 ("max_nf", ctypes.c_int64)]

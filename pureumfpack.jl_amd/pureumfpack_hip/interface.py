@@ -23,6 +23,7 @@ from ._lib import MF_BADARG, MF_DIM, MF_OK, MF_SINGULAR, c_f64p, c_i64p
 
 SCALE_NONE, SCALE_SUM, SCALE_MAX = 0, 1, 2
 _PREC = {"fp64": _lib.MF_FP64, "fp32": _lib.MF_FP32, "bf16": _lib.MF_BF16}
+_PIVOT = {"auto": _lib.MF_PIVOT_AUTO, "partial": _lib.MF_PIVOT_PARTIAL}
 
 
 class DimensionMismatch(ValueError):
@@ -162,16 +163,20 @@ def symbolic_mf(A, q=None, ordering="amd"):
 class _Plan:
     """Owns an mf_plan (device schedule + front storage); destroyed with the Python object."""
 
-    def __init__(self, A, S: SymbolicMF, precision="fp64", small_nf=0, use_graph=False, nbatch=1):
+    def __init__(self, A, S: SymbolicMF, precision="fp64", small_nf=0, use_graph=False, nbatch=1,
+                 pivoting="auto", pivot_tol=0.0):
         lib = _lib.load()
         if precision not in _PREC:
             raise ArgumentError(f"unknown precision {precision}")
+        if pivoting not in _PIVOT:
+            raise ArgumentError(f"unknown pivoting {pivoting}")
         self.lib = lib
         self.n = A.shape[0]
         self.nnz = A.nnz
         self.S = S
         self.precision = precision
-        opt = _lib.mf_options(_PREC[precision], int(small_nf), int(bool(use_graph)), int(nbatch))
+        opt = _lib.mf_options(_PREC[precision], int(small_nf), int(bool(use_graph)), int(nbatch),
+                              _PIVOT[pivoting], float(pivot_tol))
         Ap, Ai = _pattern_1based(A)
         arrs = [np.ascontiguousarray(x, dtype=np.int64) + 1 for x in
                 (S.qf, S.sstart, S.upd_ptr, S.upd_idx, S.child_ptr, S.child_idx)]
@@ -272,7 +277,7 @@ class PureLU(MFFactorization):
 
 
 def multifrontal_lu(A, q=None, tol=None, check=True, *, precision="fp64", symbolic=None, small_nf=0,
-                    use_graph=False):
+                    use_graph=False, pivoting="auto", pivot_tol=0.0):
     """Numeric multifrontal LU on the GPU (src/multifrontal.jl:146-358).  `tol` is accepted and
     ignored, as in the reference (:121).  Returns an MFFactorization with A[p, q] = L U."""
     A = _csc(A)
@@ -280,13 +285,14 @@ def multifrontal_lu(A, q=None, tol=None, check=True, *, precision="fp64", symbol
     if A.shape[0] != n:
         raise DimensionMismatch("multifrontal_lu requires a square matrix")
     S = symbolic if symbolic is not None else symbolic_mf(A, q=q)
-    plan = _Plan(A, S, precision=precision, small_nf=small_nf, use_graph=use_graph)
+    plan = _Plan(A, S, precision=precision, small_nf=small_nf, use_graph=use_graph, pivoting=pivoting,
+                 pivot_tol=pivot_tol)
     plan.factor(A.data, None, check=check)
     return MFFactorization(plan, None, A)
 
 
 def splu(A, method="gplu", ordering="amd", tol=0.1, scale=SCALE_SUM, check=True, *, precision="fp64",
-         symbolic=None, small_nf=0, use_graph=False):
+         symbolic=None, small_nf=0, use_graph=False, pivoting="auto", pivot_tol=0.0):
     """High-level factorization (src/interface.jl:93-120).  Only method="multifrontal" is served
     by this package; method="gplu" (the reference default, a different algorithm) raises
     NotImplementedError, any other method raises ArgumentError.  `ordering` and `tol` are
@@ -298,7 +304,8 @@ def splu(A, method="gplu", ordering="amd", tol=0.1, scale=SCALE_SUM, check=True,
     if method == "multifrontal":
         Rs = row_scaling(A, scale)
         S = symbolic if symbolic is not None else symbolic_mf(A)
-        plan = _Plan(A, S, precision=precision, small_nf=small_nf, use_graph=use_graph)
+        plan = _Plan(A, S, precision=precision, small_nf=small_nf, use_graph=use_graph, pivoting=pivoting,
+                     pivot_tol=pivot_tol)
         plan.factor(A.data, None if scale == SCALE_NONE else Rs, check=check)
         return PureLU(plan, Rs, A)
     if method == "gplu":

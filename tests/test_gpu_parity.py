@@ -50,14 +50,19 @@ def _check_parity(A, F, R, tol=1e-12):
 
 
 @pytest.mark.parametrize("name", list(CASES))
-@pytest.mark.parametrize("small_nf", [0, 1])
-def test_factor_parity(name, small_nf):
+@pytest.mark.parametrize("small_nf,pivoting", [(0, "auto"), (1, "auto"), (1, "partial")])
+def test_factor_parity(name, small_nf, pivoting):
+    """pivoting="auto" tries the interchange-free block schedule for the large fronts and
+    falls back to the partial-pivoting panels when a multiplier exceeds 1; "partial" always
+    runs the panels.  Both must reproduce the oracle exactly."""
     A = CASES[name]()
-    print(f"[case] {name} small_nf={small_nf} n={A.shape[0]}", flush=True)
+    print(f"[case] {name} small_nf={small_nf} pivoting={pivoting} n={A.shape[0]}", flush=True)
     S = I.symbolic_mf(A)
-    F = I.splu(A, method="multifrontal", symbolic=S, small_nf=small_nf)
+    F = I.splu(A, method="multifrontal", symbolic=S, small_nf=small_nf, pivoting=pivoting)
     R = oracle_splu(A, S)
     _check_parity(A, F, R)
+    if pivoting == "partial":
+        assert F._plan.stats()["last_schedule"] == 0
     As = OS.apply_row_scaling(A, F.Rs)
     rec = abs(As[F.p, :][:, F.q] - F.L @ F.U).max() / abs(As).max()
     assert rec <= 1e-13
@@ -85,6 +90,10 @@ def test_poisson3d_24_parity():
     F = I.splu(A, method="multifrontal", symbolic=S)
     R = oracle_splu(A, S)
     _check_parity(A, F, R)
+    # no interchange is needed on this matrix, so the interchange-free schedule must have held
+    assert F._plan.stats()["last_schedule"] == 1
+    Fp = I.splu(A, method="multifrontal", symbolic=S, pivoting="partial")
+    _check_parity(A, Fp, R)
     b = np.random.default_rng(2).standard_normal(A.shape[0])
     x = I.solve(F, b)
     xr = ON.solve(R, b)
