@@ -66,11 +66,11 @@ class Hip:
         self.h.hipFree(p)
 
 
-def run_gpu(hip, A, S, Rs, B, precision, steps, warmup, tol, max_refine):
+def run_gpu(hip, A, S, Rs, B, precision, steps, warmup, tol, max_refine, graph=False):
     lib = _lib.load()
     n, nrhs = B.shape
     t0 = time.perf_counter()
-    plan = I._Plan(A, S, precision=precision)
+    plan = I._Plan(A, S, precision=precision, use_graph=graph)
     t_plan = time.perf_counter() - t0
     dAx, dRs, dB = hip.upload(A.data), hip.upload(Rs), hip.upload(np.asfortranarray(B))
     dX = hip.upload(np.zeros((n, nrhs), order="F"))
@@ -170,6 +170,7 @@ def main():
     ap.add_argument("--max-refine", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--paths", default="fp64,fp32")
+    ap.add_argument("--graph", type=int, default=1, help="replay the factor schedule as a hipGraph")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the numeric phase runs on a single MI355X "
@@ -186,7 +187,8 @@ def main():
     hip = Hip()
     results = {}
     for prec in args.paths.split(","):
-        results[prec], X = run_gpu(hip, A, S, Rs, B, prec, args.steps, args.warmup, args.tol, args.max_refine)
+        results[prec], X = run_gpu(hip, A, S, Rs, B, prec, args.steps, args.warmup, args.tol, args.max_refine,
+                                   graph=bool(args.graph))
         print(f"# {prec}: {json.dumps(results[prec]['seconds'])} s/step, bwd {results[prec]['backward_error']:.2e}, "
               f"{results[prec]['refine_iterations']} refine", file=sys.stderr, flush=True)
     main_p = "fp32" if "fp32" in results else list(results)[0]

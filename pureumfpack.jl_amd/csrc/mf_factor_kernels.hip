@@ -396,7 +396,10 @@ __global__ __launch_bounds__(64) void k_spec_diag(FactorArgs a, int k, float tau
     const T itau = T(1) / T(tau);
     int bad = 0;
 #pragma unroll
-    for (int j = 0; j < KB; j++) {
+    for (int jc = 0; jc < KB / 32; jc++)
+#pragma unroll
+    for (int jj = 0; jj < 32; jj++) {
+        const int j = 32 * jc + jj;
         if (j >= kb) continue;
         if (lane == j) {
 #pragma unroll
@@ -458,12 +461,17 @@ __global__ __launch_bounds__(256) void k_spec_trsm(FactorArgs a, int k, float ta
 #pragma unroll
     for (int j = 0; j < KB; j++) x[j] = j < kb ? base[j * stride] : T(0);
     int bad = 0;
+    // j = 32 * jc + jj, written as two static loop levels so the compiler unrolls them fully
+    // (a single 64-trip loop with a 64-trip inner loop is not unrolled and spills x)
 #pragma unroll
-    for (int j = 0; j < KB; j++) {
+    for (int jc = 0; jc < KB / 32; jc++)
+#pragma unroll
+    for (int jj = 0; jj < 32; jj++) {
+        const int j = 32 * jc + jj;
         asm volatile("" ::: "memory");  // keep the block in LDS (no hoisting into registers)
         T v = x[j];
 #pragma unroll
-        for (int q = 0; q < KB; q++)
+        for (int q = 0; q < 32 * (jc + 1); q++)
             if (q < j) v -= x[q] * sd[q + j * KB];
         if (lrows) {
             const T u = sd[j + j * KB];
@@ -481,6 +489,75 @@ __global__ __launch_bounds__(256) void k_spec_trsm(FactorArgs a, int k, float ta
 #pragma unroll
     for (int j = 0; j < KB; j++)
         if (j < kb) base[j * stride] = x[j];
+}
+
+// ------------------------------------------------------------------------------------------
+// Inverses of the pivot-block factors of large fronts (np > INV_NP), used by the solves to turn
+// the triangular sweeps into matrix-vector products.  Leaves: the 64x64 diagonal blocks,
+// inverted by substitution on the identity, one column per thread; the off-diagonal blocks
+// come from the recursive rule inv([A 0; M B]) = [inv(A) 0; -inv(B) M inv(A) inv(B)] as two
+// batched gemms per tree level (and the transposed rule for U).  Items int4 {front, b0}.
+// ------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(128) void k_inv_leaf(FactorArgs a)
+{
+    constexpr int B = INV_LEAF, LD = INV_LEAF + 1;
+    __shared__ T sA[B * LD];
+    __shared__ T sX[B * LD];
+    const int mat = blockIdx.y;
+    const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
+    const SnDev d = a.sn[w.x];
+    const int np = d.np, nf = d.nf, b0 = w.y;
+    const int bs = min(B, np - b0);
+    T *pool = reinterpret_cast<T *>(a.pool) + mat * a.pool_stride;
+    const T *F = pool + d.foff;
+    const int tid = threadIdx.x;
+    const bool upper = tid >= 64;          // threads 0..63: inv(L) columns, 64..127: inv(U) columns
+    const int c = tid & 63;
+    T *X = pool + d.ioff + (upper ? (int64_t)np * np : 0);
+    for (int e = tid; e < B * B; e += blockDim.x) {
+        const int i = e % B, j = e / B;
+        sA[i + j * LD] = (i < bs && j < bs) ? F[(b0 + i) + (int64_t)(b0 + j) * nf] : T(0);
+    }
+    __syncthreads();
+    // L part: unit lower (strictly-lower entries of sA); U part: upper incl. diagonal.
+    if (!upper) {
+        for (int i = 0; i < B; i++) sX[i + c * LD] = T(0);
+        if (c < bs) {
+            sX[c + c * LD] = T(1);
+            for (int i = c + 1; i < bs; i++) {
+                T acc = T(0);
+                for (int q = c; q < i; q++) acc -= sA[i + q * LD] * sX[q + c * LD];
+                sX[i + c * LD] = acc;
+            }
+        }
+    }
+    __syncthreads();
+    if (!upper) {
+        for (int e = c; e < bs * bs; e += 64) {
+            const int i = e % bs, j = e / bs;
+            X[(b0 + i) + (int64_t)(b0 + j) * np] = sX[i + j * LD];
+        }
+    }
+    __syncthreads();
+    if (upper) {
+        for (int i = 0; i < B; i++) sX[i + c * LD] = T(0);
+        if (c < bs) {
+            sX[c + c * LD] = T(1) / sA[c + c * LD];
+            for (int i = c - 1; i >= 0; i--) {
+                T acc = T(0);
+                for (int q = i + 1; q <= c; q++) acc -= sA[i + q * LD] * sX[q + c * LD];
+                sX[i + c * LD] = acc / sA[i + i * LD];
+            }
+        }
+    }
+    __syncthreads();
+    if (upper) {
+        for (int e = c; e < bs * bs; e += 64) {
+            const int i = e % bs, j = e / bs;
+            X[(b0 + i) + (int64_t)(b0 + j) * np] = sX[i + j * LD];
+        }
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -562,6 +639,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs
         }
         __syncthreads();
     }
+    const bool add = pr.flags & GEMM_ADD, set = pr.flags & GEMM_SET;
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -570,7 +648,11 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 int gr = m0 + wm + 16 * i + Mfma<T>::row(lane, q);
-                if (gr < pr.m && gc < pr.n) C[gr + (int64_t)gc * pr.ldc] -= acc[i][j][q];
+                if (gr < pr.m && gc < pr.n) {
+                    T *cp = C + gr + (int64_t)gc * pr.ldc;
+                    const T v = add ? acc[i][j][q] : -acc[i][j][q];
+                    *cp = set ? v : *cp + v;
+                }
             }
         }
 }
@@ -605,6 +687,9 @@ hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int
         break;
     case L_PANEL_APPLY:
         hipLaunchKernelGGL(k_panel_apply<T>, grid, dim3(256), 0, st, a, L.k);
+        break;
+    case L_INV_LEAF:
+        hipLaunchKernelGGL(k_inv_leaf<T>, grid, dim3(128), 0, st, a);
         break;
     case L_SPEC_DIAG:
         hipLaunchKernelGGL((k_spec_diag<T, SPEC_KB>), grid, dim3(64), 0, st, a, L.k, L.tau);

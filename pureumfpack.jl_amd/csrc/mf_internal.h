@@ -32,13 +32,18 @@ struct SnDev {
     int32_t nchild;
     int32_t fcol;     // index of this front's first column in the per-front-column arrays
     int32_t pad;
+    i64 ioff;         // >= 0: inv(L11) at ioff and inv(U11) at ioff + np*np (ld np) in the pool
+    i64 pad2;
 };
 
-// Grouped gemm problem: C -= A * B, column-major, offsets in elements into one matrix's pool.
+// Grouped gemm problem, column-major, offsets in elements into one matrix's pool:
+//   C -= A B (flags 0), C += A B (GEMM_ADD), C = -A B (GEMM_SET), C = A B (GEMM_SET|GEMM_ADD).
+enum GemmFlags : int32_t { GEMM_ADD = 1, GEMM_SET = 2 };
 struct GemmProb {
     i64 aoff, boff, coff;
     int32_t m, n, k, lda, ldb, ldc;
-    int32_t pad[2];
+    int32_t flags;
+    int32_t pad;
 };
 
 // Grouped gemm tile: problem index and tile origin.
@@ -54,6 +59,7 @@ enum LaunchKind : int32_t {
     L_PANEL_APPLY = 4,  // row swaps + U-row solve + L21 solve after a panel (many workgroups)
     L_SPEC_DIAG = 5,    // speculative schedule: factor the diagonal block without interchanges
     L_SPEC_TRSM = 6,    // speculative schedule: L rows / U columns of the block + pivot check
+    L_INV_LEAF = 7,     // inverses of the 64x64 diagonal blocks of L11 and U11 (large fronts)
 };
 
 struct Launch {

@@ -16,7 +16,9 @@
 namespace mf {
 
 constexpr int PANEL_KB = 16;   // panel width of the large-front getrf (pivoted schedule)
-constexpr int SPEC_KB = 32;    // block width of the speculative (interchange-free) schedule
+constexpr int SPEC_KB = 64;    // block width of the speculative (interchange-free) schedule
+constexpr int INV_LEAF = 64;   // diagonal block of the recursive pivot-block inversion
+constexpr int INV_NP = 128;    // fronts with np > INV_NP keep inv(L11), inv(U11) for the solves
 constexpr int PINFO_STRIDE = 1 + 4 * PANEL_KB;  // per-front panel permutation record
 constexpr int GEMM_BM = 64, GEMM_BN = 64, GEMM_BK = 16;
 
@@ -79,6 +81,9 @@ struct SolveArgs {
     void *x;       // backward result in factor-column order (n x nrhs)
     void *uv;      // update vectors (nupd x nrhs)
     void *part;    // split-K partial sums of the rectangular products
+    void *zbuf;    // large fronts: permuted pivot part (forward) / U12-corrected part (backward)
+    const int32_t *ecolptr;  // extend-add lists of the factorization (per front position)
+    const int2 *ext;
     int64_t n, nupd, nrhs;
     int64_t pool_stride, n_stride, part_stride;
 };
@@ -89,7 +94,8 @@ hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int
                               const GemmProb *probs, const GemmTile *tiles, int nmat, hipStream_t st);
 int panel_variant_rows(int v);
 // which: 0 forward triangular, 1/2 forward rectangular partials/sum, 3/4 backward rectangular
-// partials/sum, 5 backward triangular
+// partials/sum, 5 backward triangular, 6/7 inv(L11) partials/sum, 8/9 inv(U11) partials/sum,
+// 10 forward gather
 template <typename T>
 hipError_t launch_solve_level(int which, const SolveArgs &a, const int32_t *lists, int list_off, int count, int nmat,
                               hipStream_t st);
@@ -102,10 +108,10 @@ hipError_t launch_finish_x(const T *x, const int32_t *qf, double *X, int64_t ldx
 hipError_t launch_residual(const int32_t *rp, const int32_t *ci, const int32_t *vmap, const double *ax,
                            int64_t ax_stride, const double *X, int64_t ldx, const double *B, int64_t ldb, double *R,
                            int64_t n, int64_t nrhs, int64_t msx, int64_t msb, int nmat, hipStream_t st);
-hipError_t launch_norms(const double *R, const double *X, int64_t ldx, const double *B, int64_t ldb, int64_t n,
-                        int64_t nrhs, int64_t msx, int64_t msb, double *out, int nmat, hipStream_t st);
-hipError_t launch_anorm(const int32_t *rp, const int32_t *vmap, const double *ax, int64_t ax_stride, int64_t n,
+hipError_t launch_norms(const double *R, const double *X, const double *B, int64_t n, int64_t nrhs, double *part,
                         double *out, int nmat, hipStream_t st);
+hipError_t launch_anorm(const int32_t *rp, const int32_t *vmap, const double *ax, int64_t ax_stride, int64_t n,
+                        double *part, double *out, int nmat, hipStream_t st);
 hipError_t launch_rowfac(const int32_t *prow, int32_t *rowfac, int64_t n, hipStream_t st);
 template <typename T>
 hipError_t launch_export(const SnDev *sn, int64_t nsuper, const T *pool, const int32_t *upd, const int32_t *rowfac,

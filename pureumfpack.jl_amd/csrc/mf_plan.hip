@@ -95,6 +95,9 @@ struct mf_plan {
     std::vector<int32_t> lvl_off, lvl_cnt;  // solve lists per level
     // per level: offsets/counts of the forward/backward split-K part and sum item lists
     std::vector<int32_t> lvl_fpo, lvl_fpc, lvl_fso, lvl_fsc, lvl_bpo, lvl_bpc, lvl_bso, lvl_bsc;
+    std::vector<int32_t> lvl_ilpo, lvl_ilpc, lvl_ilso, lvl_ilsc, lvl_iupo, lvl_iupc, lvl_iuso, lvl_iusc;
+    std::vector<int32_t> lvl_go, lvl_gc;   // forward gather items {front, pos0}
+    int64_t front_elems = 0, inv_elems = 0, tmp_elems = 0;  // pool = fronts | inverses | gemm temps
     int64_t part_slots = 0;  // partial-sum slots (256 rows x nrhs each) the solve needs
     std::vector<double> lvl_lflops, lvl_uflops, lvl_lbytes, lvl_ubytes, lvl_vec;
     // device
@@ -102,7 +105,7 @@ struct mf_plan {
     DevBuf d_lists, d_probs, d_tiles, d_solvelist, d_qf;
     DevBuf d_csr_rp, d_csr_ci, d_csr_vmap;
     DevBuf d_pool, d_prow, d_info, d_ax, d_rs, d_pinfo, d_flag;
-    DevBuf d_w0, d_y, d_x, d_uv, d_part, d_B, d_X, d_R, d_norms, d_anorm;
+    DevBuf d_w0, d_y, d_x, d_uv, d_part, d_zbuf, d_B, d_X, d_R, d_norms, d_anorm, d_npart;
     DevBuf d_Lp, d_Up, d_uoff, d_Li, d_Lx, d_Ui, d_Ux, d_rowfac;
     int64_t solve_nrhs = 0;
     bool factored = false;
@@ -354,7 +357,15 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
         fcol[s + 1] = fcol[s] + nf[s];
         P.maxnf = std::max(P.maxnf, nf[s]);
     }
-    P.pool_elems = pe;
+    P.front_elems = pe;
+    // large fronts keep inv(L11) and inv(U11) (np x np each) after the fronts
+    std::vector<int64_t> ioff(nsuper, -1);
+    for (int64_t s = 0; s < nsuper; s++)
+        if (np[s] > INV_NP) {
+            ioff[s] = pe;
+            pe += 2 * np[s] * np[s];
+        }
+    P.inv_elems = pe - P.front_elems;
     if (fcol[nsuper] >= INT32_MAX) return fail(MF_BADARG, "too many front columns");
     // original entries: V = As[qf,qf]; entry (iV, jV) goes to the front of column min(iV,jV)
     // (src/multifrontal.jl:216-231)
@@ -462,6 +473,8 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
         d.nchild = (int32_t)kids[s].size();
         d.fcol = (int32_t)fcol[s];
         d.pad = 0;
+        d.ioff = ioff[s];
+        d.pad2 = 0;
     }
     // flops
     for (int64_t s = 0; s < nsuper; s++) {
@@ -715,7 +728,77 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             sched.push_back(L);
         }
     }
+    // recursive inversion of the pivot-block factors of the large fronts (after every level)
+    {
+        std::vector<int32_t> items;
+        for (int64_t s = 0; s < nsuper; s++)
+            if (ioff[s] >= 0)
+                for (int64_t b0 = 0; b0 < np[s]; b0 += INV_LEAF) {
+                    items.push_back((int32_t)s); items.push_back((int32_t)b0); items.push_back(0); items.push_back(0);
+                }
+        if (!items.empty()) {
+            while (lists.size() & 3) lists.push_back(0);
+            Launch Lf{};
+            Lf.kind = L_INV_LEAF;
+            Lf.list_off = (int32_t)lists.size();
+            lists.insert(lists.end(), items.begin(), items.end());
+            Lf.count = (int32_t)(items.size() / 4);
+            Lf.level = (int32_t)nlev;
+            Lf.bytes = 0;
+            for (size_t q = 0; q < items.size(); q += 4) {
+                double b = (double)std::min<int64_t>(INV_LEAF, np[items[q]] - items[q + 1]);
+                Lf.flops += 2.0 * b * b * b / 3.0;
+                Lf.bytes += 3.0 * b * b * elem;
+            }
+            sched.push_back(Lf);
+            for (int64_t half = INV_LEAF;; half *= 2) {
+                std::vector<GemmProb> t1, t2;
+                int64_t toff = pe;   // temps live after the inverses
+                for (int64_t s = 0; s < nsuper; s++) {
+                    if (ioff[s] < 0) continue;
+                    const int64_t f = foff[s], Li = ioff[s], Ui = ioff[s] + np[s] * np[s], ld = np[s];
+                    for (int64_t a0 = 0; a0 + half < np[s]; a0 += 2 * half) {
+                        int64_t m = a0 + half, b = std::min(a0 + 2 * half, np[s]);
+                        int64_t s1 = m - a0, s2 = b - m;
+                        // L: T = -M inv(L1) (s2 x s1), X21 = inv(L2) T
+                        GemmProb g{};
+                        g.aoff = f + m + a0 * nf[s]; g.lda = (int32_t)nf[s];
+                        g.boff = Li + a0 + a0 * ld; g.ldb = (int32_t)ld;
+                        g.coff = toff; g.ldc = (int32_t)s2;
+                        g.m = (int32_t)s2; g.n = (int32_t)s1; g.k = (int32_t)s1; g.flags = GEMM_SET;
+                        t1.push_back(g);
+                        GemmProb h{};
+                        h.aoff = Li + m + m * ld; h.lda = (int32_t)ld;
+                        h.boff = toff; h.ldb = (int32_t)s2;
+                        h.coff = Li + m + a0 * ld; h.ldc = (int32_t)ld;
+                        h.m = (int32_t)s2; h.n = (int32_t)s1; h.k = (int32_t)s2; h.flags = GEMM_SET | GEMM_ADD;
+                        t2.push_back(h);
+                        toff += s2 * s1;
+                        // U: T = -N inv(U2) (s1 x s2), X12 = inv(U1) T
+                        GemmProb u{};
+                        u.aoff = f + a0 + m * nf[s]; u.lda = (int32_t)nf[s];
+                        u.boff = Ui + m + m * ld; u.ldb = (int32_t)ld;
+                        u.coff = toff; u.ldc = (int32_t)s1;
+                        u.m = (int32_t)s1; u.n = (int32_t)s2; u.k = (int32_t)s2; u.flags = GEMM_SET;
+                        t1.push_back(u);
+                        GemmProb v{};
+                        v.aoff = Ui + a0 + a0 * ld; v.lda = (int32_t)ld;
+                        v.boff = toff; v.ldb = (int32_t)s1;
+                        v.coff = Ui + a0 + m * ld; v.ldc = (int32_t)ld;
+                        v.m = (int32_t)s1; v.n = (int32_t)s2; v.k = (int32_t)s1; v.flags = GEMM_SET | GEMM_ADD;
+                        t2.push_back(v);
+                        toff += s1 * s2;
+                    }
+                }
+                if (t1.empty()) break;
+                P.tmp_elems = std::max(P.tmp_elems, toff - pe);
+                add_gemm(t1, (int)nlev, 2);
+                add_gemm(t2, (int)nlev, 2);
+            }
+        }
     }
+    }
+    P.pool_elems = pe + P.tmp_elems;
     // solve lists: all fronts per level
     std::vector<int32_t> slist;
     P.lvl_off.assign(nlev, 0);
@@ -729,16 +812,29 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
         P.lvl_cnt[lv] = (int32_t)bylevel[lv].size();
         // forward: rows t < nu of L21 against columns a < np; backward: rows i < np of U12
         // against columns t < nu.  Partial slots are reused level by level.
-        for (int dir = 0; dir < 2; dir++) {
+        while (slist.size() & 3) slist.push_back(0);
+        P.lvl_go.push_back((int32_t)slist.size());
+        for (int64_t s : bylevel[lv])
+            for (int64_t p0 = 0; p0 < nf[s]; p0 += 256) emit4((int32_t)s, (int32_t)p0, 0, 0);
+        P.lvl_gc.push_back((int32_t)((slist.size() - P.lvl_go.back()) / 4));
+        // dir 0: L21 (rows nu, cols np); 1: U12 (rows np, cols nu); 2: inv(L11) and 3: inv(U11)
+        // of the large fronts (rows np, cols np; chunks entirely outside the triangle skipped)
+        for (int dir = 0; dir < 4; dir++) {
             int64_t slot = 0;
             std::vector<int32_t> sums;
             while (slist.size() & 3) slist.push_back(0);
             int32_t po = (int32_t)slist.size();
             for (int64_t s : bylevel[lv]) {
-                int64_t nrows = dir == 0 ? nu[s] : np[s], ncols = dir == 0 ? np[s] : nu[s];
+                if (dir >= 2 && ioff[s] < 0) continue;
+                int64_t nrows = dir == 0 ? nu[s] : np[s], ncols = dir == 1 ? nu[s] : np[s];
                 for (int64_t r0 = 0; r0 < nrows; r0 += 256) {
-                    int64_t nch = (ncols + SOLVE_CK_HOST - 1) / SOLVE_CK_HOST;
-                    for (int64_t c = 0; c < nch; c++) emit4((int32_t)s, (int32_t)r0, (int32_t)(c * SOLVE_CK_HOST), (int32_t)(slot + c));
+                    int64_t nch = 0;
+                    for (int64_t c0 = 0; c0 < ncols; c0 += SOLVE_CK_HOST) {
+                        if (dir == 2 && c0 > r0 + 255) continue;            // above the lower triangle
+                        if (dir == 3 && c0 + SOLVE_CK_HOST <= r0) continue;  // below the upper triangle
+                        emit4((int32_t)s, (int32_t)r0, (int32_t)c0, (int32_t)(slot + nch));
+                        nch++;
+                    }
                     sums.push_back((int32_t)s); sums.push_back((int32_t)r0); sums.push_back((int32_t)nch);
                     sums.push_back((int32_t)slot);
                     slot += nch;
@@ -748,8 +844,12 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             int32_t so = (int32_t)slist.size();
             slist.insert(slist.end(), sums.begin(), sums.end());
             int32_t sc = (int32_t)(sums.size() / 4);
-            if (dir == 0) { P.lvl_fpo.push_back(po); P.lvl_fpc.push_back(pc); P.lvl_fso.push_back(so); P.lvl_fsc.push_back(sc); }
-            else { P.lvl_bpo.push_back(po); P.lvl_bpc.push_back(pc); P.lvl_bso.push_back(so); P.lvl_bsc.push_back(sc); }
+            switch (dir) {
+            case 0: P.lvl_fpo.push_back(po); P.lvl_fpc.push_back(pc); P.lvl_fso.push_back(so); P.lvl_fsc.push_back(sc); break;
+            case 1: P.lvl_bpo.push_back(po); P.lvl_bpc.push_back(pc); P.lvl_bso.push_back(so); P.lvl_bsc.push_back(sc); break;
+            case 2: P.lvl_ilpo.push_back(po); P.lvl_ilpc.push_back(pc); P.lvl_ilso.push_back(so); P.lvl_ilsc.push_back(sc); break;
+            default: P.lvl_iupo.push_back(po); P.lvl_iupc.push_back(pc); P.lvl_iuso.push_back(so); P.lvl_iusc.push_back(sc); break;
+            }
             P.part_slots = std::max(P.part_slots, slot);
         }
         double lf = 0, uf = 0, lb = 0, ub = 0, vec = 0;
@@ -831,6 +931,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
     HIPCHK(upload(P.d_csr_vmap, vmap));
     HIPCHK(upload(P.d_uoff, P.uoff));
     HIPCHK(P.d_pool.alloc((size_t)P.pool_elems * P.esize() * P.nbatch));
+    HIPCHK(hipMemset(P.d_pool.p, 0, (size_t)P.pool_elems * P.esize() * P.nbatch));
     HIPCHK(P.d_prow.alloc((size_t)n * 4 * P.nbatch));
     HIPCHK(P.d_info.alloc((size_t)4 * P.nbatch));
     HIPCHK(P.d_pinfo.alloc((size_t)4 * PINFO_STRIDE * nsuper * P.nbatch));
@@ -838,6 +939,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
     HIPCHK(P.d_ax.alloc((size_t)P.nnz * 8 * P.nbatch));
     HIPCHK(P.d_rs.alloc((size_t)n * 8 * P.nbatch));
     HIPCHK(P.d_anorm.alloc((size_t)8 * P.nbatch));
+    HIPCHK(P.d_npart.alloc((size_t)8 * 256 * P.nbatch));
     return MF_OK;
 }
 
@@ -939,7 +1041,7 @@ static int run_factor(mf_plan &P, const double *d_ax, const double *d_rs, int nm
                                          P.d_tiles.as<GemmTile>(), nmat, P.stream);
         }));
     HIPCHK(launch_anorm(P.d_csr_rp.as<int32_t>(), P.d_csr_vmap.as<int32_t>(), d_ax, P.nnz, P.n,
-                        P.d_anorm.as<double>(), nmat, P.stream));
+                        P.d_npart.as<double>(), P.d_anorm.as<double>(), nmat, P.stream));
     return MF_OK;
 }
 
@@ -1066,6 +1168,9 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
     a.x = P.d_x.p;
     a.uv = P.d_uv.p;
     a.part = P.d_part.p;
+    a.zbuf = P.d_zbuf.p;
+    a.ecolptr = P.d_ecolptr.as<int32_t>();
+    a.ext = P.d_ext.as<int2>();
     a.n = n;
     a.nupd = P.nupd;
     a.nrhs = nrhs;
@@ -1077,8 +1182,19 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
     for (int64_t lv = 0; lv < P.nlevels; lv++) {
         const double vb = 3.0 * e * P.lvl_vec[lv] * nrhs;
         HIPCHK(profiled(P, 5, 0.0, vb * nmat, [&] {
+            return launch_solve_level<T>(10, a, sl, P.lvl_go[lv], P.lvl_gc[lv], nmat, P.stream);
+        }));
+        HIPCHK(profiled(P, 5, 0.0, 0.0, [&] {
             return launch_solve_level<T>(0, a, sl, P.lvl_off[lv], P.lvl_cnt[lv], nmat, P.stream);
         }));
+        if (P.lvl_ilsc[lv] > 0) {
+            HIPCHK(profiled(P, 5, 0.0, 0.0, [&] {
+                return launch_solve_level<T>(6, a, sl, P.lvl_ilpo[lv], P.lvl_ilpc[lv], nmat, P.stream);
+            }));
+            HIPCHK(profiled(P, 5, 0.0, 0.0, [&] {
+                return launch_solve_level<T>(7, a, sl, P.lvl_ilso[lv], P.lvl_ilsc[lv], nmat, P.stream);
+            }));
+        }
         if (P.lvl_fpc[lv] > 0) {
             HIPCHK(profiled(P, 5, P.lvl_lflops[lv] * nrhs * nmat, P.lvl_lbytes[lv] * nmat, [&] {
                 return launch_solve_level<T>(1, a, sl, P.lvl_fpo[lv], P.lvl_fpc[lv], nmat, P.stream);
@@ -1097,6 +1213,14 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
         HIPCHK(profiled(P, 6, 0.0, 0.0, [&] {
             return launch_solve_level<T>(4, a, sl, P.lvl_bso[lv], P.lvl_bsc[lv], nmat, P.stream);
         }));
+        if (P.lvl_iusc[lv] > 0) {
+            HIPCHK(profiled(P, 6, 0.0, 0.0, [&] {
+                return launch_solve_level<T>(8, a, sl, P.lvl_iupo[lv], P.lvl_iupc[lv], nmat, P.stream);
+            }));
+            HIPCHK(profiled(P, 6, 0.0, 0.0, [&] {
+                return launch_solve_level<T>(9, a, sl, P.lvl_iuso[lv], P.lvl_iusc[lv], nmat, P.stream);
+            }));
+        }
         HIPCHK(profiled(P, 6, 0.0, 0.0, [&] {
             return launch_solve_level<T>(5, a, sl, P.lvl_off[lv], P.lvl_cnt[lv], nmat, P.stream);
         }));
@@ -1121,8 +1245,8 @@ static int solve_core(mf_plan *P, int64_t nrhs, int nmat, int32_t refine_steps, 
         HIPCHK(launch_residual(P->d_csr_rp.as<int32_t>(), P->d_csr_ci.as<int32_t>(), P->d_csr_vmap.as<int32_t>(),
                                P->d_ax.as<double>(), P->nnz, P->d_X.as<double>(), n, P->d_B.as<double>(), n,
                                P->d_R.as<double>(), n, nrhs, n * nrhs, n * nrhs, nmat, P->stream));
-        HIPCHK(launch_norms(P->d_R.as<double>(), P->d_X.as<double>(), n, P->d_B.as<double>(), n, n, nrhs, n * nrhs,
-                            n * nrhs, P->d_norms.as<double>(), nmat, P->stream));
+        HIPCHK(launch_norms(P->d_R.as<double>(), P->d_X.as<double>(), P->d_B.as<double>(), n, nrhs,
+                            P->d_npart.as<double>(), P->d_norms.as<double>(), nmat, P->stream));
         h.resize((size_t)5 * nrhs * nmat + nmat);
         HIPCHK(hipMemcpyAsync(h.data(), P->d_norms.p, (size_t)5 * nrhs * nmat * 8, hipMemcpyDeviceToHost, P->stream));
         HIPCHK(hipMemcpyAsync(h.data() + 5 * nrhs * nmat, P->d_anorm.p, (size_t)nmat * 8, hipMemcpyDeviceToHost,
@@ -1189,10 +1313,12 @@ static int ensure_solve_buffers(mf_plan *P, int64_t nrhs, int nmat)
     HIPCHK(P->d_x.alloc(nv * e));
     HIPCHK(P->d_uv.alloc((size_t)std::max<int64_t>(P->nupd, 1) * nrhs * nmat * e));
     HIPCHK(P->d_part.alloc((size_t)std::max<int64_t>(P->part_slots, 1) * 256 * nrhs * nmat * e));
+    HIPCHK(P->d_zbuf.alloc(nv * e));
     HIPCHK(P->d_B.alloc(nv * 8));
     HIPCHK(P->d_X.alloc(nv * 8));
     HIPCHK(P->d_R.alloc(nv * 8));
     HIPCHK(P->d_norms.alloc((size_t)5 * nrhs * nmat * 8));
+    HIPCHK(P->d_npart.alloc(std::max<size_t>((size_t)8 * 256 * P->nbatch, (size_t)5 * 64 * nrhs * nmat * 8)));
     return MF_OK;
 }
 

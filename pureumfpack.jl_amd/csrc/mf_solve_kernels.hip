@@ -85,10 +85,49 @@ __device__ __forceinline__ void panel_dot(const T *row, int ld, int bs, const T 
     }
 }
 
-// Forward sweep, part 1 (one workgroup per supernode of the level): gather the children's
-// update vectors into this supernode's pivot rows (kept in w0, V-row order) and update rows
-// (uv, this supernode's slot), permute the pivot part by the block pivoting into y (factor
-// order), then solve with the unit lower L11 block by block.  The L21 product that finishes
+// Forward sweep, part 0: pull the children's update vectors into this supernode's front
+// vector, one thread per front position (items int4 {front, pos0}): pivot rows accumulate in
+// w0 (V-row order), update rows start this supernode's update vector uv.  The children of a
+// position come from the extend-add lists of the factorization, in the same reverse child
+// order, so the sums do not depend on scheduling.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_gather(SolveArgs a, const int32_t *__restrict__ ecolptr,
+                                                    const int2 *__restrict__ ext)
+{
+    const int mat = blockIdx.y;
+    const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
+    const SnDev d = a.sn[w.x];
+    const int p = w.y + threadIdx.x;
+    if (p >= d.nf) return;
+    const int64_t vb = (int64_t)mat * a.nrhs;
+    T *w0 = reinterpret_cast<T *>(a.w0) + vb * a.n;
+    T *uv = reinterpret_cast<T *>(a.uv) + vb * a.nupd;
+    const int q0 = ecolptr[d.fcol + p], q1 = ecolptr[d.fcol + p + 1];
+    for (int r0 = 0; r0 < (int)a.nrhs; r0 += SOLVE_RC) {
+        const int rc = min(SOLVE_RC, (int)a.nrhs - r0);
+        T acc[SOLVE_RC];
+#pragma unroll
+        for (int r = 0; r < SOLVE_RC; r++)
+            acc[r] = (r < rc && p < d.np) ? w0[(int64_t)(r0 + r) * a.n + d.c1 + p] : T(0);
+        for (int q = q0; q < q1; q++) {
+            const int2 e = ext[q];
+            const int32_t off = a.sn[e.x].upd_off + e.y;
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++)
+                if (r < rc) acc[r] += uv[(int64_t)(r0 + r) * a.nupd + off];
+        }
+#pragma unroll
+        for (int r = 0; r < SOLVE_RC; r++) {
+            if (r >= rc) continue;
+            if (p < d.np) w0[(int64_t)(r0 + r) * a.n + d.c1 + p] = acc[r];
+            else uv[(int64_t)(r0 + r) * a.nupd + d.upd_off + p - d.np] = acc[r];
+        }
+    }
+}
+
+// Forward sweep, part 1 (one workgroup per supernode of the level): permute the gathered
+// pivot part by the block pivoting into y (factor order), then solve with the unit lower L11
+// block by block.  The L21 product that finishes
 // the update vector is k_fwd_rect.
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_tri(SolveArgs a)
@@ -97,33 +136,21 @@ __global__ __launch_bounds__(256) void k_fwd_tri(SolveArgs a)
     __shared__ T sdiag[SOLVE_NB * (SOLVE_NB + 1)];
     const int mat = blockIdx.y;
     const SnDev d = a.sn[a.list[blockIdx.x]];
-    const int nf = d.nf, np = d.np, nu = d.nu, c1 = d.c1;
+    const int nf = d.nf, np = d.np, c1 = d.c1;
     const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
     const int32_t *prow = a.prow + mat * a.n_stride;
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
     const int64_t vb = (int64_t)mat * a.nrhs;
     T *w0 = reinterpret_cast<T *>(a.w0) + vb * a.n;
     T *y = reinterpret_cast<T *>(a.y) + vb * a.n;
-    T *uv = reinterpret_cast<T *>(a.uv) + vb * a.nupd;
     const int nrhs = (int)a.nrhs;
-    for (int e = tid; e < nu * nrhs; e += nt) uv[(e / nu) * a.nupd + d.upd_off + e % nu] = T(0);
-    __syncthreads();
-    for (int ci = 0; ci < d.nchild; ci++) {
-        const SnDev c = a.sn[a.child_list[d.child_off + ci]];
-        const int32_t *rm = a.rmap + c.upd_off;
-        for (int e = tid; e < c.nu * nrhs; e += nt) {
-            const int t = e % c.nu, r = e / c.nu;
-            const T v = uv[r * a.nupd + c.upd_off + t];
-            const int pos = rm[t];
-            if (pos < np) w0[r * a.n + c1 + pos] += v;
-            else uv[r * a.nupd + d.upd_off + pos - np] += v;
-        }
-        __syncthreads();
-    }
+    const bool inv = d.ioff >= 0;   // large front: inv(L11) is applied by k_rect_part kind 2
+    T *z = inv ? reinterpret_cast<T *>(a.zbuf) + vb * a.n : y;
     for (int e = tid; e < np * nrhs; e += nt) {
         const int i = e % np, r = e / np;
-        y[r * a.n + c1 + i] = w0[r * a.n + prow[c1 + i]];
+        z[r * a.n + c1 + i] = w0[r * a.n + prow[c1 + i]];
     }
+    if (inv) return;
     __syncthreads();
     for (int r0 = 0; r0 < nrhs; r0 += SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
@@ -177,16 +204,25 @@ __global__ __launch_bounds__(256) void k_rect_part(SolveArgs a, int kind)
     const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
     const int tid = threadIdx.x;
     const int row = w.y + tid;
-    const int nrows = kind == 0 ? nu : np, ncols = kind == 0 ? np : nu;
+    const int nrows = kind == 0 ? nu : np, ncols = kind == 1 ? nu : np;
     const int cend = min(w.z + SOLVE_CK, ncols);
     const int64_t vb = (int64_t)mat * a.nrhs;
     const T *y = reinterpret_cast<const T *>(a.y) + vb * a.n;
     const T *x = reinterpret_cast<const T *>(a.x) + vb * a.n;
+    const T *zb = reinterpret_cast<const T *>(a.zbuf) + vb * a.n;
     T *part = reinterpret_cast<T *>(a.part) + mat * a.part_stride + (int64_t)w.w * 256 * a.nrhs;
     const int32_t *upd = a.upd + d.upd_off;
-    // row r of the operand: forward reads L21 row (np + r) over columns a; backward U12 row r
-    // over columns np + t
-    const T *rowp = kind == 0 ? F + (np + row) : F + row + (int64_t)np * nf;
+    // row r of the operand: kind 0 L21 row (np + r) over columns a; kind 1 U12 row r over
+    // columns np + t; kinds 2 / 3 row r of inv(L11) / inv(U11) (ld np)
+    const T *pool = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride;
+    const T *rowp;
+    int ld = nf;
+    switch (kind) {
+    case 0: rowp = F + (np + row); break;
+    case 1: rowp = F + row + (int64_t)np * nf; break;
+    case 2: rowp = pool + d.ioff + row; ld = np; break;
+    default: rowp = pool + d.ioff + (int64_t)np * np + row; ld = np; break;
+    }
     const int nrhs = (int)a.nrhs;
     for (int r0 = 0; r0 < nrhs; r0 += SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
@@ -197,11 +233,14 @@ __global__ __launch_bounds__(256) void k_rect_part(SolveArgs a, int kind)
             for (int e = tid; e < SOLVE_NB * SOLVE_RC; e += blockDim.x) {
                 const int j = e % SOLVE_NB, r = e / SOLVE_NB;
                 T v = T(0);
-                if (j < bs && r < rc) v = kind == 0 ? y[(r0 + r) * a.n + c1 + j0 + j] : x[(r0 + r) * a.n + upd[j0 + j]];
+                if (j < bs && r < rc) {
+                    const int64_t ro = (int64_t)(r0 + r) * a.n;
+                    v = kind == 0 ? y[ro + c1 + j0 + j] : kind == 1 ? x[ro + upd[j0 + j]] : zb[ro + c1 + j0 + j];
+                }
                 sv[e] = v;
             }
             __syncthreads();
-            if (row < nrows) panel_dot<T>(rowp + (int64_t)j0 * nf, nf, bs, sv, acc);
+            if (row < nrows) panel_dot<T>(rowp + (int64_t)j0 * ld, ld, bs, sv, acc);
         }
 #pragma unroll
         for (int r = 0; r < SOLVE_RC; r++)
@@ -221,16 +260,22 @@ __global__ __launch_bounds__(256) void k_rect_sum(SolveArgs a, int kind)
     const int nrows = kind == 0 ? nu : np;
     if (row >= nrows) return;
     const int64_t vb = (int64_t)mat * a.nrhs;
-    const T *y = reinterpret_cast<const T *>(a.y) + vb * a.n;
+    T *y = reinterpret_cast<T *>(a.y) + vb * a.n;
     T *x = reinterpret_cast<T *>(a.x) + vb * a.n;
+    T *zb = reinterpret_cast<T *>(a.zbuf) + vb * a.n;
     T *uv = reinterpret_cast<T *>(a.uv) + vb * a.nupd;
     const T *part = reinterpret_cast<const T *>(a.part) + mat * a.part_stride + (int64_t)w.w * 256 * a.nrhs;
     const int64_t cs = (int64_t)256 * a.nrhs;
     for (int r = 0; r < (int)a.nrhs; r++) {
         T s = T(0);
         for (int c = 0; c < w.z; c++) s += part[c * cs + (int64_t)r * 256 + threadIdx.x];
-        if (kind == 0) uv[r * a.nupd + d.upd_off + row] -= s;
-        else x[r * a.n + c1 + row] = y[r * a.n + c1 + row] - s;
+        const int64_t ro = (int64_t)r * a.n + c1 + row;
+        switch (kind) {
+        case 0: uv[r * a.nupd + d.upd_off + row] -= s; break;
+        case 1: (d.ioff >= 0 ? zb : x)[ro] = y[ro] - s; break;   // large fronts: inv(U11) next
+        case 2: y[ro] = s; break;
+        default: x[ro] = s; break;
+        }
     }
 }
 
@@ -242,6 +287,7 @@ __global__ __launch_bounds__(256) void k_bwd_tri(SolveArgs a)
     __shared__ T sdiag[SOLVE_NB * (SOLVE_NB + 1)];
     const int mat = blockIdx.y;
     const SnDev d = a.sn[a.list[blockIdx.x]];
+    if (d.ioff >= 0) return;  // large front: inv(U11) was applied by k_rect_part kind 3
     const int nf = d.nf, np = d.np, c1 = d.c1;
     const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
@@ -337,20 +383,20 @@ __global__ void k_residual(const int32_t *__restrict__ rp, const int32_t *__rest
     }
 }
 
-// Per (matrix, rhs) column: out[0] = max|R|, out[1] = max|X|, out[2] = max|B|,
-// out[3] = sum R^2, out[4] = sum B^2.  One workgroup per column.
-__global__ __launch_bounds__(256) void k_norms(const double *__restrict__ R, const double *__restrict__ X,
-                                               int64_t ldx, const double *__restrict__ B, int64_t ldb,
-                                               int64_t n, int64_t nrhs, int64_t mat_stride_x,
-                                               int64_t mat_stride_b, double *__restrict__ out)
+// Norms for the backward error, two passes.  Pass 1: grid (NORM_BLOCKS, nrhs, nmat); block b
+// of column c reduces its slice to part[((mat * nrhs + c) * NORM_BLOCKS + b) * 5 + q] with
+// q = 0 max|R|, 1 max|X|, 2 max|B|, 3 sum R^2, 4 sum B^2.  Pass 2 folds the blocks in order.
+constexpr int NORM_BLOCKS = 64;
+
+__global__ __launch_bounds__(256) void k_norms_part(const double *__restrict__ R, const double *__restrict__ X,
+                                                    const double *__restrict__ B, int64_t n, int64_t nrhs,
+                                                    double *__restrict__ part)
 {
-    const int col = blockIdx.x, mat = blockIdx.y;
-    const double *r = R + mat * mat_stride_b + (int64_t)col * n;
-    const double *x = X + mat * mat_stride_x + (int64_t)col * ldx;
-    const double *b = B + mat * mat_stride_b + (int64_t)col * ldb;
+    const int b = blockIdx.x, col = blockIdx.y, mat = blockIdx.z;
+    const int64_t off = ((int64_t)mat * nrhs + col) * n;
     double mr = 0, mx = 0, mb = 0, sr = 0, sb = 0;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-        double rv = r[i], xv = x[i], bv = b[i];
+    for (int64_t i = (int64_t)b * blockDim.x + threadIdx.x; i < n; i += (int64_t)NORM_BLOCKS * blockDim.x) {
+        const double rv = R[off + i], xv = X[off + i], bv = B[off + i];
         mr = fmax(mr, fabs(rv)); mx = fmax(mx, fabs(xv)); mb = fmax(mb, fabs(bv));
         sr += rv * rv; sb += bv * bv;
     }
@@ -358,24 +404,37 @@ __global__ __launch_bounds__(256) void k_norms(const double *__restrict__ R, con
     sh[0][threadIdx.x] = mr; sh[1][threadIdx.x] = mx; sh[2][threadIdx.x] = mb;
     sh[3][threadIdx.x] = sr; sh[4][threadIdx.x] = sb;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            for (int q = 0; q < 3; q++) sh[q][threadIdx.x] = fmax(sh[q][threadIdx.x], sh[q][threadIdx.x + s]);
-            for (int q = 3; q < 5; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            for (int q = 0; q < 3; q++) sh[q][threadIdx.x] = fmax(sh[q][threadIdx.x], sh[q][threadIdx.x + st]);
+            for (int q = 3; q < 5; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + st];
         }
         __syncthreads();
     }
-    if (threadIdx.x < 5) out[((int64_t)mat * nrhs + col) * 5 + threadIdx.x] = sh[threadIdx.x][0];
+    if (threadIdx.x < 5) part[(((int64_t)mat * nrhs + col) * NORM_BLOCKS + b) * 5 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-// ||A||_inf per matrix (max row sum of |a_ij|), one workgroup per matrix.
-__global__ __launch_bounds__(256) void k_anorm(const int32_t *__restrict__ rp, const int32_t *__restrict__ vmap,
-                                               const double *__restrict__ ax, int64_t ax_stride, int64_t n,
-                                               double *__restrict__ out)
+__global__ void k_norms_fold(const double *__restrict__ part, int64_t ncols, double *__restrict__ out)
 {
-    const double *axm = ax + blockIdx.x * ax_stride;
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    double v[5] = {0, 0, 0, 0, 0};
+    for (int b = 0; b < NORM_BLOCKS; b++) {
+        const double *p = part + (c * NORM_BLOCKS + b) * 5;
+        for (int q = 0; q < 3; q++) v[q] = fmax(v[q], p[q]);
+        for (int q = 3; q < 5; q++) v[q] += p[q];
+    }
+    for (int q = 0; q < 5; q++) out[c * 5 + q] = v[q];
+}
+
+// ||A||_inf per matrix (max row sum of |a_ij|): per-block maxima, then an in-order fold.
+__global__ __launch_bounds__(256) void k_anorm_part(const int32_t *__restrict__ rp, const int32_t *__restrict__ vmap,
+                                                    const double *__restrict__ ax, int64_t ax_stride, int64_t n,
+                                                    double *__restrict__ part)
+{
+    const double *axm = ax + blockIdx.y * ax_stride;
     double m = 0;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         double s = 0;
         for (int p = rp[i]; p < rp[i + 1]; p++) s += fabs(axm[vmap[p]]);
         m = fmax(m, s);
@@ -383,11 +442,20 @@ __global__ __launch_bounds__(256) void k_anorm(const int32_t *__restrict__ rp, c
     __shared__ double sh[256];
     sh[threadIdx.x] = m;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + st]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+    if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
+}
+
+__global__ void k_anorm_fold(const double *__restrict__ part, int nblk, int nmat, double *__restrict__ out)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nmat) return;
+    double v = 0;
+    for (int b = 0; b < nblk; b++) v = fmax(v, part[m * nblk + b]);
+    out[m] = v;
 }
 
 // ---- CSC export (src/multifrontal.jl:277-350) ----
@@ -453,6 +521,11 @@ hipError_t launch_solve_level(int which, const SolveArgs &a, const int32_t *list
     case 3: hipLaunchKernelGGL(k_rect_part<T>, g, blk, 0, st, b, 1); break;
     case 4: hipLaunchKernelGGL(k_rect_sum<T>, g, blk, 0, st, b, 1); break;
     case 5: hipLaunchKernelGGL(k_bwd_tri<T>, g, blk, 0, st, b); break;
+    case 6: hipLaunchKernelGGL(k_rect_part<T>, g, blk, 0, st, b, 2); break;
+    case 7: hipLaunchKernelGGL(k_rect_sum<T>, g, blk, 0, st, b, 2); break;
+    case 8: hipLaunchKernelGGL(k_rect_part<T>, g, blk, 0, st, b, 3); break;
+    case 9: hipLaunchKernelGGL(k_rect_sum<T>, g, blk, 0, st, b, 3); break;
+    case 10: hipLaunchKernelGGL(k_fwd_gather<T>, g, blk, 0, st, b, a.ecolptr, a.ext); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -484,17 +557,20 @@ hipError_t launch_residual(const int32_t *rp, const int32_t *ci, const int32_t *
                        R, n, nrhs, msx, msb);
     return hipGetLastError();
 }
-hipError_t launch_norms(const double *R, const double *X, int64_t ldx, const double *B, int64_t ldb, int64_t n,
-                        int64_t nrhs, int64_t msx, int64_t msb, double *out, int nmat, hipStream_t st)
+hipError_t launch_norms(const double *R, const double *X, const double *B, int64_t n, int64_t nrhs, double *part,
+                        double *out, int nmat, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_norms, dim3((unsigned)nrhs, nmat), dim3(256), 0, st, R, X, ldx, B, ldb, n, nrhs, msx, msb,
-                       out);
+    hipLaunchKernelGGL(k_norms_part, dim3(NORM_BLOCKS, (unsigned)nrhs, nmat), dim3(256), 0, st, R, X, B, n, nrhs, part);
+    const int64_t ncols = nrhs * nmat;
+    hipLaunchKernelGGL(k_norms_fold, dim3((unsigned)((ncols + 63) / 64)), dim3(64), 0, st, part, ncols, out);
     return hipGetLastError();
 }
 hipError_t launch_anorm(const int32_t *rp, const int32_t *vmap, const double *ax, int64_t ax_stride, int64_t n,
-                        double *out, int nmat, hipStream_t st)
+                        double *part, double *out, int nmat, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_anorm, dim3(nmat), dim3(256), 0, st, rp, vmap, ax, ax_stride, n, out);
+    const int nblk = 256;
+    hipLaunchKernelGGL(k_anorm_part, dim3(nblk, nmat), dim3(256), 0, st, rp, vmap, ax, ax_stride, n, part);
+    hipLaunchKernelGGL(k_anorm_fold, dim3((nmat + 63) / 64), dim3(64), 0, st, part, nblk, nmat, out);
     return hipGetLastError();
 }
 hipError_t launch_rowfac(const int32_t *prow, int32_t *rowfac, int64_t n, hipStream_t st)
