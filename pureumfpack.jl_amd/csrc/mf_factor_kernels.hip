@@ -372,54 +372,89 @@ __global__ __launch_bounds__(256) void k_panel_apply(FactorArgs a, int k)
 // pivoted one.  These kernels factor a 64-column block at a time without interchanges and
 // check that condition on every multiplier; a violation sets a per-matrix flag and the
 // factorization is redone on the pivoted schedule (k_panel / k_panel_apply).
-//   k_spec_diag: one workgroup per front factors the kb x kb diagonal block in LDS.
-//   k_spec_trsm: many workgroups; item int4 {front, kind, start, 0}:
-//     kind 0: rows r in [start, start+256) of [k+kb, nf): L[r, k:k+kb] = A[r, k:k+kb] U_kk^-1,
+//   k_spec_diag: one wave per front factors the kb x kb diagonal block in registers.
+//   k_spec_trsm: many workgroups; item int4 {front, kind, start, slot}:
+//     kind 0: rows r in [start, start+SPEC_ROWS) of [k+kb, nf): L[r, k:k+kb] = A[r, k:k+kb] U_kk^-1,
 //             checking the multiplier bound on the candidate rows r < np;
-//     kind 1: columns c in [start, start+256) of [k+kb, nf): U[k:k+kb, c] = L_kk^-1 A[k:k+kb, c].
+//     kind 1: columns c in [start, start+SPEC_ROWS) of [k+kb, nf): U[k:k+kb, c] = L_kk^-1 A[k:k+kb, c];
+//     slot: the front's position in the diagonal launch (its packed block in a.spec).
 // ------------------------------------------------------------------------------------------
+// value held by lane l of the wave (l uniform), via v_readlane
+__device__ __forceinline__ double lane_value(double v, int l)
+{
+    const unsigned long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const int hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __longlong_as_double(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float lane_value(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// a value known to be equal in all lanes, moved to scalar registers
+__device__ __forceinline__ double wave_uniform(double v)
+{
+    const unsigned long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(unsigned)u);
+    const int hi = __builtin_amdgcn_readfirstlane((int)(unsigned)(u >> 32));
+    return __longlong_as_double(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float wave_uniform(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 template <typename T, int KB>
 __global__ __launch_bounds__(64) void k_spec_diag(FactorArgs a, int k, float tau)
 {
-    // one wave: lane i owns row i of the block; the pivot row goes through LDS (a single
-    // wave's LDS accesses complete in issue order, so no barrier is needed)
-    __shared__ T srow[KB];
+    // One wave: lane i owns row i of the block in registers; step j reads the pivot row from
+    // lane j with v_readlane (no memory round trip).  Control flow depends only on the pivot,
+    // which is uniform: rows a step must not change (finished rows, the pivot row) get
+    // multiplier 0.  The factored block is also written, packed for the trsm, to this front's
+    // slot of a.spec: U by rows, L by columns, the reciprocal pivots and a flag for a zero or
+    // tiny pivot (which getf2 treats apart: no scaling, or a division).
+    __shared__ T sblk[KB * (KB + 1)];
     const int mat = blockIdx.y;
     const SnDev d = a.sn[a.list[blockIdx.x]];
     const int np = d.np, nf = d.nf;
     const int kb = min(KB, np - k);
     const int lane = threadIdx.x;
     T *F = reinterpret_cast<T *>(a.pool) + mat * a.pool_stride + d.foff;
+    T *S = reinterpret_cast<T *>(a.spec) + mat * a.spec_stride + (int64_t)blockIdx.x * SPEC_SLOT;
     T x[KB];
 #pragma unroll
     for (int c = 0; c < KB; c++) x[c] = (lane < kb && c < kb) ? F[(k + lane) + (int64_t)(k + c) * nf] : T(0);
     const T itau = T(1) / T(tau);
-    int bad = 0;
+    int bad = 0, special = 0;
+    T myrcp = T(1);
 #pragma unroll
     for (int jc = 0; jc < KB / 32; jc++)
 #pragma unroll
     for (int jj = 0; jj < 32; jj++) {
         const int j = 32 * jc + jj;
         if (j >= kb) continue;
-        if (lane == j) {
-#pragma unroll
-            for (int c = 0; c < KB; c++)
-                if (c >= j) srow[c] = x[c];
+        const T piv = lane_value(x[j], j);
+        const bool act = lane > j && lane < kb;
+        T v = x[j];
+        if (act && fabs(v) > fabs(piv) * itau) bad = 1;
+        const T rp = T(1) / piv;
+        if (fabs(piv) >= Tiny<T>::v()) {
+            v = v * rp;
+        } else {
+            special = 1;
+            if (piv == T(0)) {
+                if (lane == 0) atomicMin(a.info + mat, d.c1 + k + j);
+            } else {
+                v = v / piv;
+            }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        const T piv = srow[j];
-        if (lane > j && lane < kb) {
-            T v = x[j];
-            if (fabs(v) > fabs(piv) * itau) bad = 1;
-            if (piv != T(0)) v = fabs(piv) < Tiny<T>::v() ? v / piv : v * (T(1) / piv);
-            x[j] = v;
+        myrcp = lane == j ? rp : myrcp;
+        const T l = act ? v : T(0);
+        x[j] = act ? v : x[j];
 #pragma unroll
-            for (int c = 0; c < KB; c++)
-                if (c > j) x[c] -= v * srow[c];
-        }
-        if (piv == T(0) && lane == 0) atomicMin(a.info + mat, d.c1 + k + j);
-        __builtin_amdgcn_wave_barrier();
+        for (int c = 0; c < KB; c++)
+            if (c > j) x[c] -= l * lane_value(x[c], j);
     }
     if (bad) atomicOr(a.flag + mat, 1);
     if (lane < kb) {
@@ -427,30 +462,103 @@ __global__ __launch_bounds__(64) void k_spec_diag(FactorArgs a, int k, float tau
         for (int c = 0; c < KB; c++)
             if (c < kb) F[(k + lane) + (int64_t)(k + c) * nf] = x[c];
     }
+    // packed copy through LDS (row i at sblk[i * (KB + 1)]: conflict-free both ways)
+#pragma unroll
+    for (int c = 0; c < KB; c++) sblk[lane * (KB + 1) + c] = x[c];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int j = 0; j < KB; j++) {
+        // row j of U (columns lane >= j) and column j of L (rows lane > j)
+        const T u = (lane >= j && lane < kb && j < kb) ? sblk[j * (KB + 1) + lane] : T(0);
+        const T l = (lane > j && lane < kb && j < kb) ? sblk[lane * (KB + 1) + j] : T(0);
+        S[j * KB + lane] = u;
+        S[KB * KB + j * KB + lane] = l;
+    }
+    S[2 * KB * KB + lane] = lane < kb ? myrcp : T(1);
+    if (lane == 0) S[2 * KB * KB + KB] = special ? T(1) : T(0);
+}
+
+// Right-looking substitution on one row/column x held in registers (see k_spec_trsm).
+// LROWS: kind 0 (scale by the reciprocal pivots, check multipliers of candidate rows).
+// M and rc are staged in LDS (the same address for the whole wave: broadcast reads).
+template <typename T, int KB, bool LROWS>
+__device__ __forceinline__ int spec_solve_row(T (&x)[KB], const T *M, const T *rc, bool cand, T itau)
+{
+    int bad = 0;
+#pragma unroll
+    for (int jc = 0; jc < KB / 32; jc++)
+#pragma unroll
+    for (int jj = 0; jj < 32; jj++) {
+        const int j = 32 * jc + jj;
+        T v = x[j];
+        if (LROWS) {
+            bad |= (int)(cand && fabs(v) > fabs(M[j * KB + j]) * itau);
+            v = v * rc[j];                                // reciprocal scaling (getf2, trsm R/U/N/N)
+            x[j] = v;
+        }
+#pragma unroll
+        for (int c = 0; c < KB; c++)
+            if (c > j) x[c] -= v * M[j * KB + c];
+    }
+    return bad;
+}
+
+// The same substitution for a block holding a zero or tiny pivot, in dot-product form
+// (x_j -= x_q M[q][j] for q = 0..j-1: the same operation order), where getf2 leaves a
+// zero-pivot column unscaled and divides by a tiny pivot instead of scaling by its reciprocal.
+template <typename T, int KB>
+__device__ __forceinline__ int spec_solve_row_special(T (&x)[KB], const T *M, const T *rc, bool cand, T itau)
+{
+    int bad = 0;
+#pragma unroll
+    for (int jc = 0; jc < KB / 32; jc++)
+#pragma unroll
+    for (int jj = 0; jj < 32; jj++) {
+        const int j = 32 * jc + jj;
+        T v = x[j];
+#pragma unroll
+        for (int q = 0; q < 32 * (jc + 1); q++)
+            if (q < j) v -= x[q] * M[q * KB + j];
+        const T u = M[j * KB + j];
+        if (cand) {
+            if (fabs(v) > fabs(u) * itau) bad = 1;
+            if (u != T(0)) v = fabs(u) < Tiny<T>::v() ? v / u : v * rc[j];
+        } else {
+            v = v * rc[j];
+        }
+        x[j] = v;
+    }
+    return bad;
 }
 
 template <typename T, int KB>
-__global__ __launch_bounds__(256) void k_spec_trsm(FactorArgs a, int k, float tau)
+__global__ __launch_bounds__(SPEC_ROWS) void k_spec_trsm(FactorArgs a, int k, float tau,
+                                                         const T *__restrict__ spec)
 {
-    // M[q + j*KB]: kind 0 uses U_kk (M[q][j] = U(q, j)); kind 1 uses L_kk transposed
-    // (M[q][j] = L(j, q)), so both solve x_j <- (x_j - sum_{q<j} x_q M[q][j]) [/ M[j][j]].
-    __shared__ T sd[KB * KB];
+    // Right-looking substitution, one row (kind 0) or column (kind 1) per thread:
+    //   for j: finish x_j, then x_c -= x_j M[j][c] for all c > j,
+    // with M[j][c] = U(j, c) for kind 0 and L(c, j) for kind 1, staged in LDS from the packed
+    // block that k_spec_diag left in this front's slot.  Every x_c receives its updates in the order
+    // j = 0, 1, ..., the same sequence of operations as the row-by-row dot-product form.
+    __shared__ T sm[KB * KB + 2 * KB];
     const int mat = blockIdx.y;
     const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
     const SnDev d = a.sn[w.x];
     const int np = d.np, nf = d.nf;
     const int kb = min(KB, np - k);
-    const int tid = threadIdx.x;
     const bool lrows = w.y == 0;
-    T *F = reinterpret_cast<T *>(a.pool) + mat * a.pool_stride + d.foff;
-    for (int e = tid; e < KB * KB; e += blockDim.x) {
-        const int i = e % KB, j = e / KB;
-        const int r = lrows ? i : j, c = lrows ? j : i;
-        sd[e] = (i < kb && j < kb) ? F[(k + r) + (int64_t)(k + c) * nf] : (i == j ? T(1) : T(0));
+    {
+        const T *S = spec + mat * a.spec_stride + (int64_t)w.w * SPEC_SLOT;
+        const T *Mg = S + (lrows ? 0 : KB * KB);
+        for (int e = threadIdx.x; e < KB * KB; e += SPEC_ROWS) sm[e] = Mg[e];
+        for (int e = threadIdx.x; e < 2 * KB; e += SPEC_ROWS) sm[KB * KB + e] = S[2 * KB * KB + e];
     }
     __syncthreads();
-    const int idx = k + kb + w.z + tid;   // row (kind 0) or column (kind 1) index in the front
+    const T *rc = sm + KB * KB;
+    const int idx = k + kb + w.z + threadIdx.x;   // row (kind 0) or column (kind 1) in the front
     if (idx >= nf) return;
+    T *F = reinterpret_cast<T *>(a.pool) + mat * a.pool_stride + d.foff;
     const bool cand = lrows && idx < np;
     const T itau = T(1) / T(tau);
     // kind 0 reads row idx across the block columns (stride nf); kind 1 reads column idx
@@ -458,37 +566,32 @@ __global__ __launch_bounds__(256) void k_spec_trsm(FactorArgs a, int k, float ta
     T *base = lrows ? F + idx + (int64_t)k * nf : F + (int64_t)idx * nf + k;
     const int64_t stride = lrows ? nf : 1;
     T x[KB];
+    if (kb == KB) {   // full block: unconditional loads off one running pointer
+        const T *p = base;
 #pragma unroll
-    for (int j = 0; j < KB; j++) x[j] = j < kb ? base[j * stride] : T(0);
-    int bad = 0;
-    // j = 32 * jc + jj, written as two static loop levels so the compiler unrolls them fully
-    // (a single 64-trip loop with a 64-trip inner loop is not unrolled and spills x)
+        for (int j = 0; j < KB; j++, p += stride) x[j] = *p;
+    } else {
 #pragma unroll
-    for (int jc = 0; jc < KB / 32; jc++)
-#pragma unroll
-    for (int jj = 0; jj < 32; jj++) {
-        const int j = 32 * jc + jj;
-        asm volatile("" ::: "memory");  // keep the block in LDS (no hoisting into registers)
-        T v = x[j];
-#pragma unroll
-        for (int q = 0; q < 32 * (jc + 1); q++)
-            if (q < j) v -= x[q] * sd[q + j * KB];
-        if (lrows) {
-            const T u = sd[j + j * KB];
-            if (cand) {
-                if (fabs(v) > fabs(u) * itau) bad = 1;
-                // getf2 scaling of a candidate row: reciprocal unless the pivot is zero or tiny
-                if (u != T(0)) v = fabs(u) < Tiny<T>::v() ? v / u : v * (T(1) / u);
-            } else {
-                v = v * (T(1) / u);  // trsm R/U/N/N
-            }
-        }
-        x[j] = v;
+        for (int j = 0; j < KB; j++) x[j] = j < kb ? base[j * stride] : T(0);
     }
+    int bad = 0;
+    if (!lrows)
+        spec_solve_row<T, KB, false>(x, sm, rc, false, itau);
+    else if (rc[KB] == T(0))
+        bad = spec_solve_row<T, KB, true>(x, sm, rc, cand, itau);
+    else
+        bad = spec_solve_row_special<T, KB>(x, sm, rc, cand, itau);
     if (bad) atomicOr(a.flag + mat, 1);
+    T *p = base;
+    asm volatile("" : "+v"(p));  // a fresh pointer, so the load addresses are not kept live
+    if (kb == KB) {
 #pragma unroll
-    for (int j = 0; j < KB; j++)
-        if (j < kb) base[j * stride] = x[j];
+        for (int j = 0; j < KB; j++, p += stride) *p = x[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < KB; j++)
+            if (j < kb) p[j * stride] = x[j];
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -695,7 +798,8 @@ hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int
         hipLaunchKernelGGL((k_spec_diag<T, SPEC_KB>), grid, dim3(64), 0, st, a, L.k, L.tau);
         break;
     case L_SPEC_TRSM:
-        hipLaunchKernelGGL((k_spec_trsm<T, SPEC_KB>), grid, dim3(256), 0, st, a, L.k, L.tau);
+        hipLaunchKernelGGL((k_spec_trsm<T, SPEC_KB>), grid, dim3(SPEC_ROWS), 0, st, a, L.k, L.tau,
+                           reinterpret_cast<const T *>(a.spec));
         break;
     case L_GEMM:
         hipLaunchKernelGGL(k_gemm<T>, grid, dim3(256), 0, st, probs + L.prob_off,

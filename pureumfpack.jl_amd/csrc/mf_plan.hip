@@ -24,6 +24,7 @@
 using namespace mf;
 
 static constexpr int64_t SOLVE_CK_HOST = 256;  // column chunk of the split-K solve products (k_rect_part)
+static constexpr int64_t FRONT_MAXNF_HOST = 1024;  // largest front of the fused per-front sweeps
 
 static thread_local std::string g_err;
 
@@ -97,6 +98,9 @@ struct mf_plan {
     std::vector<int32_t> lvl_fpo, lvl_fpc, lvl_fso, lvl_fsc, lvl_bpo, lvl_bpc, lvl_bso, lvl_bsc;
     std::vector<int32_t> lvl_ilpo, lvl_ilpc, lvl_ilso, lvl_ilsc, lvl_iupo, lvl_iupc, lvl_iuso, lvl_iusc;
     std::vector<int32_t> lvl_go, lvl_gc;   // forward gather items {front, pos0}
+    // fronts swept by the fused per-front kernels (no stored inverse): nf <= 64 and
+    // 64 < nf <= FRONT_MAXNF; the level lists above hold only the other fronts
+    std::vector<int32_t> lvl_s64o, lvl_s64c, lvl_s1ko, lvl_s1kc;
     int64_t front_elems = 0, inv_elems = 0, tmp_elems = 0;  // pool = fronts | inverses | gemm temps
     int64_t part_slots = 0;  // partial-sum slots (256 rows x nrhs each) the solve needs
     std::vector<double> lvl_lflops, lvl_uflops, lvl_lbytes, lvl_ubytes, lvl_vec;
@@ -104,7 +108,8 @@ struct mf_plan {
     DevBuf d_sn, d_upd, d_rmap, d_child, d_orow, d_ocol, d_osrc, d_ocolptr, d_ecolptr, d_ext, d_arow;
     DevBuf d_lists, d_probs, d_tiles, d_solvelist, d_qf;
     DevBuf d_csr_rp, d_csr_ci, d_csr_vmap;
-    DevBuf d_pool, d_prow, d_info, d_ax, d_rs, d_pinfo, d_flag;
+    DevBuf d_pool, d_prow, d_info, d_ax, d_rs, d_pinfo, d_flag, d_spec;
+    int64_t spec_slots = 0;         // most fronts in one speculative diagonal launch
     DevBuf d_w0, d_y, d_x, d_uv, d_part, d_zbuf, d_B, d_X, d_R, d_norms, d_anorm, d_npart;
     DevBuf d_Lp, d_Up, d_uoff, d_Li, d_Lx, d_Ui, d_Ux, d_rowfac;
     int64_t solve_nrhs = 0;
@@ -567,16 +572,18 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                     std::vector<int32_t> items;
                     for (int64_t s : large) {
                         if (np[s] <= k) continue;
+                        const int32_t slot = (int32_t)(lists.size() - Dg.list_off);
                         lists.push_back((int32_t)s);
+                        P.spec_slots = std::max<int64_t>(P.spec_slots, slot + 1);
                         int64_t kb = std::min<int64_t>(SPEC_KB, np[s] - k), k2 = k + kb;
                         double b = (double)kb;
                         Dg.flops += 2.0 * b * b * b / 3.0;
                         Dg.bytes += 2.0 * b * b * elem;
-                        for (int64_t r0 = 0; r0 < nf[s] - k2; r0 += 256) {
-                            items.push_back((int32_t)s); items.push_back(0); items.push_back((int32_t)r0); items.push_back(0);
+                        for (int64_t r0 = 0; r0 < nf[s] - k2; r0 += SPEC_ROWS) {
+                            items.push_back((int32_t)s); items.push_back(0); items.push_back((int32_t)r0); items.push_back(slot);
                         }
-                        for (int64_t c0 = 0; c0 < nf[s] - k2; c0 += 256) {
-                            items.push_back((int32_t)s); items.push_back(1); items.push_back((int32_t)c0); items.push_back(0);
+                        for (int64_t c0 = 0; c0 < nf[s] - k2; c0 += SPEC_ROWS) {
+                            items.push_back((int32_t)s); items.push_back(1); items.push_back((int32_t)c0); items.push_back(slot);
                         }
                         if (k2 < np[s]) {
                             GemmProb g{};
@@ -616,7 +623,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                         for (size_t q = 0; q < items.size(); q += 4) {
                             int64_t s = items[q];
                             double b = (double)std::min<int64_t>(SPEC_KB, np[s] - k);
-                            double cnt = (double)std::min<int64_t>(256, nf[s] - (k + (int64_t)b) - items[q + 2]);
+                            double cnt = (double)std::min<int64_t>(SPEC_ROWS, nf[s] - (k + (int64_t)b) - items[q + 2]);
                             Tr.flops += cnt * b * b;
                             Tr.bytes += 2.0 * cnt * b * elem;
                         }
@@ -807,14 +814,25 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
         slist.push_back(a0); slist.push_back(a1); slist.push_back(a2); slist.push_back(a3);
     };
     for (int64_t lv = 0; lv < nlev; lv++) {
+        std::vector<int64_t> big;
+        P.lvl_s64o.push_back((int32_t)slist.size());
+        for (int64_t s : bylevel[lv])
+            if (ioff[s] < 0 && nf[s] <= 64) slist.push_back((int32_t)s);
+        P.lvl_s64c.push_back((int32_t)slist.size() - P.lvl_s64o.back());
+        P.lvl_s1ko.push_back((int32_t)slist.size());
+        for (int64_t s : bylevel[lv])
+            if (ioff[s] < 0 && nf[s] > 64 && nf[s] <= FRONT_MAXNF_HOST) slist.push_back((int32_t)s);
+        P.lvl_s1kc.push_back((int32_t)slist.size() - P.lvl_s1ko.back());
+        for (int64_t s : bylevel[lv])
+            if (ioff[s] >= 0 || nf[s] > FRONT_MAXNF_HOST) big.push_back(s);
         P.lvl_off[lv] = (int32_t)slist.size();
-        for (int64_t s : bylevel[lv]) slist.push_back((int32_t)s);
-        P.lvl_cnt[lv] = (int32_t)bylevel[lv].size();
+        for (int64_t s : big) slist.push_back((int32_t)s);
+        P.lvl_cnt[lv] = (int32_t)big.size();
         // forward: rows t < nu of L21 against columns a < np; backward: rows i < np of U12
         // against columns t < nu.  Partial slots are reused level by level.
         while (slist.size() & 3) slist.push_back(0);
         P.lvl_go.push_back((int32_t)slist.size());
-        for (int64_t s : bylevel[lv])
+        for (int64_t s : big)
             for (int64_t p0 = 0; p0 < nf[s]; p0 += 256) emit4((int32_t)s, (int32_t)p0, 0, 0);
         P.lvl_gc.push_back((int32_t)((slist.size() - P.lvl_go.back()) / 4));
         // dir 0: L21 (rows nu, cols np); 1: U12 (rows np, cols nu); 2: inv(L11) and 3: inv(U11)
@@ -824,7 +842,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             std::vector<int32_t> sums;
             while (slist.size() & 3) slist.push_back(0);
             int32_t po = (int32_t)slist.size();
-            for (int64_t s : bylevel[lv]) {
+            for (int64_t s : big) {
                 if (dir >= 2 && ioff[s] < 0) continue;
                 int64_t nrows = dir == 0 ? nu[s] : np[s], ncols = dir == 1 ? nu[s] : np[s];
                 for (int64_t r0 = 0; r0 < nrows; r0 += 256) {
@@ -936,6 +954,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
     HIPCHK(P.d_info.alloc((size_t)4 * P.nbatch));
     HIPCHK(P.d_pinfo.alloc((size_t)4 * PINFO_STRIDE * nsuper * P.nbatch));
     HIPCHK(P.d_flag.alloc((size_t)4 * P.nbatch));
+    HIPCHK(P.d_spec.alloc((size_t)std::max<int64_t>(1, P.spec_slots) * SPEC_SLOT * P.esize() * P.nbatch));
     HIPCHK(P.d_ax.alloc((size_t)P.nnz * 8 * P.nbatch));
     HIPCHK(P.d_rs.alloc((size_t)n * 8 * P.nbatch));
     HIPCHK(P.d_anorm.alloc((size_t)8 * P.nbatch));
@@ -1028,6 +1047,8 @@ static int run_factor(mf_plan &P, const double *d_ax, const double *d_rs, int nm
     a.info = P.d_info.as<int32_t>();
     a.pinfo = P.d_pinfo.as<int32_t>();
     a.flag = P.d_flag.as<int32_t>();
+    a.spec = P.d_spec.p;
+    a.spec_stride = P.spec_slots * SPEC_SLOT;
     a.pinfo_stride = (int64_t)PINFO_STRIDE * P.nsuper;
     a.pool_stride = P.pool_elems;
     a.ax_stride = P.nnz;
@@ -1181,6 +1202,15 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
     const int32_t *sl = P.d_solvelist.as<int32_t>();
     for (int64_t lv = 0; lv < P.nlevels; lv++) {
         const double vb = 3.0 * e * P.lvl_vec[lv] * nrhs;
+        if (P.lvl_s64c[lv] > 0)
+            HIPCHK(profiled(P, 5, 0.0, 0.0, [&] {
+                return launch_solve_level<T>(11, a, sl, P.lvl_s64o[lv], P.lvl_s64c[lv], nmat, P.stream);
+            }));
+        if (P.lvl_s1kc[lv] > 0)
+            HIPCHK(profiled(P, 5, 0.0, 0.0, [&] {
+                return launch_solve_level<T>(12, a, sl, P.lvl_s1ko[lv], P.lvl_s1kc[lv], nmat, P.stream);
+            }));
+        if (P.lvl_cnt[lv] == 0) continue;
         HIPCHK(profiled(P, 5, 0.0, vb * nmat, [&] {
             return launch_solve_level<T>(10, a, sl, P.lvl_go[lv], P.lvl_gc[lv], nmat, P.stream);
         }));
@@ -1206,6 +1236,15 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
     }
     for (int64_t lv = P.nlevels - 1; lv >= 0; lv--) {
         const double vb = 3.0 * e * P.lvl_vec[lv] * nrhs;
+        if (P.lvl_s64c[lv] > 0)
+            HIPCHK(profiled(P, 6, 0.0, 0.0, [&] {
+                return launch_solve_level<T>(13, a, sl, P.lvl_s64o[lv], P.lvl_s64c[lv], nmat, P.stream);
+            }));
+        if (P.lvl_s1kc[lv] > 0)
+            HIPCHK(profiled(P, 6, 0.0, 0.0, [&] {
+                return launch_solve_level<T>(14, a, sl, P.lvl_s1ko[lv], P.lvl_s1kc[lv], nmat, P.stream);
+            }));
+        if (P.lvl_cnt[lv] == 0) continue;
         if (P.lvl_bpc[lv] > 0)
             HIPCHK(profiled(P, 6, P.lvl_uflops[lv] * nrhs * nmat, (P.lvl_ubytes[lv] + vb) * nmat, [&] {
                 return launch_solve_level<T>(3, a, sl, P.lvl_bpo[lv], P.lvl_bpc[lv], nmat, P.stream);

@@ -330,6 +330,352 @@ __global__ __launch_bounds__(256) void k_bwd_tri(SolveArgs a)
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// Fused per-front sweeps for fronts without stored inverses and nf <= FRONT_MAXNF: one launch
+// per level and sweep instead of gather / triangle / split-K product / sum.
+//   forward:  gather the children's update vectors (the sums of k_fwd_gather), permute the
+//             pivot part by the block pivoting, then z -= [L11; L21](:, j) z_j for
+//             j = 0 .. np-1; y <- pivot part, this front's update vector <- the rest.
+//   backward: s = y_piv - U12 x_upd, then back substitution with U11.
+// k_*_front_w: one wave per front (nf <= 64), the front vector in registers, exchanges by
+// shuffles.  k_*_front_b: one 256-thread workgroup per front, the front vector in LDS,
+// diagonal blocks of 64 staged in LDS and solved by one wave, products with 16 loads in
+// flight per thread.
+// ------------------------------------------------------------------------------------------
+constexpr int FRONT_MAXNF = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_fwd_front_w(SolveArgs a, const int32_t *__restrict__ ecolptr,
+                                                    const int2 *__restrict__ ext)
+{
+    const int mat = blockIdx.y;
+    const SnDev d = a.sn[a.list[blockIdx.x]];
+    const int nf = d.nf, np = d.np, c1 = d.c1;
+    const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
+    const int32_t *prow = a.prow + mat * a.n_stride;
+    const int p = threadIdx.x;
+    const bool on = p < nf;
+    const int64_t vb = (int64_t)mat * a.nrhs;
+    const T *w0 = reinterpret_cast<const T *>(a.w0) + vb * a.n;
+    T *y = reinterpret_cast<T *>(a.y) + vb * a.n;
+    T *uv = reinterpret_cast<T *>(a.uv) + vb * a.nupd;
+    const int nrhs = (int)a.nrhs;
+    const int q0 = on ? ecolptr[d.fcol + p] : 0, q1 = on ? ecolptr[d.fcol + p + 1] : 0;
+    const int src = p < np ? prow[c1 + p] - c1 : p;   // V-row feeding factor position p
+    for (int r0 = 0; r0 < nrhs; r0 += SOLVE_RC) {
+        const int rc = min(SOLVE_RC, nrhs - r0);
+        T v[SOLVE_RC];
+#pragma unroll
+        for (int r = 0; r < SOLVE_RC; r++)
+            v[r] = (r < rc && p < np) ? w0[(int64_t)(r0 + r) * a.n + c1 + p] : T(0);
+        for (int q = q0; q < q1; q++) {
+            const int2 e = ext[q];
+            const int32_t off = a.sn[e.x].upd_off + e.y;
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++)
+                if (r < rc) v[r] += uv[(int64_t)(r0 + r) * a.nupd + off];
+        }
+#pragma unroll
+        for (int r = 0; r < SOLVE_RC; r++) v[r] = __shfl(v[r], src);
+        const T *row = F + p;
+#pragma unroll 16
+        for (int j = 0; j < 64; j++) {
+            if (j >= np) break;
+            const T l = (on && p > j) ? row[(int64_t)j * nf] : T(0);
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) v[r] -= l * __shfl(v[r], j);
+        }
+        if (on) {
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) {
+                if (r >= rc) continue;
+                if (p < np) y[(int64_t)(r0 + r) * a.n + c1 + p] = v[r];
+                else uv[(int64_t)(r0 + r) * a.nupd + d.upd_off + p - np] = v[r];
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_bwd_front_w(SolveArgs a)
+{
+    const int mat = blockIdx.y;
+    const SnDev d = a.sn[a.list[blockIdx.x]];
+    const int nf = d.nf, np = d.np, nu = d.nu, c1 = d.c1;
+    const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
+    const int p = threadIdx.x;
+    const int64_t vb = (int64_t)mat * a.nrhs;
+    const T *y = reinterpret_cast<const T *>(a.y) + vb * a.n;
+    T *x = reinterpret_cast<T *>(a.x) + vb * a.n;
+    const int nrhs = (int)a.nrhs;
+    const int64_t src = p < np ? c1 + p : (p < nf ? a.upd[d.upd_off + p - np] : 0);
+    for (int r0 = 0; r0 < nrhs; r0 += SOLVE_RC) {
+        const int rc = min(SOLVE_RC, nrhs - r0);
+        T v[SOLVE_RC];
+#pragma unroll
+        for (int r = 0; r < SOLVE_RC; r++)
+            v[r] = (r < rc && p < nf) ? (p < np ? y : x)[(int64_t)(r0 + r) * a.n + src] : T(0);
+        // s_i -= sum_t U12[i, t] x_upd[t] (the update values sit in lanes np + t)
+        T acc[SOLVE_RC] = {};
+        const T *urow = F + p + (int64_t)np * nf;
+#pragma unroll 16
+        for (int t = 0; t < 64; t++) {
+            if (t >= nu) break;
+            const T u = p < np ? urow[(int64_t)t * nf] : T(0);
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) acc[r] += u * __shfl(v[r], np + t);
+        }
+#pragma unroll
+        for (int r = 0; r < SOLVE_RC; r++) v[r] -= acc[r];
+        const T *col = F + p;
+#pragma unroll 16
+        for (int jj = 0; jj < 64; jj++) {
+            const int j = np - 1 - jj;
+            if (j < 0) break;
+            const T ujj = F[j + (int64_t)j * nf];
+            const T u = p < j ? col[(int64_t)j * nf] : T(0);
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) {
+                if (p == j) v[r] = v[r] / ujj;
+                v[r] -= u * __shfl(v[r], j);
+            }
+        }
+        if (p < np) {
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++)
+                if (r < rc) x[(int64_t)(r0 + r) * a.n + c1 + p] = v[r];
+        }
+    }
+}
+
+// acc[r] += sum_{j < bs} M[j * ld] * s[r * sld + j], 16 loads in flight
+template <typename T>
+__device__ __forceinline__ void dot_rows16(const T *M, int64_t ld, int bs, const T *s, int sld, T (&acc)[SOLVE_RC])
+{
+    int j = 0;
+    for (; j + 16 <= bs; j += 16) {
+        T l[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) l[q] = M[(int64_t)(j + q) * ld];
+#pragma unroll
+        for (int q = 0; q < 16; q++)
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) acc[r] += l[q] * s[r * sld + j + q];
+    }
+    for (; j < bs; j++) {
+        const T l = M[(int64_t)j * ld];
+#pragma unroll
+        for (int r = 0; r < SOLVE_RC; r++) acc[r] += l * s[r * sld + j];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_front_b(SolveArgs a, const int32_t *__restrict__ ecolptr,
+                                                     const int2 *__restrict__ ext)
+{
+    constexpr int NT = 256, MAXNF = FRONT_MAXNF, PT = MAXNF / NT, LDD = SOLVE_NB + 1;
+    __shared__ T g[SOLVE_RC * MAXNF];            // g[r * MAXNF + p]
+    __shared__ T sd[SOLVE_NB * LDD];             // diagonal block of L11
+    const int mat = blockIdx.y;
+    const SnDev d = a.sn[a.list[blockIdx.x]];
+    const int nf = d.nf, np = d.np, c1 = d.c1;
+    const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
+    const int32_t *prow = a.prow + mat * a.n_stride;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int64_t vb = (int64_t)mat * a.nrhs;
+    const T *w0 = reinterpret_cast<const T *>(a.w0) + vb * a.n;
+    T *y = reinterpret_cast<T *>(a.y) + vb * a.n;
+    T *uv = reinterpret_cast<T *>(a.uv) + vb * a.nupd;
+    const int nrhs = (int)a.nrhs;
+    for (int r0 = 0; r0 < nrhs; r0 += SOLVE_RC) {
+        const int rc = min(SOLVE_RC, nrhs - r0);
+        for (int p = tid; p < nf; p += NT) {
+            const int q0 = ecolptr[d.fcol + p], q1 = ecolptr[d.fcol + p + 1];
+            T acc[SOLVE_RC];
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++)
+                acc[r] = (r < rc && p < np) ? w0[(int64_t)(r0 + r) * a.n + c1 + p] : T(0);
+            for (int q = q0; q < q1; q++) {
+                const int2 e = ext[q];
+                const int32_t off = a.sn[e.x].upd_off + e.y;
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++)
+                    if (r < rc) acc[r] += uv[(int64_t)(r0 + r) * a.nupd + off];
+            }
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + p] = acc[r];
+        }
+        __syncthreads();
+        T t[PT][SOLVE_RC];
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            const int i = tid + k * NT;
+            if (i < np) {
+                const int src = prow[c1 + i] - c1;
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) t[k][r] = g[r * MAXNF + src];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            const int i = tid + k * NT;
+            if (i < np) {
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + i] = t[k][r];
+            }
+        }
+        for (int b0 = 0; b0 < np; b0 += SOLVE_NB) {
+            const int bs = min(SOLVE_NB, np - b0);
+            for (int e = tid; e < SOLVE_NB * SOLVE_NB; e += NT) {
+                const int i = e % SOLVE_NB, j = e / SOLVE_NB;
+                sd[i + j * LDD] = (i < bs && j < bs) ? F[(b0 + i) + (int64_t)(b0 + j) * nf] : T(0);
+            }
+            __syncthreads();
+            if (tid < 64) {
+                T v[SOLVE_RC];
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) v[r] = lane < bs ? g[r * MAXNF + b0 + lane] : T(0);
+                for (int j = 0; j < bs; j++) {
+                    const T l = lane > j ? sd[lane + j * LDD] : T(0);
+#pragma unroll
+                    for (int r = 0; r < SOLVE_RC; r++) v[r] -= l * __shfl(v[r], j);
+                }
+                if (lane < bs) {
+#pragma unroll
+                    for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + b0 + lane] = v[r];
+                }
+            }
+            __syncthreads();
+            for (int p = b0 + bs + tid; p < nf; p += NT) {
+                T acc[SOLVE_RC] = {};
+                dot_rows16<T>(F + p + (int64_t)b0 * nf, nf, bs, g + b0, MAXNF, acc);
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + p] -= acc[r];
+            }
+            __syncthreads();
+        }
+        for (int p = tid; p < nf; p += NT) {
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) {
+                if (r >= rc) continue;
+                if (p < np) y[(int64_t)(r0 + r) * a.n + c1 + p] = g[r * MAXNF + p];
+                else uv[(int64_t)(r0 + r) * a.nupd + d.upd_off + p - np] = g[r * MAXNF + p];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_front_b(SolveArgs a)
+{
+    constexpr int NT = 256, MAXNF = FRONT_MAXNF, LDD = SOLVE_NB + 1;
+    __shared__ T g[SOLVE_RC * MAXNF];            // s at [0, np), x of the update rows at np + t
+    __shared__ T sd[SOLVE_NB * LDD];             // diagonal block of U11
+    __shared__ T red[NT * SOLVE_RC];             // partial sums of the U12 product
+    const int mat = blockIdx.y;
+    const SnDev d = a.sn[a.list[blockIdx.x]];
+    const int nf = d.nf, np = d.np, nu = d.nu, c1 = d.c1;
+    const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
+    const int32_t *upd = a.upd + d.upd_off;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int64_t vb = (int64_t)mat * a.nrhs;
+    const T *y = reinterpret_cast<const T *>(a.y) + vb * a.n;
+    T *x = reinterpret_cast<T *>(a.x) + vb * a.n;
+    const int nrhs = (int)a.nrhs;
+    // U12 product layout: thread (i, ts) with i = tid % npad over rows, ts = tid / npad over
+    // columns t = ts, ts + ts_n, ...; the ts_n partial sums of a row are added in ts order
+    int npad = 1;
+    while (npad < np && npad < NT) npad <<= 1;
+    const int ts_n = NT / npad, ti = tid % npad, ts = tid / npad;
+    for (int r0 = 0; r0 < nrhs; r0 += SOLVE_RC) {
+        const int rc = min(SOLVE_RC, nrhs - r0);
+        for (int p = tid; p < nf; p += NT) {
+            const int64_t src = p < np ? c1 + p : upd[p - np];
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++)
+                g[r * MAXNF + p] = r < rc ? (p < np ? y : x)[(int64_t)(r0 + r) * a.n + src] : T(0);
+        }
+        __syncthreads();
+        for (int i0 = 0; i0 < np; i0 += npad) {
+            const int i = i0 + ti;
+            T acc[SOLVE_RC] = {};
+            if (i < np) {
+                const T *row = F + i + (int64_t)(np + ts) * nf;
+                const int cnt = nu > ts ? (nu - ts + ts_n - 1) / ts_n : 0;   // columns ts + k * ts_n
+                int k = 0;
+                for (; k + 8 <= cnt; k += 8) {
+                    T u[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) u[q] = row[(int64_t)(k + q) * ts_n * nf];
+#pragma unroll
+                    for (int q = 0; q < 8; q++)
+#pragma unroll
+                        for (int r = 0; r < SOLVE_RC; r++) acc[r] += u[q] * g[r * MAXNF + np + ts + (k + q) * ts_n];
+                }
+                for (; k < cnt; k++) {
+                    const T u = row[(int64_t)k * ts_n * nf];
+#pragma unroll
+                    for (int r = 0; r < SOLVE_RC; r++) acc[r] += u * g[r * MAXNF + np + ts + k * ts_n];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) red[r * NT + tid] = acc[r];
+            __syncthreads();
+            if (tid < npad && i0 + tid < np) {
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) {
+                    T s = T(0);
+                    for (int q = 0; q < ts_n; q++) s += red[r * NT + q * npad + tid];
+                    g[r * MAXNF + i0 + tid] -= s;
+                }
+            }
+            __syncthreads();
+        }
+        for (int b0 = ((np - 1) / SOLVE_NB) * SOLVE_NB; b0 >= 0; b0 -= SOLVE_NB) {
+            const int bs = min(SOLVE_NB, np - b0);
+            for (int e = tid; e < SOLVE_NB * SOLVE_NB; e += NT) {
+                const int i = e % SOLVE_NB, j = e / SOLVE_NB;
+                sd[i + j * LDD] = (i < bs && j < bs) ? F[(b0 + i) + (int64_t)(b0 + j) * nf] : T(0);
+            }
+            __syncthreads();
+            if (tid < 64) {
+                T v[SOLVE_RC];
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) v[r] = lane < bs ? g[r * MAXNF + b0 + lane] : T(0);
+                for (int j = bs - 1; j >= 0; j--) {
+                    const T ujj = sd[j + j * LDD];
+                    const T u = lane < j ? sd[lane + j * LDD] : T(0);
+#pragma unroll
+                    for (int r = 0; r < SOLVE_RC; r++) {
+                        if (lane == j) v[r] = v[r] / ujj;
+                        v[r] -= u * __shfl(v[r], j);
+                    }
+                }
+                if (lane < bs) {
+#pragma unroll
+                    for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + b0 + lane] = v[r];
+                }
+            }
+            __syncthreads();
+            for (int i = tid; i < b0; i += NT) {
+                T acc[SOLVE_RC] = {};
+                dot_rows16<T>(F + i + (int64_t)b0 * nf, nf, bs, g + b0, MAXNF, acc);
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + i] -= acc[r];
+            }
+            __syncthreads();
+        }
+        for (int i = tid; i < np; i += NT) {
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++)
+                if (r < rc) x[(int64_t)(r0 + r) * a.n + c1 + i] = g[r * MAXNF + i];
+        }
+        __syncthreads();
+    }
+}
+
 // w0[i] = Rs[qf[i]] * B[qf[i]] in the solve type: the scale-and-permute of
 // src/interface.jl:128-130 expressed in V-row order.
 template <typename T>
@@ -526,6 +872,10 @@ hipError_t launch_solve_level(int which, const SolveArgs &a, const int32_t *list
     case 8: hipLaunchKernelGGL(k_rect_part<T>, g, blk, 0, st, b, 3); break;
     case 9: hipLaunchKernelGGL(k_rect_sum<T>, g, blk, 0, st, b, 3); break;
     case 10: hipLaunchKernelGGL(k_fwd_gather<T>, g, blk, 0, st, b, a.ecolptr, a.ext); break;
+    case 11: hipLaunchKernelGGL(k_fwd_front_w<T>, g, dim3(64), 0, st, b, a.ecolptr, a.ext); break;
+    case 12: hipLaunchKernelGGL(k_fwd_front_b<T>, g, blk, 0, st, b, a.ecolptr, a.ext); break;
+    case 13: hipLaunchKernelGGL(k_bwd_front_w<T>, g, dim3(64), 0, st, b); break;
+    case 14: hipLaunchKernelGGL(k_bwd_front_b<T>, g, blk, 0, st, b); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

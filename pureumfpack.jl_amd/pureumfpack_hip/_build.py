@@ -17,6 +17,9 @@ SOURCES = ["mf_analysis.cpp", "mf_plan.hip", "mf_factor_kernels.hip", "mf_solve_
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         # the 64-step register-resident substitutions are fully unrolled (constant register
+         # indices); the default pragma threshold stops short of that
+         "-mllvm", "-pragma-unroll-threshold=200000",
          "-Wall", "-Wno-unused-result"]
 
 
@@ -24,16 +27,21 @@ def build(verbose: bool = False) -> str:
     """Compile every source to an object, then link libmfgpu.so. Returns the library path."""
     objdir = os.path.join(ROOT, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in SOURCES:
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
-        if src.endswith(".cpp"):
-            cmd = [HIPCC, *FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+        lang = ["-x", "hip"] if src.endswith(".cpp") else []
+        cmds.append([HIPCC, *FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    # the sources compile independently: run them side by side
+    procs = []
+    for cmd in cmds:
         if verbose:
             print(" ".join(cmd), file=sys.stderr)
-        subprocess.run(cmd, check=True)
-        objs.append(obj)
+        procs.append(subprocess.Popen(cmd))
+    failed = [c for c, p in zip(cmds, procs) if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, failed[0])
     tmp = LIB_PATH + ".tmp"
     subprocess.run([HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp, *objs], check=True)
     os.replace(tmp, LIB_PATH)
