@@ -12,7 +12,10 @@ factor+solve seconds per step; the FP64 path, their ratio, the front TFLOP/s, th
 the dominant kernel (Schur gemm, from HIP events in a profiled pass) and a CPU baseline (the
 oracle's C restatement with OpenBLAS on the host cores) are reported beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--k 48] [--nrhs 8] [--no-cpu]
+A same-pattern batch (512 matrices on the poisson3d(16) pattern, 64 RHS each, FP32 fronts +
+FP64 refinement) is measured after it and reported under "batched".
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--k 48] [--nrhs 8] [--batch 512,16,64] [--no-cpu]
 """
 import argparse
 import ctypes
@@ -129,6 +132,67 @@ def run_gpu(hip, A, S, Rs, B, precision, steps, warmup, tol, max_refine, graph=F
     return res, X
 
 
+def run_batch(hip, k, nmat, nrhs, precision, steps, warmup, tol, max_refine, graph=True):
+    """Same-pattern batch: nmat matrices with the poisson3d(k) pattern and unsymmetric values
+    (diag 6 + U[0,1], off-diagonals -U[0.5,1.5]), nrhs randn right-hand sides each; one step =
+    mf_refactor_batch_device + mf_solve_batch_device refined to bwd err <= tol."""
+    lib = _lib.load()
+    P, Ax = M.poisson3d_batch_values(k, nmat, seed=4)
+    n = P.shape[0]
+    t0 = time.perf_counter()
+    B = I.BatchedLU(P, nmat, precision=precision, use_graph=graph)
+    Rs = np.stack([I.row_scaling(B.matrix(v), I.SCALE_SUM) for v in Ax])
+    t_plan = time.perf_counter() - t0
+    rhs = np.random.default_rng(3).standard_normal((nmat, nrhs, n))   # per matrix n x nrhs, column-major
+    dAx, dRs, dB = hip.upload(Ax), hip.upload(Rs), hip.upload(rhs)
+    dX = hip.upload(np.zeros_like(rhs))
+    info = np.zeros(nmat, np.int64)
+    rep = _lib.mf_solve_report()
+    plan = B._plan
+
+    def step():
+        a = time.perf_counter()
+        rc = lib.mf_refactor_batch_device(plan.h, nmat, dAx, dRs, info.ctypes.data_as(_lib.c_i64p))
+        if rc:
+            raise RuntimeError(lib.mf_last_error().decode())
+        b = time.perf_counter()
+        rc = lib.mf_solve_batch_device(plan.h, nmat, nrhs, dB, dX, max_refine, tol, ctypes.byref(rep))
+        if rc:
+            raise RuntimeError(lib.mf_last_error().decode())
+        return b - a, time.perf_counter() - b
+
+    for _ in range(warmup):
+        step()
+    hip.sync()
+    fac, sol = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f, s_ = step()
+        fac.append(f)
+        sol.append(s_)
+    hip.sync()
+    total = time.perf_counter() - t0
+    X = hip.download(dX, rhs.shape, order="C")
+    # independent check of every matrix's backward error (host, FP64)
+    worst = 0.0
+    for m in range(nmat):
+        A = B.matrix(Ax[m])
+        R = rhs[m].T - A @ X[m].T
+        den = abs(A).sum(axis=1).max() * np.abs(X[m]).max(axis=1) + np.abs(rhs[m]).max(axis=1)
+        worst = max(worst, float((np.abs(R).max(axis=0) / den).max()))
+    st = plan.stats()
+    for p_ in (dAx, dRs, dB, dX):
+        hip.free(p_)
+    fac_s = float(np.mean(fac))
+    return {"workload": f"{nmat} x poisson3d-{k} pattern (n={n}), unsymmetric values, {nrhs} RHS each, "
+                        f"{precision} fronts + FP64 refinement to bwd err <= {tol:g}, hipGraph={int(graph)}",
+            "seconds": total / steps, "factor_s": fac_s, "solve_s": float(np.mean(sol)),
+            "matrices_per_s": nmat * steps / total, "refine_iterations": int(rep.iterations),
+            "backward_error": float(rep.backward_error), "backward_error_host_check": worst,
+            "front_tflops": st["factor_flops"] * nmat / fac_s / 1e12, "plan_s": t_plan,
+            "launches_per_factor": st["launches"]}
+
+
 def cpu_baseline(A, S, Rs, B, refine):
     """The oracle's C restatement of the reference multifrontal path (FP64, dense fronts through
     OpenBLAS), timed on the same matrix: numeric factor + solve(F, b; refine) per RHS column.
@@ -171,6 +235,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--paths", default="fp64,fp32")
     ap.add_argument("--graph", type=int, default=1, help="replay the factor schedule as a hipGraph")
+    ap.add_argument("--batch", default="512,16,64", help="nmat,k,nrhs of the same-pattern batch run ('' = skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the numeric phase runs on a single MI355X "
@@ -208,6 +273,12 @@ def main():
         "speedup_vs_fp64": (results["fp64"]["seconds"] / R["seconds"]) if "fp64" in results else None,
         "roofline": roof,
     }
+    if args.batch:
+        nmat, kb, nrb = (int(v) for v in args.batch.split(","))
+        out["batched"] = run_batch(hip, kb, nmat, nrb, "fp32", args.steps, args.warmup, args.tol, args.max_refine,
+                                   graph=bool(args.graph))
+        print(f"# batched: {out['batched']['seconds']:.4f} s/step, {out['batched']['matrices_per_s']:.0f} matrices/s, "
+              f"bwd {out['batched']['backward_error_host_check']:.2e}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A, S, Rs, B, refine=1)
     print(json.dumps(out))

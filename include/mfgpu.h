@@ -126,6 +126,9 @@ int mf_factor_device(mf_plan *P, const double *d_Ax, const double *d_Rs, int32_t
  * one matrix per call, src/multifrontal.jl:146). */
 int mf_refactor_batch(mf_plan *P, int64_t nmat, const double *Ax_batch, const double *Rs_batch,
                       int64_t *info_col);
+/* Same with the values and scalings in device memory. */
+int mf_refactor_batch_device(mf_plan *P, int64_t nmat, const double *d_Ax_batch, const double *d_Rs_batch,
+                             int64_t *info_col);
 
 /* Solve A X = B for nrhs right-hand sides (B: n x nrhs, leading dimension ldb; X: ldx).
  * refine_steps >= 0 fixed refinement steps as solve(F, b; refine=k)
@@ -139,6 +142,9 @@ int mf_solve_device(mf_plan *P, int64_t nrhs, const double *d_B, int64_t ldb, do
 /* Batched solve after mf_refactor_batch: matrix m solves B[m] (n x nrhs, contiguous). */
 int mf_solve_batch(mf_plan *P, int64_t nmat, int64_t nrhs, const double *B, double *X,
                    int32_t refine_steps, double bwd_tol, mf_solve_report *rep);
+/* Same with B and X in device memory. */
+int mf_solve_batch_device(mf_plan *P, int64_t nmat, int64_t nrhs, const double *d_B, double *d_X,
+                          int32_t refine_steps, double bwd_tol, mf_solve_report *rep);
 
 /* nnz(L) and nnz(U) of the factor (each = predicted_fill/2, src/symbolic.jl:230-237). */
 int mf_nnz(const mf_plan *P, int64_t *nnzL, int64_t *nnzU);
@@ -149,6 +155,10 @@ int mf_nnz(const mf_plan *P, int64_t *nnzL, int64_t *nnzU);
 int mf_export_factors(mf_plan *P, int64_t *Lcolptr, int64_t *Lrowval, double *Lnzval,
                       int64_t *Ucolptr, int64_t *Urowval, double *Unzval, int64_t *p,
                       int64_t *q);
+/* mf_export_factors for matrix mat (0-based) of the last mf_refactor_batch.  No reference
+ * counterpart (batches are an extension of src/multifrontal.jl:146). */
+int mf_export_factors_batch(mf_plan *P, int64_t mat, int64_t *Lcolptr, int64_t *Lrowval, double *Lnzval,
+                            int64_t *Ucolptr, int64_t *Urowval, double *Unzval, int64_t *p, int64_t *q);
 
 /* Statistics of the plan: factor flops, Schur-gemm flops, flops in fronts with nf >= 1024,
  * front storage bytes, number of levels, kernel launches per factorization. */

@@ -1157,15 +1157,24 @@ extern "C" int mf_factor(mf_plan *P, const double *Ax, const double *Rs, int32_t
     return factor_common(P, P->d_ax.as<double>(), Rs ? P->d_rs.as<double>() : nullptr, 1, check, info_col);
 }
 
-extern "C" int mf_refactor_batch(mf_plan *P, int64_t nmat, const double *Ax_batch, const double *Rs_batch,
-                                 int64_t *info_col)
+static int refactor_batch(mf_plan *P, int64_t nmat, const double *Ax_batch, const double *Rs_batch,
+                          int64_t *info_col, hipMemcpyKind kind)
 {
     if (!P || !Ax_batch) return fail(MF_BADARG, "NULL argument");
     if (nmat < 1 || nmat > P->nbatch) return fail(MF_BADARG, "nmat exceeds the plan's nbatch");
-    HIPCHK(hipMemcpyAsync(P->d_ax.p, Ax_batch, (size_t)P->nnz * 8 * nmat, hipMemcpyHostToDevice, P->stream));
-    if (Rs_batch)
-        HIPCHK(hipMemcpyAsync(P->d_rs.p, Rs_batch, (size_t)P->n * 8 * nmat, hipMemcpyHostToDevice, P->stream));
+    HIPCHK(hipMemcpyAsync(P->d_ax.p, Ax_batch, (size_t)P->nnz * 8 * nmat, kind, P->stream));
+    if (Rs_batch) HIPCHK(hipMemcpyAsync(P->d_rs.p, Rs_batch, (size_t)P->n * 8 * nmat, kind, P->stream));
     return factor_common(P, P->d_ax.as<double>(), Rs_batch ? P->d_rs.as<double>() : nullptr, (int)nmat, 0, info_col);
+}
+extern "C" int mf_refactor_batch(mf_plan *P, int64_t nmat, const double *Ax_batch, const double *Rs_batch,
+                                 int64_t *info_col)
+{
+    return refactor_batch(P, nmat, Ax_batch, Rs_batch, info_col, hipMemcpyHostToDevice);
+}
+extern "C" int mf_refactor_batch_device(mf_plan *P, int64_t nmat, const double *d_Ax_batch, const double *d_Rs_batch,
+                                        int64_t *info_col)
+{
+    return refactor_batch(P, nmat, d_Ax_batch, d_Rs_batch, info_col, hipMemcpyDeviceToDevice);
 }
 
 // ------------------------------------------------------------------ solve
@@ -1393,8 +1402,8 @@ extern "C" int mf_solve(mf_plan *P, int64_t nrhs, const double *B, int64_t ldb, 
     return MF_OK;
 }
 
-extern "C" int mf_solve_batch(mf_plan *P, int64_t nmat, int64_t nrhs, const double *B, double *X,
-                              int32_t refine_steps, double bwd_tol, mf_solve_report *rep)
+static int solve_batch(mf_plan *P, int64_t nmat, int64_t nrhs, const double *B, double *X, int32_t refine_steps,
+                       double bwd_tol, mf_solve_report *rep, hipMemcpyKind in_kind, hipMemcpyKind out_kind)
 {
     if (!P || !B || !X) return fail(MF_BADARG, "NULL argument");
     if (!P->factored || nmat != P->nmat_factored) return fail(MF_BADARG, "solve_batch needs a matching refactor_batch");
@@ -1402,26 +1411,39 @@ extern "C" int mf_solve_batch(mf_plan *P, int64_t nmat, int64_t nrhs, const doub
     int rc = ensure_solve_buffers(P, nrhs, (int)nmat);
     if (rc != MF_OK) return rc;
     size_t bytes = (size_t)P->n * nrhs * nmat * 8;
-    HIPCHK(hipMemcpyAsync(P->d_B.p, B, bytes, hipMemcpyHostToDevice, P->stream));
+    HIPCHK(hipMemcpyAsync(P->d_B.p, B, bytes, in_kind, P->stream));
     if ((rc = solve_core(P, nrhs, (int)nmat, refine_steps, bwd_tol, rep)) != MF_OK) return rc;
-    HIPCHK(hipMemcpyAsync(X, P->d_X.p, bytes, hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipMemcpyAsync(X, P->d_X.p, bytes, out_kind, P->stream));
     HIPCHK(hipStreamSynchronize(P->stream));
     return MF_OK;
 }
+extern "C" int mf_solve_batch(mf_plan *P, int64_t nmat, int64_t nrhs, const double *B, double *X,
+                              int32_t refine_steps, double bwd_tol, mf_solve_report *rep)
+{
+    return solve_batch(P, nmat, nrhs, B, X, refine_steps, bwd_tol, rep, hipMemcpyHostToDevice, hipMemcpyDeviceToHost);
+}
+extern "C" int mf_solve_batch_device(mf_plan *P, int64_t nmat, int64_t nrhs, const double *d_B, double *d_X,
+                                     int32_t refine_steps, double bwd_tol, mf_solve_report *rep)
+{
+    return solve_batch(P, nmat, nrhs, d_B, d_X, refine_steps, bwd_tol, rep, hipMemcpyDeviceToDevice,
+                       hipMemcpyDeviceToDevice);
+}
 
 // ------------------------------------------------------------------ export
-extern "C" int mf_export_factors(mf_plan *P, int64_t *Lcolptr, int64_t *Lrowval, double *Lnzval, int64_t *Ucolptr,
-                                 int64_t *Urowval, double *Unzval, int64_t *p, int64_t *q)
+static int export_factors(mf_plan *P, int64_t mat, int64_t *Lcolptr, int64_t *Lrowval, double *Lnzval,
+                          int64_t *Ucolptr, int64_t *Urowval, double *Unzval, int64_t *p, int64_t *q)
 {
     if (!P) return fail(MF_BADARG, "NULL plan");
     if (!P->factored) return fail(MF_BADARG, "plan has no factorization");
+    if (mat < 0 || mat >= P->nmat_factored) return fail(MF_BADARG, "matrix index outside the factored batch");
     const int64_t n = P->n;
+    const int32_t *d_prow = P->d_prow.as<int32_t>() + mat * n;
     if (Lcolptr)
         for (int64_t j = 0; j <= n; j++) Lcolptr[j] = P->Lp[j] + 1;
     if (Ucolptr)
         for (int64_t j = 0; j <= n; j++) Ucolptr[j] = P->Up[j] + 1;
     std::vector<int32_t> prow(n);
-    HIPCHK(hipMemcpy(prow.data(), P->d_prow.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(prow.data(), d_prow, (size_t)n * 4, hipMemcpyDeviceToHost));
     if (p)
         for (int64_t k = 0; k < n; k++) p[k] = P->qf[prow[k]] + 1;
     if (q)
@@ -1434,14 +1456,16 @@ extern "C" int mf_export_factors(mf_plan *P, int64_t *Lcolptr, int64_t *Lrowval,
         HIPCHK(P->d_Lx.alloc((size_t)P->nnzL * 8));
         HIPCHK(P->d_Ui.alloc((size_t)P->nnzU * 8));
         HIPCHK(P->d_Ux.alloc((size_t)P->nnzU * 8));
-        HIPCHK(launch_rowfac(P->d_prow.as<int32_t>(), P->d_rowfac.as<int32_t>(), n, P->stream));
+        HIPCHK(launch_rowfac(d_prow, P->d_rowfac.as<int32_t>(), n, P->stream));
         if (P->prec == MF_FP64)
-            HIPCHK(launch_export<double>(P->d_sn.as<SnDev>(), P->nsuper, P->d_pool.as<double>(), P->d_upd.as<int32_t>(),
+            HIPCHK(launch_export<double>(P->d_sn.as<SnDev>(), P->nsuper, P->d_pool.as<double>() + mat * P->pool_elems,
+                                         P->d_upd.as<int32_t>(),
                                          P->d_rowfac.as<int32_t>(), P->d_Lp.as<int64_t>(), P->d_Up.as<int64_t>(),
                                          P->d_uoff.as<int64_t>(), P->d_Li.as<int64_t>(), P->d_Lx.as<double>(),
                                          P->d_Ui.as<int64_t>(), P->d_Ux.as<double>(), P->stream));
         else
-            HIPCHK(launch_export<float>(P->d_sn.as<SnDev>(), P->nsuper, P->d_pool.as<float>(), P->d_upd.as<int32_t>(),
+            HIPCHK(launch_export<float>(P->d_sn.as<SnDev>(), P->nsuper, P->d_pool.as<float>() + mat * P->pool_elems,
+                                        P->d_upd.as<int32_t>(),
                                         P->d_rowfac.as<int32_t>(), P->d_Lp.as<int64_t>(), P->d_Up.as<int64_t>(),
                                         P->d_uoff.as<int64_t>(), P->d_Li.as<int64_t>(), P->d_Lx.as<double>(),
                                         P->d_Ui.as<int64_t>(), P->d_Ux.as<double>(), P->stream));
@@ -1452,6 +1476,18 @@ extern "C" int mf_export_factors(mf_plan *P, int64_t *Lcolptr, int64_t *Lrowval,
         if (Unzval) HIPCHK(hipMemcpy(Unzval, P->d_Ux.p, (size_t)P->nnzU * 8, hipMemcpyDeviceToHost));
     }
     return MF_OK;
+}
+
+extern "C" int mf_export_factors(mf_plan *P, int64_t *Lcolptr, int64_t *Lrowval, double *Lnzval, int64_t *Ucolptr,
+                                 int64_t *Urowval, double *Unzval, int64_t *p, int64_t *q)
+{
+    return export_factors(P, 0, Lcolptr, Lrowval, Lnzval, Ucolptr, Urowval, Unzval, p, q);
+}
+
+extern "C" int mf_export_factors_batch(mf_plan *P, int64_t mat, int64_t *Lcolptr, int64_t *Lrowval, double *Lnzval,
+                                       int64_t *Ucolptr, int64_t *Urowval, double *Unzval, int64_t *p, int64_t *q)
+{
+    return export_factors(P, mat, Lcolptr, Lrowval, Lnzval, Ucolptr, Urowval, Unzval, p, q);
 }
 
 extern "C" int mf_plan_set_profiling(mf_plan *P, int32_t on)

@@ -55,8 +55,15 @@ SIGNATURES = {
                                        ctypes.POINTER(mf_solve_report)]),
     "mf_solve_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, c_f64p, c_f64p,
                                       ctypes.c_int32, ctypes.c_double, ctypes.POINTER(mf_solve_report)]),
+    "mf_solve_batch_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_int32, ctypes.c_double,
+                                             ctypes.POINTER(mf_solve_report)]),
+    "mf_refactor_batch_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                                c_i64p]),
     "mf_nnz": (ctypes.c_int, [ctypes.c_void_p, c_i64p, c_i64p]),
     "mf_export_factors": (ctypes.c_int, [ctypes.c_void_p, c_i64p, c_i64p, c_f64p, c_i64p, c_i64p, c_f64p, c_i64p,
+                                         c_i64p]),
+    "mf_export_factors_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, c_i64p, c_i64p, c_f64p, c_i64p, c_i64p, c_f64p, c_i64p,
                                          c_i64p]),
     "mf_plan_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(mf_plan_stats)]),
     "mf_plan_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),

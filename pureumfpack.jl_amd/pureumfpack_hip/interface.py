@@ -222,7 +222,41 @@ class _Plan:
         self.last_report = {f: getattr(rep, f) for f, _ in rep._fields_}
         return X[:, 0].copy() if vec else X
 
-    def export(self):
+    def factor_batch(self, Ax, Rs, check=True):
+        """mf_refactor_batch: Ax is nmat x nnz (values in the plan pattern's CSC order), Rs is
+        nmat x n or None.  Returns the per-matrix singular columns (0 = nonsingular)."""
+        Ax = np.ascontiguousarray(Ax, dtype=np.float64)
+        nmat = Ax.shape[0]
+        if Ax.shape[1] != self.nnz:
+            raise DimensionMismatch("batch values do not match the pattern")
+        rs = None if Rs is None else np.ascontiguousarray(Rs, dtype=np.float64)
+        info = np.zeros(nmat, np.int64)
+        rc = self.lib.mf_refactor_batch(self.h, nmat, _lib.ptr(Ax, c_f64p), _lib.ptr(rs, c_f64p), _lib.ptr(info))
+        if rc == MF_SINGULAR and check:
+            raise SingularException(int(info[np.nonzero(info)[0][0]]))
+        if rc != MF_SINGULAR:
+            _check(rc, "mf_refactor_batch")
+        self.nmat = nmat
+        return info
+
+    def solve_batch(self, B, refine=0, tol=0.0):
+        """mf_solve_batch: B is nmat x n x nrhs (or nmat x n); returns X of the same shape."""
+        B = np.asarray(B, dtype=np.float64)
+        vec = B.ndim == 2
+        B3 = B[:, :, None] if vec else B
+        nmat, n, nrhs = B3.shape
+        if n != self.n:
+            raise DimensionMismatch("right-hand sides have the wrong length")
+        Bt = np.ascontiguousarray(np.transpose(B3, (0, 2, 1)))   # per matrix: n x nrhs, column-major
+        Xt = np.zeros_like(Bt)
+        rep = _lib.mf_solve_report()
+        _check(self.lib.mf_solve_batch(self.h, nmat, nrhs, _lib.ptr(Bt, c_f64p), _lib.ptr(Xt, c_f64p), int(refine),
+                                       float(tol), ctypes.byref(rep)), "mf_solve_batch")
+        self.last_report = {f: getattr(rep, f) for f, _ in rep._fields_}
+        X = np.transpose(Xt, (0, 2, 1))
+        return X[:, :, 0].copy() if vec else np.ascontiguousarray(X)
+
+    def export(self, mat=None):
         lib = self.lib
         nL, nU = ctypes.c_int64(), ctypes.c_int64()
         _check(lib.mf_nnz(self.h, ctypes.byref(nL), ctypes.byref(nU)))
@@ -230,8 +264,12 @@ class _Plan:
         Lp = np.zeros(n + 1, np.int64); Li = np.zeros(nL.value, np.int64); Lx = np.zeros(nL.value)
         Up = np.zeros(n + 1, np.int64); Ui = np.zeros(nU.value, np.int64); Ux = np.zeros(nU.value)
         p = np.zeros(n, np.int64); q = np.zeros(n, np.int64)
-        _check(lib.mf_export_factors(self.h, _lib.ptr(Lp), _lib.ptr(Li), _lib.ptr(Lx, c_f64p), _lib.ptr(Up),
-                                     _lib.ptr(Ui), _lib.ptr(Ux, c_f64p), _lib.ptr(p), _lib.ptr(q)), "export")
+        args = (_lib.ptr(Lp), _lib.ptr(Li), _lib.ptr(Lx, c_f64p), _lib.ptr(Up), _lib.ptr(Ui), _lib.ptr(Ux, c_f64p),
+                _lib.ptr(p), _lib.ptr(q))
+        if mat is None:
+            _check(lib.mf_export_factors(self.h, *args), "export")
+        else:
+            _check(lib.mf_export_factors_batch(self.h, int(mat), *args), "export")
         L = sp.csc_matrix((Lx, Li - 1, Lp - 1), shape=(n, n))
         U = sp.csc_matrix((Ux, Ui - 1, Up - 1), shape=(n, n))
         return L, U, p - 1, q - 1
@@ -311,6 +349,59 @@ def splu(A, method="gplu", ordering="amd", tol=0.1, scale=SCALE_SUM, check=True,
     if method == "gplu":
         raise NotImplementedError("method='gplu' is not part of the GPU numeric phase")
     raise ArgumentError(f"unknown method {method}")
+
+
+class BatchedLU:
+    """Many matrices sharing one sparsity pattern: one symbolic analysis and one device plan;
+    every launch of the factor and solve schedules covers the whole batch (grid dimension y
+    = matrix), optionally replayed from a captured hipGraph.  The per-matrix semantics are
+    those of splu(A_m; method=:multifrontal) and solve(F_m, b; refine) (src/interface.jl:93-177).
+
+    values: nmat x nnz in the CSC order of `pattern` (the pattern's own data are ignored)."""
+
+    def __init__(self, pattern, nbatch, precision="fp32", scale=SCALE_SUM, symbolic=None, use_graph=True,
+                 pivoting="auto", pivot_tol=0.0):
+        A = _csc(pattern)
+        if A.shape[0] != A.shape[1]:
+            raise DimensionMismatch("BatchedLU requires square matrices")
+        self.pattern = A
+        self.scale = scale
+        self.S = symbolic if symbolic is not None else symbolic_mf(A)
+        self._plan = _Plan(A, self.S, precision=precision, use_graph=use_graph, nbatch=nbatch,
+                           pivoting=pivoting, pivot_tol=pivot_tol)
+        self.nbatch = int(nbatch)
+        self.Rs = None
+
+    def matrix(self, values_m):
+        """Matrix m as a scipy CSC matrix with the shared pattern."""
+        A = self.pattern
+        return sp.csc_matrix((np.asarray(values_m, dtype=np.float64), A.indices, A.indptr), shape=A.shape)
+
+    def refactor(self, values, check=True):
+        """Factor every matrix of the batch (row scaling per matrix as splu computes it)."""
+        values = np.ascontiguousarray(values, dtype=np.float64)
+        if values.ndim != 2 or values.shape[1] != self.pattern.nnz:
+            raise DimensionMismatch("values must be nmat x nnz(pattern)")
+        if values.shape[0] > self.nbatch:
+            raise ArgumentError("more matrices than the plan's nbatch")
+        if self.scale == SCALE_NONE:
+            self.Rs = None
+        else:
+            self.Rs = np.stack([row_scaling(self.matrix(v), self.scale) for v in values])
+        self.values = values
+        return self._plan.factor_batch(values, self.Rs, check=check)
+
+    def solve(self, B, refine=0, tol=0.0):
+        """X[m] = A_m^-1 B[m] for B of shape nmat x n (x nrhs)."""
+        return self._plan.solve_batch(B, refine=refine, tol=tol)
+
+    def factors(self, m):
+        """(L, U, p, q) of matrix m: (Rs_m .* A_m)[p, q] = L U."""
+        return self._plan.export(mat=m)
+
+    @property
+    def last_report(self):
+        return getattr(self._plan, "last_report", None)
 
 
 def solve(F: MFFactorization, b, refine: int = 0, tol: float = 0.0):

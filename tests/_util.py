@@ -32,3 +32,34 @@ def max_rel(a, b):
 
 def same_pattern(X, Y):
     return (np.array_equal(X.indptr, Y.indptr) and np.array_equal(X.indices, Y.indices))
+
+
+class DeviceArrays:
+    """Device copies of numpy arrays through the HIP runtime (for the *_device entry points)."""
+
+    def __init__(self):
+        import ctypes
+        self.ct = ctypes
+        self.h = ctypes.CDLL("libamdhip64.so.7")
+        self.h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.h.hipFree.argtypes = [ctypes.c_void_p]
+        self.ptrs = []
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        p = self.ct.c_void_p()
+        assert self.h.hipMalloc(self.ct.byref(p), max(a.nbytes, 16)) == 0
+        assert self.h.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0
+        self.ptrs.append(p)
+        return p
+
+    def download(self, p, like):
+        out = np.empty_like(np.ascontiguousarray(like))
+        assert self.h.hipMemcpy(out.ctypes.data, p, out.nbytes, 2) == 0
+        return out
+
+    def free(self):
+        for p in self.ptrs:
+            self.h.hipFree(p)
+        self.ptrs = []
