@@ -189,6 +189,35 @@ int mf_plan_get_profile(const mf_plan *P, double *ms, int64_t *launches, double 
 const char *mf_last_error(void);
 void mf_plan_destroy(mf_plan *P);
 
+/* ---- gemm tile policy and the autotuner's measurement (no reference counterpart: tile
+ * shapes are a property of the GPU kernels; every shape gives bit-identical results) ---- */
+/* policy[use * nbuckets + bucket] = tile index for use 0 (panel updates) and 1 (Schur
+ * complements), bucket = size class of sqrt(m n); applies to plans created afterwards. */
+int mf_set_gemm_tiles(int32_t precision, const int32_t *policy);
+int mf_get_gemm_tiles(int32_t precision, int32_t *policy);
+/* tile shapes (rows x columns per workgroup) and bucket upper bounds (nbuckets - 1 values). */
+int mf_gemm_tile_info(int32_t *ntiles, int32_t *tile_m, int32_t *tile_n, int32_t *nbuckets, int32_t *bucket_hi);
+/* average seconds of one launch of nprob independent C(m x n) -= A(m x k) B(k x n) problems
+ * with tile shape `tile` (reps launches timed with HIP events after one warm-up). */
+int mf_time_gemm(int32_t precision, int64_t m, int64_t n, int64_t k, int64_t nprob, int32_t tile, int32_t reps,
+                 double *seconds);
+
+/* ---- small MLP cost model, trained on the GPU (no reference counterpart) ----
+ * d_in -> hidden (ReLU) -> hidden (ReLU) -> 1, FP32; samples row-major (n x d_in); loss =
+ * mean squared error; parameters: per layer W (d_out x d_in, row-major) then b. */
+typedef struct mf_mlp mf_mlp;
+int mf_mlp_create(int32_t d_in, int32_t hidden, uint64_t seed, mf_mlp **out);
+void mf_mlp_destroy(mf_mlp *m);
+int64_t mf_mlp_num_params(const mf_mlp *m);
+int mf_mlp_get_params(mf_mlp *m, float *params);
+int mf_mlp_set_params(mf_mlp *m, const float *params);
+int mf_mlp_forward(mf_mlp *m, const float *X, int64_t n, float *Y);
+/* loss and its gradient with respect to every parameter (same layout as the parameters) */
+int mf_mlp_gradient(mf_mlp *m, const float *X, const float *t, int64_t n, float *loss, float *grad);
+/* full-batch Adam (lr, beta1 0.9, beta2 0.999, eps 1e-8) for `epochs` steps */
+int mf_mlp_train(mf_mlp *m, const float *X, const float *t, int64_t n, int32_t epochs, float lr, float *final_loss);
+const char *mf_mlp_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

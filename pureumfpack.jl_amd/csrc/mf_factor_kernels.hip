@@ -689,15 +689,17 @@ template <> struct Mfma<float> {
     static __device__ __forceinline__ int row(int lane, int q) { return 4 * (lane >> 4) + q; }
 };
 
-template <typename T>
+template <typename T, int TM, int TN>
 __global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs,
                                               const GemmTile *__restrict__ tiles, void *pool_v,
                                               int64_t pool_stride)
 {
+    // 4 waves in a 2 x 2 arrangement, each owning a (TM/2) x (TN/2) block of C as FM x FN
+    // MFMA 16 x 16 accumulators; A and B slices of BK = 16 staged in LDS per step.
     typedef typename Mfma<T>::acc_t acc_t;
-    constexpr int BM = GEMM_BM, BN = GEMM_BN, BK = GEMM_BK;
-    __shared__ T As[BK][BM + 1];
-    __shared__ T Bs[BK][BN + 1];
+    constexpr int BK = GEMM_BK, WM = TM / 2, WN = TN / 2, FM = WM / 16, FN = WN / 16;
+    __shared__ T As[BK][TM + 1];
+    __shared__ T Bs[BK][TN + 1];
     const GemmTile tl = tiles[blockIdx.x];
     const GemmProb pr = probs[tl.prob];
     T *pool = reinterpret_cast<T *>(pool_v) + blockIdx.y * pool_stride;
@@ -705,48 +707,49 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs
     const T *B = pool + pr.boff;
     T *C = pool + pr.coff;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int wm = (wid & 1) * 32, wn = (wid >> 1) * 32;
+    const int wm = (wid & 1) * WM, wn = (wid >> 1) * WN;
     const int m0 = tl.tm, n0 = tl.tn;
-    acc_t acc[2][2];
+    acc_t acc[FM][FN];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < FM; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = acc_t{0, 0, 0, 0};
+        for (int j = 0; j < FN; j++) acc[i][j] = acc_t{0, 0, 0, 0};
 
     for (int k0 = 0; k0 < pr.k; k0 += BK) {
-        // A tile: BM x BK, thread -> row tid%64, k (tid/64) + 4i
+        // A slice TM x BK: consecutive threads walk a column (coalesced)
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            int r = tid & 63, kk = (tid >> 6) + 4 * i;
-            int gr = m0 + r, gk = k0 + kk;
+        for (int i = 0; i < TM * BK / 256; i++) {
+            const int e = tid + 256 * i, r = e % TM, kk = e / TM;
+            const int gr = m0 + r, gk = k0 + kk;
             As[kk][r] = (gr < pr.m && gk < pr.k) ? A[gr + (int64_t)gk * pr.lda] : T(0);
         }
+        // B slice BK x TN: consecutive threads walk k within a column
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            int kk = tid & 15, cc = (tid >> 4) + 16 * i;
-            int gc = n0 + cc, gk = k0 + kk;
+        for (int i = 0; i < TN * BK / 256; i++) {
+            const int e = tid + 256 * i, kk = e % BK, cc = e / BK;
+            const int gc = n0 + cc, gk = k0 + kk;
             Bs[kk][cc] = (gc < pr.n && gk < pr.k) ? B[gk + (int64_t)gc * pr.ldb] : T(0);
         }
         __syncthreads();
 #pragma unroll
         for (int kq = 0; kq < BK; kq += 4) {
-            T av[2], bv[2];
+            T av[FM], bv[FN];
 #pragma unroll
-            for (int i = 0; i < 2; i++) av[i] = As[kq + (lane >> 4)][wm + 16 * i + (lane & 15)];
+            for (int i = 0; i < FM; i++) av[i] = As[kq + (lane >> 4)][wm + 16 * i + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < 2; j++) bv[j] = Bs[kq + (lane >> 4)][wn + 16 * j + (lane & 15)];
+            for (int j = 0; j < FN; j++) bv[j] = Bs[kq + (lane >> 4)][wn + 16 * j + (lane & 15)];
 #pragma unroll
-            for (int i = 0; i < 2; i++)
+            for (int i = 0; i < FM; i++)
 #pragma unroll
-                for (int j = 0; j < 2; j++) acc[i][j] = Mfma<T>::mma(av[i], bv[j], acc[i][j]);
+                for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::mma(av[i], bv[j], acc[i][j]);
         }
         __syncthreads();
     }
     const bool add = pr.flags & GEMM_ADD, set = pr.flags & GEMM_SET;
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < FM; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
+        for (int j = 0; j < FN; j++) {
             int gc = n0 + wn + 16 * j + (lane & 15);
 #pragma unroll
             for (int q = 0; q < 4; q++) {
@@ -759,6 +762,25 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs
             }
         }
 }
+
+template <typename T>
+hipError_t launch_gemm_tile(int tile, const GemmProb *probs, const GemmTile *tiles, int count, void *pool,
+                            int64_t pool_stride, int nmat, hipStream_t st)
+{
+    const dim3 grid(count, nmat), blk(256);
+    switch (tile) {
+    case 0: hipLaunchKernelGGL((k_gemm<T, 64, 64>), grid, blk, 0, st, probs, tiles, pool, pool_stride); break;
+    case 1: hipLaunchKernelGGL((k_gemm<T, 128, 64>), grid, blk, 0, st, probs, tiles, pool, pool_stride); break;
+    case 2: hipLaunchKernelGGL((k_gemm<T, 64, 128>), grid, blk, 0, st, probs, tiles, pool, pool_stride); break;
+    case 3: hipLaunchKernelGGL((k_gemm<T, 128, 128>), grid, blk, 0, st, probs, tiles, pool, pool_stride); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+template hipError_t launch_gemm_tile<double>(int, const GemmProb *, const GemmTile *, int, void *, int64_t, int,
+                                             hipStream_t);
+template hipError_t launch_gemm_tile<float>(int, const GemmProb *, const GemmTile *, int, void *, int64_t, int,
+                                            hipStream_t);
 
 // ------------------------------------------------------------------------------------------
 // Host launchers
@@ -802,9 +824,8 @@ hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int
                            reinterpret_cast<const T *>(a.spec));
         break;
     case L_GEMM:
-        hipLaunchKernelGGL(k_gemm<T>, grid, dim3(256), 0, st, probs + L.prob_off,
-                           tiles + L.list_off, base.pool, base.pool_stride);
-        break;
+        return launch_gemm_tile<T>(L.tile, probs + L.prob_off, tiles + L.list_off, L.count, base.pool,
+                                   base.pool_stride, nmat, st);
     default:
         return hipErrorInvalidValue;
     }

@@ -71,6 +71,7 @@ struct Launch {
     int32_t prob_off;   // L_GEMM: offset into problems; tiles at list_off in tile array
     int32_t shmem;      // dynamic LDS bytes
     int32_t level;
+    int32_t tile;       // L_GEMM: tile shape (GEMM_TILE_* index)
     float tau;          // speculative schedule: multiplier bound 1/tau
     double flops;       // algorithmic flops of the launch (host-side bookkeeping)
     double bytes;       // algorithmic HBM bytes of the launch

@@ -25,6 +25,23 @@ constexpr int INV_LEAF = 64;   // diagonal block of the recursive pivot-block in
 constexpr int INV_NP = 128;    // fronts with np > INV_NP keep inv(L11), inv(U11) for the solves
 constexpr int PINFO_STRIDE = 1 + 4 * PANEL_KB;  // per-front panel permutation record
 constexpr int GEMM_BM = 64, GEMM_BN = 64, GEMM_BK = 16;
+// Tile shapes of the grouped MFMA gemm (rows x columns of C per 256-thread workgroup).  All
+// shapes accumulate every C element over k in the same order (BK = 16 slices, MFMA k = 4),
+// so the choice of tile never changes a result bit.
+constexpr int GEMM_NTILES = 4;
+constexpr int GEMM_TILE_M[GEMM_NTILES] = {64, 128, 64, 128};
+constexpr int GEMM_TILE_N[GEMM_NTILES] = {64, 64, 128, 128};
+// tile policy: size buckets of sqrt(m n) (upper bounds, the last open-ended) for the two gemm
+// uses (0: panel updates, 1: Schur complements)
+constexpr int GEMM_NBUCKETS = 6;
+constexpr int GEMM_BUCKET_HI[GEMM_NBUCKETS - 1] = {96, 192, 384, 768, 1536};
+inline int gemm_bucket(int64_t m, int64_t n)
+{
+    const double s = __builtin_sqrt((double)m * (double)n);
+    int b = 0;
+    while (b < GEMM_NBUCKETS - 1 && s >= GEMM_BUCKET_HI[b]) b++;
+    return b;
+}
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -94,6 +111,9 @@ struct SolveArgs {
 };
 
 // ---- launchers (mf_factor_kernels.hip, mf_solve_kernels.hip) ----
+template <typename T>
+hipError_t launch_gemm_tile(int tile, const GemmProb *probs, const GemmTile *tiles, int count, void *pool,
+                            int64_t pool_stride, int nmat, hipStream_t st);
 template <typename T>
 hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int32_t *lists,
                               const GemmProb *probs, const GemmTile *tiles, int nmat, hipStream_t st);

@@ -27,6 +27,9 @@ static constexpr int64_t SOLVE_CK_HOST = 256;  // column chunk of the split-K so
 static constexpr int64_t FRONT_MAXNF_HOST = 1024;  // largest front of the fused per-front sweeps
 
 static thread_local std::string g_err;
+// gemm tile policy per precision (0: FP64, 1: FP32/BF16), use (panel update / Schur) and size
+// bucket; read when a plan is created.  Default: the 64 x 64 tile everywhere.
+static int32_t g_tile_policy[2][2 * GEMM_NBUCKETS] = {};
 
 static int fail(int code, const std::string &msg)
 {
@@ -499,26 +502,37 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
     std::vector<GemmTile> tiles;
     const double ebytes = (double)P.esize();
     std::vector<Launch> *gemm_dst = nullptr;
+    // gemm launches: the problems of one call are grouped by the tile shape the policy picks
+    // for their size bucket (one launch per shape used)
+    const int32_t *policy = g_tile_policy[P.prec == MF_FP64 ? 0 : 1];
     auto add_gemm = [&](std::vector<GemmProb> &pv, int lvl, int schur) {
         if (pv.empty()) return;
-        Launch L{};
-        L.kind = L_GEMM;
-        L.variant = schur;
-        for (const GemmProb &g : pv) {
-            L.flops += 2.0 * g.m * (double)g.n * g.k;
-            L.bytes += ((double)g.m * g.k + (double)g.k * g.n + 2.0 * g.m * (double)g.n) * ebytes;
+        const int use = schur == 1 ? 1 : 0;
+        for (int tile = 0; tile < GEMM_NTILES; tile++) {
+            std::vector<const GemmProb *> sel;
+            for (const GemmProb &g : pv)
+                if (policy[use * GEMM_NBUCKETS + gemm_bucket(g.m, g.n)] == tile) sel.push_back(&g);
+            if (sel.empty()) continue;
+            Launch L{};
+            L.kind = L_GEMM;
+            L.variant = schur;
+            L.tile = tile;
+            for (const GemmProb *g : sel) {
+                L.flops += 2.0 * g->m * (double)g->n * g->k;
+                L.bytes += ((double)g->m * g->k + (double)g->k * g->n + 2.0 * g->m * (double)g->n) * ebytes;
+            }
+            L.prob_off = (int32_t)probs.size();
+            L.list_off = (int32_t)tiles.size();
+            for (size_t q = 0; q < sel.size(); q++) {
+                const GemmProb &g = *sel[q];
+                for (int tn = 0; tn < g.n; tn += GEMM_TILE_N[tile])
+                    for (int tm = 0; tm < g.m; tm += GEMM_TILE_M[tile]) tiles.push_back({(int32_t)q, tm, tn, 0});
+                probs.push_back(g);
+            }
+            L.count = (int32_t)(tiles.size() - L.list_off);
+            L.level = lvl;
+            if (L.count > 0) gemm_dst->push_back(L);
         }
-        L.prob_off = (int32_t)probs.size();
-        L.list_off = (int32_t)tiles.size();
-        for (size_t q = 0; q < pv.size(); q++) {
-            const GemmProb &g = pv[q];
-            for (int tn = 0; tn < g.n; tn += GEMM_BN)
-                for (int tm = 0; tm < g.m; tm += GEMM_BM) tiles.push_back({(int32_t)q, tm, tn, 0});
-            probs.push_back(g);
-        }
-        L.count = (int32_t)(tiles.size() - L.list_off);
-        L.level = lvl;
-        if (L.count > 0) gemm_dst->push_back(L);
     };
     const int elem = (int)P.esize();
     // algorithmic bytes of assembling front s: the front written once, every child
@@ -1512,5 +1526,91 @@ extern "C" int mf_plan_get_profile(const mf_plan *P, double *ms, int64_t *launch
         if (flops) flops[c] = P->prof_flops[c];
         if (bytes) bytes[c] = P->prof_bytes[c];
     }
+    return MF_OK;
+}
+
+// ------------------------------------------------------------------ tile policy and timing
+extern "C" int mf_set_gemm_tiles(int32_t precision, const int32_t *policy)
+{
+    if (!policy) return fail(MF_BADARG, "NULL policy");
+    if (precision != MF_FP64 && precision != MF_FP32 && precision != MF_BF16) return fail(MF_BADARG, "bad precision");
+    for (int i = 0; i < 2 * GEMM_NBUCKETS; i++)
+        if (policy[i] < 0 || policy[i] >= GEMM_NTILES) return fail(MF_BADARG, "tile index out of range");
+    std::memcpy(g_tile_policy[precision == MF_FP64 ? 0 : 1], policy, sizeof(int32_t) * 2 * GEMM_NBUCKETS);
+    return MF_OK;
+}
+
+extern "C" int mf_get_gemm_tiles(int32_t precision, int32_t *policy)
+{
+    if (!policy) return fail(MF_BADARG, "NULL policy");
+    std::memcpy(policy, g_tile_policy[precision == MF_FP64 ? 0 : 1], sizeof(int32_t) * 2 * GEMM_NBUCKETS);
+    return MF_OK;
+}
+
+extern "C" int mf_gemm_tile_info(int32_t *ntiles, int32_t *tile_m, int32_t *tile_n, int32_t *nbuckets,
+                                 int32_t *bucket_hi)
+{
+    if (ntiles) *ntiles = GEMM_NTILES;
+    if (nbuckets) *nbuckets = GEMM_NBUCKETS;
+    for (int t = 0; t < GEMM_NTILES; t++) {
+        if (tile_m) tile_m[t] = GEMM_TILE_M[t];
+        if (tile_n) tile_n[t] = GEMM_TILE_N[t];
+    }
+    if (bucket_hi)
+        for (int b = 0; b + 1 < GEMM_NBUCKETS; b++) bucket_hi[b] = GEMM_BUCKET_HI[b];
+    return MF_OK;
+}
+
+// Times nprob independent C(m x n) -= A(m x k) B(k x n) problems in one launch of tile shape
+// `tile` (the autotuner's measurement): average seconds per launch over reps launches.
+extern "C" int mf_time_gemm(int32_t precision, int64_t m, int64_t n, int64_t k, int64_t nprob, int32_t tile,
+                            int32_t reps, double *seconds)
+{
+    if (!seconds || m < 1 || n < 1 || k < 1 || nprob < 1 || reps < 1) return fail(MF_BADARG, "bad gemm timing request");
+    if (tile < 0 || tile >= GEMM_NTILES) return fail(MF_BADARG, "tile index out of range");
+    if (m > INT32_MAX / 2 || n > INT32_MAX / 2 || k > INT32_MAX / 2) return fail(MF_BADARG, "gemm too large");
+    const size_t esz = precision == MF_FP64 ? 8 : 4;
+    const int64_t per = m * k + k * n + m * n;
+    DevBuf pool, dprobs, dtiles;
+    HIPCHK(pool.alloc((size_t)per * nprob * esz));
+    HIPCHK(hipMemset(pool.p, 0, (size_t)per * nprob * esz));
+    std::vector<GemmProb> pv;
+    std::vector<GemmTile> tv;
+    for (int64_t q = 0; q < nprob; q++) {
+        GemmProb g{};
+        g.aoff = q * per;
+        g.boff = q * per + m * k;
+        g.coff = q * per + m * k + k * n;
+        g.m = (int32_t)m; g.n = (int32_t)n; g.k = (int32_t)k;
+        g.lda = (int32_t)m; g.ldb = (int32_t)k; g.ldc = (int32_t)m;
+        pv.push_back(g);
+        for (int64_t tn = 0; tn < n; tn += GEMM_TILE_N[tile])
+            for (int64_t tm = 0; tm < m; tm += GEMM_TILE_M[tile]) tv.push_back({(int32_t)q, (int32_t)tm, (int32_t)tn, 0});
+    }
+    HIPCHK(upload(dprobs, pv));
+    HIPCHK(upload(dtiles, tv));
+    hipStream_t st;
+    HIPCHK(hipStreamCreate(&st));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    auto launch = [&]() {
+        return esz == 8 ? launch_gemm_tile<double>(tile, dprobs.as<GemmProb>(), dtiles.as<GemmTile>(), (int)tv.size(),
+                                                    pool.p, 0, 1, st)
+                        : launch_gemm_tile<float>(tile, dprobs.as<GemmProb>(), dtiles.as<GemmTile>(), (int)tv.size(),
+                                                   pool.p, 0, 1, st);
+    };
+    hipError_t err = launch();   // warm-up
+    if (err == hipSuccess) err = hipEventRecord(e0, st);
+    for (int r = 0; r < reps && err == hipSuccess; r++) err = launch();
+    if (err == hipSuccess) err = hipEventRecord(e1, st);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    float ms = 0;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    HIPCHK(err);
+    *seconds = ms * 1e-3 / reps;
     return MF_OK;
 }

@@ -13,7 +13,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG_DIR)                  # pureumfpack.jl_amd/
 CSRC = os.path.join(ROOT, "csrc")
 LIB_PATH = os.path.join(ROOT, "libmfgpu.so")
-SOURCES = ["mf_analysis.cpp", "mf_plan.hip", "mf_factor_kernels.hip", "mf_solve_kernels.hip"]
+SOURCES = ["mf_analysis.cpp", "mf_plan.hip", "mf_factor_kernels.hip", "mf_solve_kernels.hip", "mf_mlp.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

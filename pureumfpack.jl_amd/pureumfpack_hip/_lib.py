@@ -16,6 +16,8 @@ MF_PIVOT_AUTO, MF_PIVOT_PARTIAL = 0, 1
 
 c_i64p = ctypes.POINTER(ctypes.c_int64)
 c_f64p = ctypes.POINTER(ctypes.c_double)
+c_i32p = ctypes.POINTER(ctypes.c_int32)
+c_f32p = ctypes.POINTER(ctypes.c_float)
 c_void_pp = ctypes.POINTER(ctypes.c_void_p)
 
 
@@ -65,6 +67,21 @@ SIGNATURES = {
                                          c_i64p]),
     "mf_export_factors_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, c_i64p, c_i64p, c_f64p, c_i64p, c_i64p, c_f64p, c_i64p,
                                          c_i64p]),
+    "mf_set_gemm_tiles": (ctypes.c_int, [ctypes.c_int32, c_i32p]),
+    "mf_get_gemm_tiles": (ctypes.c_int, [ctypes.c_int32, c_i32p]),
+    "mf_gemm_tile_info": (ctypes.c_int, [c_i32p, c_i32p, c_i32p, c_i32p, c_i32p]),
+    "mf_time_gemm": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                    ctypes.c_int32, ctypes.c_int32, c_f64p]),
+    "mf_mlp_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+    "mf_mlp_destroy": (None, [ctypes.c_void_p]),
+    "mf_mlp_num_params": (ctypes.c_int64, [ctypes.c_void_p]),
+    "mf_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, c_f32p]),
+    "mf_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, c_f32p]),
+    "mf_mlp_forward": (ctypes.c_int, [ctypes.c_void_p, c_f32p, ctypes.c_int64, c_f32p]),
+    "mf_mlp_gradient": (ctypes.c_int, [ctypes.c_void_p, c_f32p, c_f32p, ctypes.c_int64, c_f32p, c_f32p]),
+    "mf_mlp_train": (ctypes.c_int, [ctypes.c_void_p, c_f32p, c_f32p, ctypes.c_int64, ctypes.c_int32, ctypes.c_float,
+                                    c_f32p]),
+    "mf_mlp_last_error": (ctypes.c_char_p, []),
     "mf_plan_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(mf_plan_stats)]),
     "mf_plan_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "mf_plan_get_profile": (ctypes.c_int, [ctypes.c_void_p, c_f64p, c_i64p, c_f64p, c_f64p]),
