@@ -223,6 +223,18 @@ def cpu_baseline(A, S, Rs, B, refine):
             "sample": f"full workload: FP64 factor + {B.shape[1]} x solve(F, b; refine={refine})"}
 
 
+def measured_traffic(k, path):
+    """HBM bytes per Schur-gemm launch from the committed rocprofv3 PMC summary (FETCH_SIZE and
+    WRITE_SIZE calibrated against a copy kernel), when it covers this workload and path."""
+    f = os.path.join(ROOT, "profiles", f"pmc_traffic_poisson3d{k}_{path}.md")
+    if not os.path.exists(f):
+        return None
+    for line in open(f):
+        if line.startswith("schur_gemm_traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,7 +248,14 @@ def main():
     ap.add_argument("--paths", default="fp64,fp32")
     ap.add_argument("--graph", type=int, default=1, help="replay the factor schedule as a hipGraph")
     ap.add_argument("--batch", default="512,16,64", help="nmat,k,nrhs of the same-pattern batch run ('' = skip)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a plain device copy of this many bytes (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        sec = ctypes.c_double()
+        rc = _lib.load().mf_copy_probe(args.copy_probe, ctypes.byref(sec))
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "seconds": sec.value, "rc": rc}))
+        sys.exit(0 if rc == 0 else 1)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the numeric phase runs on a single MI355X "
               "(multi-GPU sharding is a design, not a build target).", file=sys.stderr)
@@ -260,7 +279,10 @@ def main():
     R = results[main_p]
     prof = R["profile"]["schur_gemm"]
     roof = {"bound": "mfma", "kernel": "k_gemm (Schur update)", "achieved": prof["tflops"],
-            "peak": PEAK[main_p], "unit": "TFLOP/s", "frac": prof["tflops"] / PEAK[main_p], "traffic": None}
+            "peak": PEAK[main_p], "unit": "TFLOP/s", "frac": prof["tflops"] / PEAK[main_p],
+            "traffic": measured_traffic(args.k, main_p),
+            "algorithmic_bytes_per_launch": prof["gbytes"] * 1e9 / max(1, prof["launches"]),
+            "launch_us": prof["ms"] * 1e3 / max(1, prof["launches"])}
     out = {
         "metric": METRIC, "value": R["seconds"], "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": R["seconds"] * 1e3, "higher_is_better": False, "scaling": "weak",

@@ -202,6 +202,10 @@ int mf_gemm_tile_info(int32_t *ntiles, int32_t *tile_m, int32_t *tile_n, int32_t
 int mf_time_gemm(int32_t precision, int64_t m, int64_t n, int64_t k, int64_t nprob, int32_t tile, int32_t reps,
                  double *seconds);
 
+/* Copies `bytes` (a multiple of 16) between two fresh device buffers with a plain 16-byte
+ * copy kernel: the known traffic for calibrating hardware byte counters. */
+int mf_copy_probe(int64_t bytes, double *seconds);
+
 /* ---- small MLP cost model, trained on the GPU (no reference counterpart) ----
  * d_in -> hidden (ReLU) -> hidden (ReLU) -> 1, FP32; samples row-major (n x d_in); loss =
  * mean squared error; parameters: per layer W (d_out x d_in, row-major) then b. */

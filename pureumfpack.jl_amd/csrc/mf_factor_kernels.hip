@@ -689,7 +689,9 @@ template <> struct Mfma<float> {
     static __device__ __forceinline__ int row(int lane, int q) { return 4 * (lane >> 4) + q; }
 };
 
-template <typename T, int TM, int TN>
+// USE (0 panel update / inversion, 1 Schur complement) only names the instantiation, so that
+// profilers report the Schur complement launches apart; the code is the same.
+template <typename T, int TM, int TN, int USE>
 __global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs,
                                               const GemmTile *__restrict__ tiles, void *pool_v,
                                               int64_t pool_stride)
@@ -764,23 +766,31 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs
 }
 
 template <typename T>
-hipError_t launch_gemm_tile(int tile, const GemmProb *probs, const GemmTile *tiles, int count, void *pool,
+hipError_t launch_gemm_tile(int tile, int use, const GemmProb *probs, const GemmTile *tiles, int count, void *pool,
                             int64_t pool_stride, int nmat, hipStream_t st)
 {
     const dim3 grid(count, nmat), blk(256);
+#define MF_GEMM_CASE(TI, TM, TN)                                                                              \
+    case TI:                                                                                                 \
+        if (use == 1)                                                                                        \
+            hipLaunchKernelGGL((k_gemm<T, TM, TN, 1>), grid, blk, 0, st, probs, tiles, pool, pool_stride);   \
+        else                                                                                                 \
+            hipLaunchKernelGGL((k_gemm<T, TM, TN, 0>), grid, blk, 0, st, probs, tiles, pool, pool_stride);   \
+        break;
     switch (tile) {
-    case 0: hipLaunchKernelGGL((k_gemm<T, 64, 64>), grid, blk, 0, st, probs, tiles, pool, pool_stride); break;
-    case 1: hipLaunchKernelGGL((k_gemm<T, 128, 64>), grid, blk, 0, st, probs, tiles, pool, pool_stride); break;
-    case 2: hipLaunchKernelGGL((k_gemm<T, 64, 128>), grid, blk, 0, st, probs, tiles, pool, pool_stride); break;
-    case 3: hipLaunchKernelGGL((k_gemm<T, 128, 128>), grid, blk, 0, st, probs, tiles, pool, pool_stride); break;
+        MF_GEMM_CASE(0, 64, 64)
+        MF_GEMM_CASE(1, 128, 64)
+        MF_GEMM_CASE(2, 64, 128)
+        MF_GEMM_CASE(3, 128, 128)
     default: return hipErrorInvalidValue;
     }
+#undef MF_GEMM_CASE
     return hipGetLastError();
 }
-template hipError_t launch_gemm_tile<double>(int, const GemmProb *, const GemmTile *, int, void *, int64_t, int,
-                                             hipStream_t);
-template hipError_t launch_gemm_tile<float>(int, const GemmProb *, const GemmTile *, int, void *, int64_t, int,
-                                            hipStream_t);
+template hipError_t launch_gemm_tile<double>(int, int, const GemmProb *, const GemmTile *, int, void *, int64_t,
+                                             int, hipStream_t);
+template hipError_t launch_gemm_tile<float>(int, int, const GemmProb *, const GemmTile *, int, void *, int64_t,
+                                            int, hipStream_t);
 
 // ------------------------------------------------------------------------------------------
 // Host launchers
@@ -824,7 +834,8 @@ hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int
                            reinterpret_cast<const T *>(a.spec));
         break;
     case L_GEMM:
-        return launch_gemm_tile<T>(L.tile, probs + L.prob_off, tiles + L.list_off, L.count, base.pool,
+        return launch_gemm_tile<T>(L.tile, L.variant == 1 ? 1 : 0, probs + L.prob_off, tiles + L.list_off, L.count,
+                                   base.pool,
                                    base.pool_stride, nmat, st);
     default:
         return hipErrorInvalidValue;

@@ -112,8 +112,9 @@ struct SolveArgs {
 
 // ---- launchers (mf_factor_kernels.hip, mf_solve_kernels.hip) ----
 template <typename T>
-hipError_t launch_gemm_tile(int tile, const GemmProb *probs, const GemmTile *tiles, int count, void *pool,
+hipError_t launch_gemm_tile(int tile, int use, const GemmProb *probs, const GemmTile *tiles, int count, void *pool,
                             int64_t pool_stride, int nmat, hipStream_t st);
+hipError_t launch_copy(void *dst, const void *src, int64_t bytes, hipStream_t st);
 template <typename T>
 hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int32_t *lists,
                               const GemmProb *probs, const GemmTile *tiles, int nmat, hipStream_t st);

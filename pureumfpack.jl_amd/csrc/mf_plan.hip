@@ -1595,9 +1595,9 @@ extern "C" int mf_time_gemm(int32_t precision, int64_t m, int64_t n, int64_t k, 
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     auto launch = [&]() {
-        return esz == 8 ? launch_gemm_tile<double>(tile, dprobs.as<GemmProb>(), dtiles.as<GemmTile>(), (int)tv.size(),
+        return esz == 8 ? launch_gemm_tile<double>(tile, 1, dprobs.as<GemmProb>(), dtiles.as<GemmTile>(), (int)tv.size(),
                                                     pool.p, 0, 1, st)
-                        : launch_gemm_tile<float>(tile, dprobs.as<GemmProb>(), dtiles.as<GemmTile>(), (int)tv.size(),
+                        : launch_gemm_tile<float>(tile, 1, dprobs.as<GemmProb>(), dtiles.as<GemmTile>(), (int)tv.size(),
                                                    pool.p, 0, 1, st);
     };
     hipError_t err = launch();   // warm-up
@@ -1612,5 +1612,31 @@ extern "C" int mf_time_gemm(int32_t precision, int64_t m, int64_t n, int64_t k, 
     (void)hipStreamDestroy(st);
     HIPCHK(err);
     *seconds = ms * 1e-3 / reps;
+    return MF_OK;
+}
+
+// A plain copy of `bytes` bytes between two fresh device buffers (16-byte vectors): the known
+// traffic against which hardware byte counters are calibrated.
+extern "C" int mf_copy_probe(int64_t bytes, double *seconds)
+{
+    if (bytes < 16 || bytes % 16) return fail(MF_BADARG, "copy probe size must be a positive multiple of 16");
+    DevBuf src, dst;
+    HIPCHK(src.alloc((size_t)bytes));
+    HIPCHK(dst.alloc((size_t)bytes));
+    HIPCHK(hipMemset(src.p, 1, (size_t)bytes));
+    HIPCHK(hipDeviceSynchronize());
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    hipError_t err = hipEventRecord(e0, 0);
+    if (err == hipSuccess) err = launch_copy(dst.p, src.p, bytes, 0);
+    if (err == hipSuccess) err = hipEventRecord(e1, 0);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    float ms = 0;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(err);
+    if (seconds) *seconds = ms * 1e-3;
     return MF_OK;
 }

@@ -676,6 +676,21 @@ __global__ __launch_bounds__(256) void k_bwd_front_b(SolveArgs a)
     }
 }
 
+__global__ __launch_bounds__(256) void k_copy16(int4 *__restrict__ dst, const int4 *__restrict__ src, int64_t n16)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy(void *dst, const void *src, int64_t bytes, hipStream_t st)
+{
+    const int64_t n16 = bytes / 16;
+    const unsigned grid = (unsigned)std::min<int64_t>((n16 + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, st, reinterpret_cast<int4 *>(dst),
+                       reinterpret_cast<const int4 *>(src), n16);
+    return hipGetLastError();
+}
+
 // w0[i] = Rs[qf[i]] * B[qf[i]] in the solve type: the scale-and-permute of
 // src/interface.jl:128-130 expressed in V-row order.
 template <typename T>

@@ -82,6 +82,7 @@ SIGNATURES = {
     "mf_mlp_train": (ctypes.c_int, [ctypes.c_void_p, c_f32p, c_f32p, ctypes.c_int64, ctypes.c_int32, ctypes.c_float,
                                     c_f32p]),
     "mf_mlp_last_error": (ctypes.c_char_p, []),
+    "mf_copy_probe": (ctypes.c_int, [ctypes.c_int64, c_f64p]),
     "mf_plan_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(mf_plan_stats)]),
     "mf_plan_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "mf_plan_get_profile": (ctypes.c_int, [ctypes.c_void_p, c_f64p, c_i64p, c_f64p, c_f64p]),
