@@ -765,6 +765,136 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs
         }
 }
 
+// ------------------------------------------------------------------------------------------
+// Schur complement in BF16 (precision MF_BF16): C -= A B with A, B held in FP32 fronts.  Each
+// k-slice of BK = 32 is scaled per tile by powers of two (A slice by 2^-ea, B slice by 2^-eb,
+// ea and eb the exponents of the slices' largest magnitudes), rounded to BF16 (nearest even),
+// multiplied on MFMA 16x16x16 BF16 with FP32 accumulation into a per-slice accumulator, and
+// added to the FP32 result times 2^(ea+eb) (exact).  64 x 64 tiles, 4 waves of 32 x 32.
+// ------------------------------------------------------------------------------------------
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short to_bf16(float x)
+{
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even (finite inputs)
+    return (short)(u >> 16);
+}
+
+__device__ __forceinline__ float wave_max(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// 2^-e with e the exponent of mx (mx in [2^(e-1), 2^e)); 1 for mx == 0
+__device__ __forceinline__ float pow2_scale(float mx)
+{
+    if (!(mx > 0.0f) || !isfinite(mx)) return 1.0f;
+    int e;
+    (void)frexpf(mx, &e);
+    return ldexpf(1.0f, -e);
+}
+
+__global__ __launch_bounds__(256) void k_gemm_bf16(const GemmProb *__restrict__ probs,
+                                                   const GemmTile *__restrict__ tiles, void *pool_v,
+                                                   int64_t pool_stride)
+{
+    constexpr int TM = 64, TN = 64, BK = 32, WM = 32, WN = 32, FM = 2, FN = 2;
+    __shared__ float As[BK][TM + 1];
+    __shared__ float Bs[BK][TN + 1];
+    __shared__ float smax[2][4];
+    typedef float acc4 __attribute__((ext_vector_type(4)));
+    const GemmTile tl = tiles[blockIdx.x];
+    const GemmProb pr = probs[tl.prob];
+    float *pool = reinterpret_cast<float *>(pool_v) + blockIdx.y * pool_stride;
+    const float *A = pool + pr.aoff;
+    const float *B = pool + pr.boff;
+    float *C = pool + pr.coff;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int wm = (wid & 1) * WM, wn = (wid >> 1) * WN;
+    const int m0 = tl.tm, n0 = tl.tn;
+    acc4 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++) acc[i][j] = acc4{0, 0, 0, 0};
+    for (int k0 = 0; k0 < pr.k; k0 += BK) {
+        float ma = 0.0f, mb = 0.0f;
+#pragma unroll
+        for (int i = 0; i < TM * BK / 256; i++) {
+            const int e = tid + 256 * i, r = e % TM, kk = e / TM;
+            const int gr = m0 + r, gk = k0 + kk;
+            const float v = (gr < pr.m && gk < pr.k) ? A[gr + (int64_t)gk * pr.lda] : 0.0f;
+            As[kk][r] = v;
+            ma = fmaxf(ma, fabsf(v));
+        }
+#pragma unroll
+        for (int i = 0; i < TN * BK / 256; i++) {
+            const int e = tid + 256 * i, kk = e % BK, cc = e / BK;
+            const int gc = n0 + cc, gk = k0 + kk;
+            const float v = (gc < pr.n && gk < pr.k) ? B[gk + (int64_t)gc * pr.ldb] : 0.0f;
+            Bs[kk][cc] = v;
+            mb = fmaxf(mb, fabsf(v));
+        }
+        ma = wave_max(ma);
+        mb = wave_max(mb);
+        if (lane == 0) {
+            smax[0][wid] = ma;
+            smax[1][wid] = mb;
+        }
+        __syncthreads();
+        const float sa = pow2_scale(fmaxf(fmaxf(smax[0][0], smax[0][1]), fmaxf(smax[0][2], smax[0][3])));
+        const float sb = pow2_scale(fmaxf(fmaxf(smax[1][0], smax[1][1]), fmaxf(smax[1][2], smax[1][3])));
+        acc4 sl[FM][FN];
+#pragma unroll
+        for (int i = 0; i < FM; i++)
+#pragma unroll
+            for (int j = 0; j < FN; j++) sl[i][j] = acc4{0, 0, 0, 0};
+#pragma unroll
+        for (int kq = 0; kq < BK; kq += 16) {
+            // lane l holds A[row l%16][k 4(l/16) .. +3] and B[k 4(l/16) .. +3][col l%16]
+            const int kb = kq + 4 * (lane >> 4);
+            bf16x4_t av[FM], bv[FN];
+#pragma unroll
+            for (int i = 0; i < FM; i++) {
+                const int r = wm + 16 * i + (lane & 15);
+                av[i] = bf16x4_t{to_bf16(As[kb][r] * sa), to_bf16(As[kb + 1][r] * sa), to_bf16(As[kb + 2][r] * sa),
+                                 to_bf16(As[kb + 3][r] * sa)};
+            }
+#pragma unroll
+            for (int j = 0; j < FN; j++) {
+                const int c = wn + 16 * j + (lane & 15);
+                bv[j] = bf16x4_t{to_bf16(Bs[kb][c] * sb), to_bf16(Bs[kb + 1][c] * sb), to_bf16(Bs[kb + 2][c] * sb),
+                                 to_bf16(Bs[kb + 3][c] * sb)};
+            }
+#pragma unroll
+            for (int i = 0; i < FM; i++)
+#pragma unroll
+                for (int j = 0; j < FN; j++)
+                    sl[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[i], bv[j], sl[i][j], 0, 0, 0);
+        }
+        const float un = 1.0f / (sa * sb);   // a power of two: exact
+#pragma unroll
+        for (int i = 0; i < FM; i++)
+#pragma unroll
+            for (int j = 0; j < FN; j++) acc[i][j] += sl[i][j] * un;
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++) {
+            const int gc = n0 + wn + 16 * j + (lane & 15);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int gr = m0 + wm + 16 * i + Mfma<float>::row(lane, q);
+                if (gr < pr.m && gc < pr.n) C[gr + (int64_t)gc * pr.ldc] -= acc[i][j][q];
+            }
+        }
+}
+
 template <typename T>
 hipError_t launch_gemm_tile(int tile, int use, const GemmProb *probs, const GemmTile *tiles, int count, void *pool,
                             int64_t pool_stride, int nmat, hipStream_t st)
@@ -777,6 +907,11 @@ hipError_t launch_gemm_tile(int tile, int use, const GemmProb *probs, const Gemm
         else                                                                                                 \
             hipLaunchKernelGGL((k_gemm<T, TM, TN, 0>), grid, blk, 0, st, probs, tiles, pool, pool_stride);   \
         break;
+    if (tile == GEMM_TILE_BF16) {
+        if (sizeof(T) != 4) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_gemm_bf16, grid, blk, 0, st, probs, tiles, pool, pool_stride);
+        return hipGetLastError();
+    }
     switch (tile) {
         MF_GEMM_CASE(0, 64, 64)
         MF_GEMM_CASE(1, 128, 64)

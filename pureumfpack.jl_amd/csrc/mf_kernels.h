@@ -29,6 +29,7 @@ constexpr int GEMM_BM = 64, GEMM_BN = 64, GEMM_BK = 16;
 // shapes accumulate every C element over k in the same order (BK = 16 slices, MFMA k = 4),
 // so the choice of tile never changes a result bit.
 constexpr int GEMM_NTILES = 4;
+constexpr int GEMM_TILE_BF16 = 64;   // the BF16 Schur-complement kernel (64 x 64, FP32 fronts)
 constexpr int GEMM_TILE_M[GEMM_NTILES] = {64, 128, 64, 128};
 constexpr int GEMM_TILE_N[GEMM_NTILES] = {64, 64, 128, 128};
 // tile policy: size buckets of sqrt(m n) (upper bounds, the last open-ended) for the two gemm

@@ -508,15 +508,16 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
     auto add_gemm = [&](std::vector<GemmProb> &pv, int lvl, int schur) {
         if (pv.empty()) return;
         const int use = schur == 1 ? 1 : 0;
+        const bool bf16 = use == 1 && P.prec == MF_BF16;
         for (int tile = 0; tile < GEMM_NTILES; tile++) {
             std::vector<const GemmProb *> sel;
             for (const GemmProb &g : pv)
-                if (policy[use * GEMM_NBUCKETS + gemm_bucket(g.m, g.n)] == tile) sel.push_back(&g);
+                if (bf16 ? tile == 0 : policy[use * GEMM_NBUCKETS + gemm_bucket(g.m, g.n)] == tile) sel.push_back(&g);
             if (sel.empty()) continue;
             Launch L{};
             L.kind = L_GEMM;
             L.variant = schur;
-            L.tile = tile;
+            L.tile = bf16 ? GEMM_TILE_BF16 : tile;
             for (const GemmProb *g : sel) {
                 L.flops += 2.0 * g->m * (double)g->n * g->k;
                 L.bytes += ((double)g->m * g->k + (double)g->k * g->n + 2.0 * g->m * (double)g->n) * ebytes;
