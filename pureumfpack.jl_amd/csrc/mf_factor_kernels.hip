@@ -691,17 +691,24 @@ template <> struct Mfma<float> {
 
 // USE (0 panel update / inversion, 1 Schur complement) only names the instantiation, so that
 // profilers report the Schur complement launches apart; the code is the same.
+//
+// 4 waves in a 2 x 2 arrangement, each owning a (TM/2) x (TN/2) block of C as FM x FN MFMA
+// 16 x 16 accumulators.  Slices of BK columns of A and rows of B are double-buffered in LDS:
+// the global loads of slice s+1 are in flight (registers) while slice s is multiplied, then
+// stored to the other buffer, one barrier per slice.  The MFMA computes B^T A^T, so each lane
+// ends up with C elements of 16 consecutive rows: the read-modify-write of C is coalesced.
+// Every element of C is accumulated over k in the same order for every tile shape.
 template <typename T, int TM, int TN, int USE>
 __global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs,
                                               const GemmTile *__restrict__ tiles, void *pool_v,
                                               int64_t pool_stride)
 {
-    // 4 waves in a 2 x 2 arrangement, each owning a (TM/2) x (TN/2) block of C as FM x FN
-    // MFMA 16 x 16 accumulators; A and B slices of BK = 16 staged in LDS per step.
     typedef typename Mfma<T>::acc_t acc_t;
-    constexpr int BK = GEMM_BK, WM = TM / 2, WN = TN / 2, FM = WM / 16, FN = WN / 16;
-    __shared__ T As[BK][TM + 1];
-    __shared__ T Bs[BK][TN + 1];
+    constexpr int BK = sizeof(T) == 8 ? 16 : 32;
+    constexpr int WM = TM / 2, WN = TN / 2, FM = WM / 16, FN = WN / 16;
+    constexpr int NA = TM * BK / 256, NB = TN * BK / 256;
+    __shared__ T As[2][BK][TM + 1];
+    __shared__ T Bs[2][BK][TN + 1];
     const GemmTile tl = tiles[blockIdx.x];
     const GemmProb pr = probs[tl.prob];
     T *pool = reinterpret_cast<T *>(pool_v) + blockIdx.y * pool_stride;
@@ -716,46 +723,68 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs
     for (int i = 0; i < FM; i++)
 #pragma unroll
         for (int j = 0; j < FN; j++) acc[i][j] = acc_t{0, 0, 0, 0};
-
-    for (int k0 = 0; k0 < pr.k; k0 += BK) {
-        // A slice TM x BK: consecutive threads walk a column (coalesced)
+    T ra[NA], rb[NB];
+    // A slice TM x BK: consecutive threads walk a column (coalesced); B slice BK x TN:
+    // consecutive threads walk k within a column
+    auto gload = [&](int k0) {
 #pragma unroll
-        for (int i = 0; i < TM * BK / 256; i++) {
+        for (int i = 0; i < NA; i++) {
             const int e = tid + 256 * i, r = e % TM, kk = e / TM;
             const int gr = m0 + r, gk = k0 + kk;
-            As[kk][r] = (gr < pr.m && gk < pr.k) ? A[gr + (int64_t)gk * pr.lda] : T(0);
+            ra[i] = (gr < pr.m && gk < pr.k) ? A[gr + (int64_t)gk * pr.lda] : T(0);
         }
-        // B slice BK x TN: consecutive threads walk k within a column
 #pragma unroll
-        for (int i = 0; i < TN * BK / 256; i++) {
+        for (int i = 0; i < NB; i++) {
             const int e = tid + 256 * i, kk = e % BK, cc = e / BK;
             const int gc = n0 + cc, gk = k0 + kk;
-            Bs[kk][cc] = (gc < pr.n && gk < pr.k) ? B[gk + (int64_t)gc * pr.ldb] : T(0);
+            rb[i] = (gc < pr.n && gk < pr.k) ? B[gk + (int64_t)gc * pr.ldb] : T(0);
         }
-        __syncthreads();
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < NA; i++) {
+            const int e = tid + 256 * i;
+            As[buf][e / TM][e % TM] = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            const int e = tid + 256 * i;
+            Bs[buf][e % BK][e / BK] = rb[i];
+        }
+    };
+    const int nk = (pr.k + BK - 1) / BK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int s = 0; s < nk; s++) {
+        const int cur = s & 1;
+        if (s + 1 < nk) gload((s + 1) * BK);
 #pragma unroll
         for (int kq = 0; kq < BK; kq += 4) {
             T av[FM], bv[FN];
 #pragma unroll
-            for (int i = 0; i < FM; i++) av[i] = As[kq + (lane >> 4)][wm + 16 * i + (lane & 15)];
+            for (int i = 0; i < FM; i++) av[i] = As[cur][kq + (lane >> 4)][wm + 16 * i + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < FN; j++) bv[j] = Bs[kq + (lane >> 4)][wn + 16 * j + (lane & 15)];
+            for (int j = 0; j < FN; j++) bv[j] = Bs[cur][kq + (lane >> 4)][wn + 16 * j + (lane & 15)];
 #pragma unroll
             for (int i = 0; i < FM; i++)
 #pragma unroll
-                for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::mma(av[i], bv[j], acc[i][j]);
+                for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::mma(bv[j], av[i], acc[i][j]);
         }
+        if (s + 1 < nk) sstore(cur ^ 1);
         __syncthreads();
     }
+    // acc[i][j][q] = C[wm + 16 i + lane % 16][wn + 16 j + row(lane, q)]
     const bool add = pr.flags & GEMM_ADD, set = pr.flags & GEMM_SET;
+    const int gr0 = m0 + wm + (lane & 15);
 #pragma unroll
     for (int i = 0; i < FM; i++)
 #pragma unroll
         for (int j = 0; j < FN; j++) {
-            int gc = n0 + wn + 16 * j + (lane & 15);
+            const int gr = gr0 + 16 * i;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                int gr = m0 + wm + 16 * i + Mfma<T>::row(lane, q);
+                const int gc = n0 + wn + 16 * j + Mfma<T>::row(lane, q);
                 if (gr < pr.m && gc < pr.n) {
                     T *cp = C + gr + (int64_t)gc * pr.ldc;
                     const T v = add ? acc[i][j][q] : -acc[i][j][q];

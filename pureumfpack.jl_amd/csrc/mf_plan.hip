@@ -31,6 +31,19 @@ static thread_local std::string g_err;
 // bucket; read when a plan is created.  Default: the 64 x 64 tile everywhere.
 static int32_t g_tile_policy[2][2 * GEMM_NBUCKETS] = {};
 
+// Workgroup i of a launch runs on XCD i % 8 (round-robin dispatch over the 8 XCDs, each with
+// its own L2).  Reorder a launch's tiles [begin, end) so that XCD x receives the contiguous
+// x-th eighth of the original (problem, column block, row block) order: neighbouring tiles,
+// which share rows of A or columns of B, then share an L2.
+static void xcd_order(std::vector<GemmTile> &t, size_t begin)
+{
+    constexpr size_t NX = 8;
+    const size_t cnt = t.size() - begin, per = cnt / NX;
+    if (per < 2) return;
+    std::vector<GemmTile> o(t.begin() + begin, t.end());
+    for (size_t p = 0; p < per * NX; p++) t[begin + p] = o[(p % NX) * per + p / NX];
+}
+
 static int fail(int code, const std::string &msg)
 {
     g_err = msg;
@@ -532,6 +545,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             }
             L.count = (int32_t)(tiles.size() - L.list_off);
             L.level = lvl;
+            xcd_order(tiles, L.list_off);
             if (L.count > 0) gemm_dst->push_back(L);
         }
     };
@@ -1588,6 +1602,7 @@ extern "C" int mf_time_gemm(int32_t precision, int64_t m, int64_t n, int64_t k, 
         for (int64_t tn = 0; tn < n; tn += GEMM_TILE_N[tile])
             for (int64_t tm = 0; tm < m; tm += GEMM_TILE_M[tile]) tv.push_back({(int32_t)q, (int32_t)tm, (int32_t)tn, 0});
     }
+    xcd_order(tv, 0);
     HIPCHK(upload(dprobs, pv));
     HIPCHK(upload(dtiles, tv));
     hipStream_t st;
