@@ -689,8 +689,9 @@ template <> struct Mfma<float> {
     static __device__ __forceinline__ int row(int lane, int q) { return 4 * (lane >> 4) + q; }
 };
 
-// USE (0 panel update / inversion, 1 Schur complement) only names the instantiation, so that
-// profilers report the Schur complement launches apart; the code is the same.
+// USE (0 panel update / inversion, 1 Schur complement, 2 Schur complement of a front with
+// nf >= 1024) only names the instantiation, so that profilers report them apart; the code is
+// the same.
 //
 // 4 waves in a 2 x 2 arrangement, each owning a (TM/2) x (TN/2) block of C as FM x FN MFMA
 // 16 x 16 accumulators.  Slices of BK columns of A and rows of B are double-buffered in LDS:
@@ -931,7 +932,9 @@ hipError_t launch_gemm_tile(int tile, int use, const GemmProb *probs, const Gemm
     const dim3 grid(count, nmat), blk(256);
 #define MF_GEMM_CASE(TI, TM, TN)                                                                              \
     case TI:                                                                                                 \
-        if (use == 1)                                                                                        \
+        if (use == 2)                                                                                        \
+            hipLaunchKernelGGL((k_gemm<T, TM, TN, 2>), grid, blk, 0, st, probs, tiles, pool, pool_stride);   \
+        else if (use == 1)                                                                                   \
             hipLaunchKernelGGL((k_gemm<T, TM, TN, 1>), grid, blk, 0, st, probs, tiles, pool, pool_stride);   \
         else                                                                                                 \
             hipLaunchKernelGGL((k_gemm<T, TM, TN, 0>), grid, blk, 0, st, probs, tiles, pool, pool_stride);   \
@@ -998,7 +1001,8 @@ hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int
                            reinterpret_cast<const T *>(a.spec));
         break;
     case L_GEMM:
-        return launch_gemm_tile<T>(L.tile, L.variant == 1 ? 1 : 0, probs + L.prob_off, tiles + L.list_off, L.count,
+        return launch_gemm_tile<T>(L.tile, L.variant == 3 ? 2 : L.variant == 1 ? 1 : 0, probs + L.prob_off,
+                                   tiles + L.list_off, L.count,
                                    base.pool,
                                    base.pool_stride, nmat, st);
     default:

@@ -12,6 +12,7 @@
 #include <climits>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <new>
 #include <string>
@@ -208,7 +209,7 @@ static int launch_class(const Launch &L)
     case L_PANEL_APPLY: return 2;
     case L_SPEC_DIAG: return 2;
     case L_SPEC_TRSM: return 2;
-    default: return L.variant == 1 ? 4 : 3;
+    default: return L.variant == 3 ? 7 : L.variant == 1 ? 4 : 3;
     }
 }
 
@@ -518,9 +519,19 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
     // gemm launches: the problems of one call are grouped by the tile shape the policy picks
     // for their size bucket (one launch per shape used)
     const int32_t *policy = g_tile_policy[P.prec == MF_FP64 ? 0 : 1];
-    auto add_gemm = [&](std::vector<GemmProb> &pv, int lvl, int schur) {
+    std::function<void(std::vector<GemmProb> &, int, int)> add_gemm;
+    add_gemm = [&](std::vector<GemmProb> &pv, int lvl, int schur) {
         if (pv.empty()) return;
-        const int use = schur == 1 ? 1 : 0;
+        if (schur == 1) {
+            // Schur complements of fronts with nf >= 1024 get launches of their own (variant 3),
+            // so their MFMA utilisation is measured apart
+            std::vector<GemmProb> big, rest;
+            for (const GemmProb &g : pv) ((int64_t)g.m + g.k >= 1024 ? big : rest).push_back(g);
+            if (!big.empty()) add_gemm(big, lvl, 3);
+            if (rest.empty()) return;
+            pv.swap(rest);
+        }
+        const int use = schur == 1 || schur == 3 ? 1 : 0;
         const bool bf16 = use == 1 && P.prec == MF_BF16;
         for (int tile = 0; tile < GEMM_NTILES; tile++) {
             std::vector<const GemmProb *> sel;

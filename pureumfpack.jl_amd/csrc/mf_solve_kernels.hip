@@ -103,7 +103,7 @@ __global__ __launch_bounds__(256) void k_fwd_gather(SolveArgs a, const int32_t *
     T *w0 = reinterpret_cast<T *>(a.w0) + vb * a.n;
     T *uv = reinterpret_cast<T *>(a.uv) + vb * a.nupd;
     const int q0 = ecolptr[d.fcol + p], q1 = ecolptr[d.fcol + p + 1];
-    for (int r0 = 0; r0 < (int)a.nrhs; r0 += SOLVE_RC) {
+    for (int r0 = blockIdx.z * SOLVE_RC; r0 < (int)a.nrhs; r0 += gridDim.z * SOLVE_RC) {
         const int rc = min(SOLVE_RC, (int)a.nrhs - r0);
         T acc[SOLVE_RC];
 #pragma unroll
@@ -148,11 +148,12 @@ __global__ __launch_bounds__(256) void k_fwd_tri(SolveArgs a)
     T *z = inv ? reinterpret_cast<T *>(a.zbuf) + vb * a.n : y;
     for (int e = tid; e < np * nrhs; e += nt) {
         const int i = e % np, r = e / np;
+        if ((r / SOLVE_RC) % (int)gridDim.z != (int)blockIdx.z) continue;   // another chunk's block
         z[r * a.n + c1 + i] = w0[r * a.n + prow[c1 + i]];
     }
     if (inv) return;
     __syncthreads();
-    for (int r0 = 0; r0 < nrhs; r0 += SOLVE_RC) {
+    for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
         for (int b0 = 0; b0 < np; b0 += SOLVE_NB) {
             const int bs = min(SOLVE_NB, np - b0);
@@ -224,7 +225,7 @@ __global__ __launch_bounds__(256) void k_rect_part(SolveArgs a, int kind)
     default: rowp = pool + d.ioff + (int64_t)np * np + row; ld = np; break;
     }
     const int nrhs = (int)a.nrhs;
-    for (int r0 = 0; r0 < nrhs; r0 += SOLVE_RC) {
+    for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
         T acc[SOLVE_RC] = {};
         for (int j0 = w.z; j0 < cend; j0 += SOLVE_NB) {
@@ -294,7 +295,7 @@ __global__ __launch_bounds__(256) void k_bwd_tri(SolveArgs a)
     const int64_t vb = (int64_t)mat * a.nrhs;
     T *x = reinterpret_cast<T *>(a.x) + vb * a.n;
     const int nrhs = (int)a.nrhs;
-    for (int r0 = 0; r0 < nrhs; r0 += SOLVE_RC) {
+    for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
         for (int b0 = ((np - 1) / SOLVE_NB) * SOLVE_NB; b0 >= 0; b0 -= SOLVE_NB) {
             const int bs = min(SOLVE_NB, np - b0);
@@ -345,15 +346,14 @@ __global__ __launch_bounds__(256) void k_bwd_tri(SolveArgs a)
 constexpr int FRONT_MAXNF = 1024;
 
 template <typename T>
-__global__ __launch_bounds__(64) void k_fwd_front_w(SolveArgs a, const int32_t *__restrict__ ecolptr,
-                                                    const int2 *__restrict__ ext)
+__device__ __forceinline__ void fwd_front_w(const SolveArgs &a, const int32_t *__restrict__ ecolptr,
+                                            const int2 *__restrict__ ext, int slot, int mat, int zc, int nz)
 {
-    const int mat = blockIdx.y;
-    const SnDev d = a.sn[a.list[blockIdx.x]];
+    const SnDev d = a.sn[a.list[slot]];
     const int nf = d.nf, np = d.np, c1 = d.c1;
     const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
     const int32_t *prow = a.prow + mat * a.n_stride;
-    const int p = threadIdx.x;
+    const int p = threadIdx.x & 63;
     const bool on = p < nf;
     const int64_t vb = (int64_t)mat * a.nrhs;
     const T *w0 = reinterpret_cast<const T *>(a.w0) + vb * a.n;
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(64) void k_fwd_front_w(SolveArgs a, const int32_t *
     const int nrhs = (int)a.nrhs;
     const int q0 = on ? ecolptr[d.fcol + p] : 0, q1 = on ? ecolptr[d.fcol + p + 1] : 0;
     const int src = p < np ? prow[c1 + p] - c1 : p;   // V-row feeding factor position p
-    for (int r0 = 0; r0 < nrhs; r0 += SOLVE_RC) {
+    for (int r0 = zc * SOLVE_RC; r0 < nrhs; r0 += nz * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
         T v[SOLVE_RC];
 #pragma unroll
@@ -397,19 +397,18 @@ __global__ __launch_bounds__(64) void k_fwd_front_w(SolveArgs a, const int32_t *
 }
 
 template <typename T>
-__global__ __launch_bounds__(64) void k_bwd_front_w(SolveArgs a)
+__device__ __forceinline__ void bwd_front_w(const SolveArgs &a, int slot, int mat, int zc, int nz)
 {
-    const int mat = blockIdx.y;
-    const SnDev d = a.sn[a.list[blockIdx.x]];
+    const SnDev d = a.sn[a.list[slot]];
     const int nf = d.nf, np = d.np, nu = d.nu, c1 = d.c1;
     const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
-    const int p = threadIdx.x;
+    const int p = threadIdx.x & 63;
     const int64_t vb = (int64_t)mat * a.nrhs;
     const T *y = reinterpret_cast<const T *>(a.y) + vb * a.n;
     T *x = reinterpret_cast<T *>(a.x) + vb * a.n;
     const int nrhs = (int)a.nrhs;
     const int64_t src = p < np ? c1 + p : (p < nf ? a.upd[d.upd_off + p - np] : 0);
-    for (int r0 = 0; r0 < nrhs; r0 += SOLVE_RC) {
+    for (int r0 = zc * SOLVE_RC; r0 < nrhs; r0 += nz * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
         T v[SOLVE_RC];
 #pragma unroll
@@ -445,6 +444,32 @@ __global__ __launch_bounds__(64) void k_bwd_front_w(SolveArgs a)
             for (int r = 0; r < SOLVE_RC; r++)
                 if (r < rc) x[(int64_t)(r0 + r) * a.n + c1 + p] = v[r];
         }
+    }
+}
+
+// The one-wave sweeps as grid-stride loops: each wave of a 256-thread workgroup takes work
+// items (front, matrix, right-hand-side chunk) in turn, so a level of thousands of tiny fronts
+// times a batch of matrices is a few thousand workgroups, not millions.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_front_w(SolveArgs a, const int32_t *__restrict__ ecolptr,
+                                                     const int2 *__restrict__ ext, int count, int nmat, int nz)
+{
+    const int64_t items = (int64_t)count * nmat * nz;
+    for (int64_t it = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); it < items; it += (int64_t)gridDim.x * 4) {
+        const int slot = (int)(it % count);
+        const int64_t rest = it / count;
+        fwd_front_w<T>(a, ecolptr, ext, slot, (int)(rest % nmat), (int)(rest / nmat), nz);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_front_w(SolveArgs a, int count, int nmat, int nz)
+{
+    const int64_t items = (int64_t)count * nmat * nz;
+    for (int64_t it = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); it < items; it += (int64_t)gridDim.x * 4) {
+        const int slot = (int)(it % count);
+        const int64_t rest = it / count;
+        bwd_front_w<T>(a, slot, (int)(rest % nmat), (int)(rest / nmat), nz);
     }
 }
 
@@ -487,7 +512,7 @@ __global__ __launch_bounds__(256) void k_fwd_front_b(SolveArgs a, const int32_t 
     T *y = reinterpret_cast<T *>(a.y) + vb * a.n;
     T *uv = reinterpret_cast<T *>(a.uv) + vb * a.nupd;
     const int nrhs = (int)a.nrhs;
-    for (int r0 = 0; r0 < nrhs; r0 += SOLVE_RC) {
+    for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
         for (int p = tid; p < nf; p += NT) {
             const int q0 = ecolptr[d.fcol + p], q1 = ecolptr[d.fcol + p + 1];
@@ -589,7 +614,7 @@ __global__ __launch_bounds__(256) void k_bwd_front_b(SolveArgs a)
     int npad = 1;
     while (npad < np && npad < NT) npad <<= 1;
     const int ts_n = NT / npad, ti = tid % npad, ts = tid / npad;
-    for (int r0 = 0; r0 < nrhs; r0 += SOLVE_RC) {
+    for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
         for (int p = tid; p < nf; p += NT) {
             const int64_t src = p < np ? c1 + p : upd[p - np];
@@ -874,7 +899,10 @@ hipError_t launch_solve_level(int which, const SolveArgs &a, const int32_t *list
 {
     SolveArgs b = a;
     b.list = lists + list_off;
-    dim3 g(count, nmat), blk(256);
+    // right-hand-side chunks of SOLVE_RC in grid z (the split-K sums take every chunk at once)
+    const bool sum = which == 2 || which == 4 || which == 7 || which == 9;
+    const int nz = sum ? 1 : (int)((a.nrhs + SOLVE_RC - 1) / SOLVE_RC);
+    dim3 g(count, nmat, nz), blk(256);
     switch (which) {
     case 0: hipLaunchKernelGGL(k_fwd_tri<T>, g, blk, 0, st, b); break;
     case 1: hipLaunchKernelGGL(k_rect_part<T>, g, blk, 0, st, b, 0); break;
@@ -887,9 +915,17 @@ hipError_t launch_solve_level(int which, const SolveArgs &a, const int32_t *list
     case 8: hipLaunchKernelGGL(k_rect_part<T>, g, blk, 0, st, b, 3); break;
     case 9: hipLaunchKernelGGL(k_rect_sum<T>, g, blk, 0, st, b, 3); break;
     case 10: hipLaunchKernelGGL(k_fwd_gather<T>, g, blk, 0, st, b, a.ecolptr, a.ext); break;
-    case 11: hipLaunchKernelGGL(k_fwd_front_w<T>, g, dim3(64), 0, st, b, a.ecolptr, a.ext); break;
+    case 11:
+    case 13: {
+        const int64_t items = (int64_t)count * nmat * nz;
+        const unsigned wg = (unsigned)std::min<int64_t>((items + 3) / 4, 8192);
+        if (which == 11)
+            hipLaunchKernelGGL(k_fwd_front_w<T>, dim3(wg), blk, 0, st, b, a.ecolptr, a.ext, count, nmat, nz);
+        else
+            hipLaunchKernelGGL(k_bwd_front_w<T>, dim3(wg), blk, 0, st, b, count, nmat, nz);
+        break;
+    }
     case 12: hipLaunchKernelGGL(k_fwd_front_b<T>, g, blk, 0, st, b, a.ecolptr, a.ext); break;
-    case 13: hipLaunchKernelGGL(k_bwd_front_w<T>, g, dim3(64), 0, st, b); break;
     case 14: hipLaunchKernelGGL(k_bwd_front_b<T>, g, blk, 0, st, b); break;
     default: return hipErrorInvalidValue;
     }

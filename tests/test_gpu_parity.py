@@ -167,3 +167,16 @@ def test_deterministic():
     F1 = I.splu(A, method="multifrontal", symbolic=S, small_nf=1)
     F2 = I.splu(A, method="multifrontal", symbolic=S, small_nf=1)
     assert np.array_equal(F1.L.data, F2.L.data) and np.array_equal(F1.U.data, F2.U.data)
+
+
+@pytest.mark.parametrize("name,k", [("poisson3d-10", 10), ("poisson3d-24", 24)])
+def test_many_rhs_chunks_match_columns(name, k):
+    """More right-hand sides than one chunk of the solve kernels (chunks run side by side in
+    grid z): every column equals its own single-RHS solve."""
+    A = M.poisson3d(k)
+    F = I.splu(A, method="multifrontal")
+    B = np.random.default_rng(13).standard_normal((A.shape[0], 21))
+    X = I.solve(F, B, refine=1)
+    for j in (0, 7, 8, 15, 16, 20):
+        xj = I.solve(F, B[:, j], refine=1)
+        assert np.abs(X[:, j] - xj).max() <= 1e-14 * np.abs(xj).max()
