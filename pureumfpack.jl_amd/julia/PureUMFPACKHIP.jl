@@ -137,4 +137,37 @@ function hip_solve(P::MFPlan, b::AbstractVecOrMat{Float64}; refine::Integer = 0,
     return b isa AbstractVector ? vec(X) : X
 end
 
+"""
+    hip_refactor_batch!(P, As; scale=SCALE_SUM) -> info
+
+Refactor `length(As)` matrices sharing `P`'s pattern in one batch (`mf_refactor_batch`);
+`P` must come from `MFPlan(A, S; opts=MFOptions(nbatch=...))`.  `info[m]` is the first singular
+column of matrix m (0 if none).
+"""
+function hip_refactor_batch!(P::MFPlan, As::Vector{SparseMatrixCSC{Float64, Int}}; scale::ScaleKind = SCALE_SUM)
+    nmat = length(As)
+    Ax = reduce(hcat, (nonzeros(A) for A in As))             # nnz x nmat
+    Rs = reduce(hcat, (row_scaling(A, scale) for A in As))   # n x nmat
+    info = zeros(Int, nmat)
+    rc = GC.@preserve Ax Rs ccall((:mf_refactor_batch, libmfgpu), Cint,
+        (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Int64}),
+        P.ptr, nmat, Ax, scale == SCALE_NONE ? C_NULL : Rs, info)
+    rc == MF_SINGULAR || check(rc)
+    return info
+end
+
+"""
+    hip_solve_batch(P, B; refine=0, tol=0.0) -> X
+
+`B` is `n x nrhs x nmat`; matrix m of the last batch solves `B[:, :, m]` (`mf_solve_batch`).
+"""
+function hip_solve_batch(P::MFPlan, B::Array{Float64, 3}; refine::Integer = 0, tol::Real = 0.0)
+    X = similar(B)
+    rep = Ref{MFSolveReport}()
+    check(ccall((:mf_solve_batch, libmfgpu), Cint,
+        (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}, Ptr{Float64}, Int32, Float64, Ref{MFSolveReport}),
+        P.ptr, size(B, 3), size(B, 2), B, X, refine, tol, rep))
+    return X
+end
+
 end # module
