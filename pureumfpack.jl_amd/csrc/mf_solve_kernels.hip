@@ -310,11 +310,11 @@ __global__ __launch_bounds__(256) void k_bwd_tri(SolveArgs a)
                 for (int r = 0; r < SOLVE_RC; r++)
                     v[r] = (r < rc && lane < bs) ? x[(r0 + r) * a.n + c1 + b0 + lane] : T(0);
                 for (int j = bs - 1; j >= 0; j--) {
-                    const T ujj = sdiag[j + j * (SOLVE_NB + 1)];
+                    const T rjj = T(1) / sdiag[j + j * (SOLVE_NB + 1)];   // one division per step
                     const T u = (lane < j) ? sdiag[lane + j * (SOLVE_NB + 1)] : T(0);
 #pragma unroll
                     for (int r = 0; r < SOLVE_RC; r++) {
-                        if (lane == j) v[r] = v[r] / ujj;
+                        if (lane == j) v[r] = v[r] * rjj;
                         v[r] -= u * __shfl(v[r], j);
                     }
                 }
@@ -431,11 +431,11 @@ __device__ __forceinline__ void bwd_front_w(const SolveArgs &a, int slot, int ma
         for (int jj = 0; jj < 64; jj++) {
             const int j = np - 1 - jj;
             if (j < 0) break;
-            const T ujj = F[j + (int64_t)j * nf];
+            const T rjj = T(1) / F[j + (int64_t)j * nf];   // one division per step
             const T u = p < j ? col[(int64_t)j * nf] : T(0);
 #pragma unroll
             for (int r = 0; r < SOLVE_RC; r++) {
-                if (p == j) v[r] = v[r] / ujj;
+                if (p == j) v[r] = v[r] * rjj;
                 v[r] -= u * __shfl(v[r], j);
             }
         }
@@ -670,11 +670,11 @@ __global__ __launch_bounds__(256) void k_bwd_front_b(SolveArgs a)
 #pragma unroll
                 for (int r = 0; r < SOLVE_RC; r++) v[r] = lane < bs ? g[r * MAXNF + b0 + lane] : T(0);
                 for (int j = bs - 1; j >= 0; j--) {
-                    const T ujj = sd[j + j * LDD];
+                    const T rjj = T(1) / sd[j + j * LDD];   // one division per step
                     const T u = lane < j ? sd[lane + j * LDD] : T(0);
 #pragma unroll
                     for (int r = 0; r < SOLVE_RC; r++) {
-                        if (lane == j) v[r] = v[r] / ujj;
+                        if (lane == j) v[r] = v[r] * rjj;
                         v[r] -= u * __shfl(v[r], j);
                     }
                 }
