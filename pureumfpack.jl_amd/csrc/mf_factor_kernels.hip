@@ -479,68 +479,66 @@ __global__ __launch_bounds__(64) void k_spec_diag(FactorArgs a, int k, float tau
     if (lane == 0) S[2 * KB * KB + KB] = special ? T(1) : T(0);
 }
 
-// Right-looking substitution on one row/column x held in registers (see k_spec_trsm).
-// LROWS: kind 0 (scale by the reciprocal pivots, check multipliers of candidate rows).
-// M and rc are staged in LDS (the same address for the whole wave: broadcast reads).
-template <typename T, int KB, bool LROWS>
-__device__ __forceinline__ int spec_solve_row(T (&x)[KB], const T *M, const T *rc, bool cand, T itau)
+// front entries (idx, k + c) / (k + c, idx) of matrix mat: base pointers of a row / column
+template <typename T>
+__device__ __forceinline__ T *F_row_base(const FactorArgs &a, int mat, const SnDev &d, int idx, int k)
+{
+    return reinterpret_cast<T *>(a.pool) + mat * a.pool_stride + d.foff + idx + (int64_t)k * d.nf;
+}
+template <typename T>
+__device__ __forceinline__ T *F_col_base(const FactorArgs &a, int mat, const SnDev &d, int idx, int k)
+{
+    return reinterpret_cast<T *>(a.pool) + mat * a.pool_stride + d.foff + (int64_t)idx * d.nf + k;
+}
+
+// Right-looking substitution on one row (kind 0) or column (kind 1) of the block, shared by a
+// group of 4 lanes: lane g of the group holds the entries c = 4 t + g (t = 0..15) in x[t].  Step
+// j: the owner lane of column j finishes x_j (kind 0: multiplier check and scaling), shuffles
+// it to its group, and every lane updates its entries c > j with M[j][c] from LDS.  Each x_c
+// receives its updates in the order j = 0, 1, ..., as in the row-by-row dot-product form.
+// SPECIAL: some pivot of the block is zero or below the smallest normal number, where getf2
+// leaves the column unscaled or divides instead of scaling by the reciprocal.
+template <typename T, int KB, bool LROWS, bool SPECIAL>
+__device__ __forceinline__ int spec_solve_group(T (&x)[KB / 4], const T *M, const T *rc, int g, int gbase,
+                                                bool cand, T itau)
 {
     int bad = 0;
 #pragma unroll
     for (int jc = 0; jc < KB / 32; jc++)
 #pragma unroll
     for (int jj = 0; jj < 32; jj++) {
-        const int j = 32 * jc + jj;
-        T v = x[j];
+        const int j = 32 * jc + jj, owner = j & 3, tj = j >> 2;
+        T v = x[tj];
         if (LROWS) {
-            bad |= (int)(cand && fabs(v) > fabs(M[j * KB + j]) * itau);
-            v = v * rc[j];                                // reciprocal scaling (getf2, trsm R/U/N/N)
-            x[j] = v;
+            const T u = M[j * KB + j];
+            bad |= (int)(g == owner && cand && fabs(v) > fabs(u) * itau);
+            if (SPECIAL) {
+                if (u == T(0)) v = cand ? v : v * rc[j];
+                else if (fabs(u) < Tiny<T>::v()) v = cand ? v / u : v * rc[j];
+                else v = v * rc[j];
+            } else {
+                v = v * rc[j];   // reciprocal scaling (getf2, trsm R/U/N/N)
+            }
+            if (g == owner) x[tj] = v;
         }
+        v = __shfl(v, gbase + owner);
 #pragma unroll
-        for (int c = 0; c < KB; c++)
-            if (c > j) x[c] -= v * M[j * KB + c];
-    }
-    return bad;
-}
-
-// The same substitution for a block holding a zero or tiny pivot, in dot-product form
-// (x_j -= x_q M[q][j] for q = 0..j-1: the same operation order), where getf2 leaves a
-// zero-pivot column unscaled and divides by a tiny pivot instead of scaling by its reciprocal.
-template <typename T, int KB>
-__device__ __forceinline__ int spec_solve_row_special(T (&x)[KB], const T *M, const T *rc, bool cand, T itau)
-{
-    int bad = 0;
-#pragma unroll
-    for (int jc = 0; jc < KB / 32; jc++)
-#pragma unroll
-    for (int jj = 0; jj < 32; jj++) {
-        const int j = 32 * jc + jj;
-        T v = x[j];
-#pragma unroll
-        for (int q = 0; q < 32 * (jc + 1); q++)
-            if (q < j) v -= x[q] * M[q * KB + j];
-        const T u = M[j * KB + j];
-        if (cand) {
-            if (fabs(v) > fabs(u) * itau) bad = 1;
-            if (u != T(0)) v = fabs(u) < Tiny<T>::v() ? v / u : v * rc[j];
-        } else {
-            v = v * rc[j];
-        }
-        x[j] = v;
+        for (int t = 0; t < KB / 4; t++)
+            if (4 * t + 3 > j) {
+                const int c = 4 * t + g;
+                if (c > j) x[t] -= v * M[j * KB + c];
+            }
     }
     return bad;
 }
 
 template <typename T, int KB>
-__global__ __launch_bounds__(SPEC_ROWS) void k_spec_trsm(FactorArgs a, int k, float tau,
-                                                         const T *__restrict__ spec)
+__global__ __launch_bounds__(4 * SPEC_ROWS) void k_spec_trsm(FactorArgs a, int k, float tau,
+                                                             const T *__restrict__ spec)
 {
-    // Right-looking substitution, one row (kind 0) or column (kind 1) per thread:
-    //   for j: finish x_j, then x_c -= x_j M[j][c] for all c > j,
-    // with M[j][c] = U(j, c) for kind 0 and L(c, j) for kind 1, staged in LDS from the packed
-    // block that k_spec_diag left in this front's slot.  Every x_c receives its updates in the order
-    // j = 0, 1, ..., the same sequence of operations as the row-by-row dot-product form.
+    // SPEC_ROWS rows (kind 0) or columns (kind 1) per workgroup, 4 lanes each (see
+    // spec_solve_group), M[j][c] = U(j, c) for kind 0 and L(c, j) for kind 1 staged in LDS
+    // from the packed block k_spec_diag left in this front's slot.
     __shared__ T sm[KB * KB + 2 * KB];
     const int mat = blockIdx.y;
     const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
@@ -551,46 +549,40 @@ __global__ __launch_bounds__(SPEC_ROWS) void k_spec_trsm(FactorArgs a, int k, fl
     {
         const T *S = spec + mat * a.spec_stride + (int64_t)w.w * SPEC_SLOT;
         const T *Mg = S + (lrows ? 0 : KB * KB);
-        for (int e = threadIdx.x; e < KB * KB; e += SPEC_ROWS) sm[e] = Mg[e];
-        for (int e = threadIdx.x; e < 2 * KB; e += SPEC_ROWS) sm[KB * KB + e] = S[2 * KB * KB + e];
+        for (int e = threadIdx.x; e < KB * KB; e += 4 * SPEC_ROWS) sm[e] = Mg[e];
+        for (int e = threadIdx.x; e < 2 * KB; e += 4 * SPEC_ROWS) sm[KB * KB + e] = S[2 * KB * KB + e];
     }
     __syncthreads();
     const T *rc = sm + KB * KB;
-    const int idx = k + kb + w.z + threadIdx.x;   // row (kind 0) or column (kind 1) in the front
-    if (idx >= nf) return;
-    T *F = reinterpret_cast<T *>(a.pool) + mat * a.pool_stride + d.foff;
+    const int g = threadIdx.x & 3, gbase = threadIdx.x & 60;   // lane of the group, group's first lane in the wave
+    const int idx = k + kb + w.z + (threadIdx.x >> 2);        // row (kind 0) or column (kind 1) in the front
+    const bool on = idx < nf;
     const bool cand = lrows && idx < np;
     const T itau = T(1) / T(tau);
     // kind 0 reads row idx across the block columns (stride nf); kind 1 reads column idx
     // down the block rows (contiguous)
-    T *base = lrows ? F + idx + (int64_t)k * nf : F + (int64_t)idx * nf + k;
+    T *base = lrows ? F_row_base<T>(a, mat, d, idx, k) : F_col_base<T>(a, mat, d, idx, k);
     const int64_t stride = lrows ? nf : 1;
-    T x[KB];
-    if (kb == KB) {   // full block: unconditional loads off one running pointer
-        const T *p = base;
+    T x[KB / 4];
 #pragma unroll
-        for (int j = 0; j < KB; j++, p += stride) x[j] = *p;
-    } else {
-#pragma unroll
-        for (int j = 0; j < KB; j++) x[j] = j < kb ? base[j * stride] : T(0);
+    for (int t = 0; t < KB / 4; t++) {
+        const int c = 4 * t + g;
+        x[t] = (on && c < kb) ? base[c * stride] : T(0);
     }
     int bad = 0;
     if (!lrows)
-        spec_solve_row<T, KB, false>(x, sm, rc, false, itau);
+        spec_solve_group<T, KB, false, false>(x, sm, rc, g, gbase, false, itau);
     else if (rc[KB] == T(0))
-        bad = spec_solve_row<T, KB, true>(x, sm, rc, cand, itau);
+        bad = spec_solve_group<T, KB, true, false>(x, sm, rc, g, gbase, cand, itau);
     else
-        bad = spec_solve_row_special<T, KB>(x, sm, rc, cand, itau);
+        bad = spec_solve_group<T, KB, true, true>(x, sm, rc, g, gbase, cand, itau);
     if (bad) atomicOr(a.flag + mat, 1);
-    T *p = base;
-    asm volatile("" : "+v"(p));  // a fresh pointer, so the load addresses are not kept live
-    if (kb == KB) {
+    if (on) {
 #pragma unroll
-        for (int j = 0; j < KB; j++, p += stride) *p = x[j];
-    } else {
-#pragma unroll
-        for (int j = 0; j < KB; j++)
-            if (j < kb) p[j * stride] = x[j];
+        for (int t = 0; t < KB / 4; t++) {
+            const int c = 4 * t + g;
+            if (c < kb) base[c * stride] = x[t];
+        }
     }
 }
 
@@ -997,7 +989,7 @@ hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int
         hipLaunchKernelGGL((k_spec_diag<T, SPEC_KB>), grid, dim3(64), 0, st, a, L.k, L.tau);
         break;
     case L_SPEC_TRSM:
-        hipLaunchKernelGGL((k_spec_trsm<T, SPEC_KB>), grid, dim3(SPEC_ROWS), 0, st, a, L.k, L.tau,
+        hipLaunchKernelGGL((k_spec_trsm<T, SPEC_KB>), grid, dim3(4 * SPEC_ROWS), 0, st, a, L.k, L.tau,
                            reinterpret_cast<const T *>(a.spec));
         break;
     case L_GEMM:
