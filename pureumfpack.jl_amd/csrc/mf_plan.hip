@@ -118,6 +118,7 @@ struct mf_plan {
     // fronts swept by the fused per-front kernels (no stored inverse): nf <= 64 and
     // 64 < nf <= FRONT_MAXNF; the level lists above hold only the other fronts
     std::vector<int32_t> lvl_s64o, lvl_s64c, lvl_s1ko, lvl_s1kc;
+    std::vector<int32_t> lvl_s1k_small;   // every such front of the level has nf <= 256
     int64_t front_elems = 0, inv_elems = 0, tmp_elems = 0;  // pool = fronts | inverses | gemm temps
     int64_t part_slots = 0;  // partial-sum slots (256 rows x nrhs each) the solve needs
     std::vector<double> lvl_lflops, lvl_uflops, lvl_lbytes, lvl_ubytes, lvl_vec;
@@ -860,8 +861,13 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             if (ioff[s] < 0 && nf[s] <= 64) slist.push_back((int32_t)s);
         P.lvl_s64c.push_back((int32_t)slist.size() - P.lvl_s64o.back());
         P.lvl_s1ko.push_back((int32_t)slist.size());
+        int64_t mnf1k = 0;
         for (int64_t s : bylevel[lv])
-            if (ioff[s] < 0 && nf[s] > 64 && nf[s] <= FRONT_MAXNF_HOST) slist.push_back((int32_t)s);
+            if (ioff[s] < 0 && nf[s] > 64 && nf[s] <= FRONT_MAXNF_HOST) {
+                slist.push_back((int32_t)s);
+                mnf1k = std::max(mnf1k, nf[s]);
+            }
+        P.lvl_s1k_small.push_back(mnf1k <= 256 ? 1 : 0);
         P.lvl_s1kc.push_back((int32_t)slist.size() - P.lvl_s1ko.back());
         for (int64_t s : bylevel[lv])
             if (ioff[s] >= 0 || nf[s] > FRONT_MAXNF_HOST) big.push_back(s);
@@ -1257,7 +1263,8 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
             }));
         if (P.lvl_s1kc[lv] > 0)
             HIPCHK(profiled(P, 5, 0.0, 0.0, [&] {
-                return launch_solve_level<T>(12, a, sl, P.lvl_s1ko[lv], P.lvl_s1kc[lv], nmat, P.stream);
+                return launch_solve_level<T>(P.lvl_s1k_small[lv] ? 15 : 12, a, sl, P.lvl_s1ko[lv], P.lvl_s1kc[lv], nmat,
+                                             P.stream);
             }));
         if (P.lvl_cnt[lv] == 0) continue;
         HIPCHK(profiled(P, 5, 0.0, vb * nmat, [&] {
@@ -1291,7 +1298,8 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
             }));
         if (P.lvl_s1kc[lv] > 0)
             HIPCHK(profiled(P, 6, 0.0, 0.0, [&] {
-                return launch_solve_level<T>(14, a, sl, P.lvl_s1ko[lv], P.lvl_s1kc[lv], nmat, P.stream);
+                return launch_solve_level<T>(P.lvl_s1k_small[lv] ? 16 : 14, a, sl, P.lvl_s1ko[lv], P.lvl_s1kc[lv], nmat,
+                                             P.stream);
             }));
         if (P.lvl_cnt[lv] == 0) continue;
         if (P.lvl_bpc[lv] > 0)

@@ -494,11 +494,11 @@ __device__ __forceinline__ void dot_rows16(const T *M, int64_t ld, int bs, const
     }
 }
 
-template <typename T>
+template <typename T, int MAXNF>
 __global__ __launch_bounds__(256) void k_fwd_front_b(SolveArgs a, const int32_t *__restrict__ ecolptr,
                                                      const int2 *__restrict__ ext)
 {
-    constexpr int NT = 256, MAXNF = FRONT_MAXNF, PT = MAXNF / NT, LDD = SOLVE_NB + 1;
+    constexpr int NT = 256, PT = MAXNF / NT, LDD = SOLVE_NB + 1;
     __shared__ T g[SOLVE_RC * MAXNF];            // g[r * MAXNF + p]
     __shared__ T sd[SOLVE_NB * LDD];             // diagonal block of L11
     const int mat = blockIdx.y;
@@ -592,13 +592,15 @@ __global__ __launch_bounds__(256) void k_fwd_front_b(SolveArgs a, const int32_t 
     }
 }
 
-template <typename T>
+template <typename T, int MAXNF>
 __global__ __launch_bounds__(256) void k_bwd_front_b(SolveArgs a)
 {
-    constexpr int NT = 256, MAXNF = FRONT_MAXNF, LDD = SOLVE_NB + 1;
+    constexpr int NT = 256, LDD = SOLVE_NB + 1;
+    constexpr int SDN = SOLVE_NB * LDD > NT * SOLVE_RC ? SOLVE_NB * LDD : NT * SOLVE_RC;
     __shared__ T g[SOLVE_RC * MAXNF];            // s at [0, np), x of the update rows at np + t
-    __shared__ T sd[SOLVE_NB * LDD];             // diagonal block of U11
-    __shared__ T red[NT * SOLVE_RC];             // partial sums of the U12 product
+    __shared__ T sdred[SDN];                     // U12 product partial sums, then diagonal blocks of U11
+    T *const sd = sdred;
+    T *const red = sdred;
     const int mat = blockIdx.y;
     const SnDev d = a.sn[a.list[blockIdx.x]];
     const int nf = d.nf, np = d.np, nu = d.nu, c1 = d.c1;
@@ -925,8 +927,10 @@ hipError_t launch_solve_level(int which, const SolveArgs &a, const int32_t *list
             hipLaunchKernelGGL(k_bwd_front_w<T>, dim3(wg), blk, 0, st, b, count, nmat, nz);
         break;
     }
-    case 12: hipLaunchKernelGGL(k_fwd_front_b<T>, g, blk, 0, st, b, a.ecolptr, a.ext); break;
-    case 14: hipLaunchKernelGGL(k_bwd_front_b<T>, g, blk, 0, st, b); break;
+    case 12: hipLaunchKernelGGL((k_fwd_front_b<T, FRONT_MAXNF>), g, blk, 0, st, b, a.ecolptr, a.ext); break;
+    case 14: hipLaunchKernelGGL((k_bwd_front_b<T, FRONT_MAXNF>), g, blk, 0, st, b); break;
+    case 15: hipLaunchKernelGGL((k_fwd_front_b<T, 256>), g, blk, 0, st, b, a.ecolptr, a.ext); break;
+    case 16: hipLaunchKernelGGL((k_bwd_front_b<T, 256>), g, blk, 0, st, b); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
