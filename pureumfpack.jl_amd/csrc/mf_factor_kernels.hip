@@ -405,6 +405,47 @@ __device__ __forceinline__ float wave_uniform(float v)
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
+// The KB steps of the diagonal block factorization (see k_spec_diag).  FAST: every pivot is
+// assumed normal (|u_jj| >= smallest normal): scaling by the reciprocal, branch-free, and
+// `special` records whether the assumption failed.  Otherwise getf2's rules per step: a zero
+// pivot leaves its column unscaled (and is reported through info), a tiny one divides.
+template <typename T, int KB, bool FAST>
+__device__ __forceinline__ void spec_diag_steps(T (&x)[KB], int lane, int kb, T itau, int &bad, int &special,
+                                                T &myrcp, int32_t *info, int infoc)
+{
+#pragma unroll
+    for (int jc = 0; jc < KB / 32; jc++)
+#pragma unroll
+    for (int jj = 0; jj < 32; jj++) {
+        const int j = 32 * jc + jj;
+        if (!FAST && j >= kb) continue;
+        const T piv = lane_value(x[j], j);
+        const bool act = lane > j && (FAST || lane < kb);
+        T v = x[j];
+        bad |= (int)(act && fabs(v) > fabs(piv) * itau);
+        const T rp = T(1) / piv;
+        if (FAST) {
+            special |= (int)!(fabs(piv) >= Tiny<T>::v());
+            v = v * rp;
+        } else if (fabs(piv) >= Tiny<T>::v()) {
+            v = v * rp;
+        } else {
+            special = 1;
+            if (piv == T(0)) {
+                if (lane == 0) atomicMin(info, infoc + j);
+            } else {
+                v = v / piv;
+            }
+        }
+        myrcp = lane == j ? rp : myrcp;
+        const T l = act ? v : T(0);
+        x[j] = act ? v : x[j];
+#pragma unroll
+        for (int c = 0; c < KB; c++)
+            if (c > j) x[c] -= l * lane_value(x[c], j);
+    }
+}
+
 template <typename T, int KB>
 __global__ __launch_bounds__(64) void k_spec_diag(FactorArgs a, int k, float tau)
 {
@@ -428,33 +469,20 @@ __global__ __launch_bounds__(64) void k_spec_diag(FactorArgs a, int k, float tau
     const T itau = T(1) / T(tau);
     int bad = 0, special = 0;
     T myrcp = T(1);
+    if (kb == KB) {
+        // full block, first assuming normal pivots: no branch in the steps, so the compiler
+        // can overlap a step's reciprocal with the previous step's updates
+        spec_diag_steps<T, KB, true>(x, lane, kb, itau, bad, special, myrcp, nullptr, 0);
+        if (special) {
+            // a zero or tiny pivot: redo the block from its input with getf2's rules
 #pragma unroll
-    for (int jc = 0; jc < KB / 32; jc++)
-#pragma unroll
-    for (int jj = 0; jj < 32; jj++) {
-        const int j = 32 * jc + jj;
-        if (j >= kb) continue;
-        const T piv = lane_value(x[j], j);
-        const bool act = lane > j && lane < kb;
-        T v = x[j];
-        if (act && fabs(v) > fabs(piv) * itau) bad = 1;
-        const T rp = T(1) / piv;
-        if (fabs(piv) >= Tiny<T>::v()) {
-            v = v * rp;
-        } else {
-            special = 1;
-            if (piv == T(0)) {
-                if (lane == 0) atomicMin(a.info + mat, d.c1 + k + j);
-            } else {
-                v = v / piv;
-            }
+            for (int c = 0; c < KB; c++) x[c] = F[(k + lane) + (int64_t)(k + c) * nf];
+            bad = 0;
+            myrcp = T(1);
+            spec_diag_steps<T, KB, false>(x, lane, kb, itau, bad, special, myrcp, a.info + mat, d.c1 + k);
         }
-        myrcp = lane == j ? rp : myrcp;
-        const T l = act ? v : T(0);
-        x[j] = act ? v : x[j];
-#pragma unroll
-        for (int c = 0; c < KB; c++)
-            if (c > j) x[c] -= l * lane_value(x[c], j);
+    } else {
+        spec_diag_steps<T, KB, false>(x, lane, kb, itau, bad, special, myrcp, a.info + mat, d.c1 + k);
     }
     if (bad) atomicOr(a.flag + mat, 1);
     if (lane < kb) {
