@@ -650,13 +650,17 @@ __global__ __launch_bounds__(256) void k_bwd_front_b(SolveArgs a)
 #pragma unroll
             for (int r = 0; r < SOLVE_RC; r++) red[r * NT + tid] = acc[r];
             __syncthreads();
+            // fixed binary tree over the ts_n partial sums of each row (log2(ts_n) steps)
+            for (int half = ts_n >> 1; half > 0; half >>= 1) {
+                if (ts < half) {
+#pragma unroll
+                    for (int r = 0; r < SOLVE_RC; r++) red[r * NT + tid] += red[r * NT + tid + half * npad];
+                }
+                __syncthreads();
+            }
             if (tid < npad && i0 + tid < np) {
 #pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++) {
-                    T s = T(0);
-                    for (int q = 0; q < ts_n; q++) s += red[r * NT + q * npad + tid];
-                    g[r * MAXNF + i0 + tid] -= s;
-                }
+                for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + i0 + tid] -= red[r * NT + tid];
             }
             __syncthreads();
         }
