@@ -378,12 +378,18 @@ __device__ __forceinline__ void fwd_front_w(const SolveArgs &a, const int32_t *_
 #pragma unroll
         for (int r = 0; r < SOLVE_RC; r++) v[r] = __shfl(v[r], src);
         const T *row = F + p;
-#pragma unroll 16
-        for (int j = 0; j < 64; j++) {
-            if (j >= np) break;
-            const T l = (on && p > j) ? row[(int64_t)j * nf] : T(0);
+        // columns of [L11; L21] in groups of 8: the 8 loads are issued before the 8 steps
+        for (int j0 = 0; j0 < np; j0 += 8) {
+            T lv[8];
 #pragma unroll
-            for (int r = 0; r < SOLVE_RC; r++) v[r] -= l * __shfl(v[r], j);
+            for (int q = 0; q < 8; q++) lv[q] = (on && p > j0 + q && j0 + q < np) ? row[(int64_t)(j0 + q) * nf] : T(0);
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int j = j0 + q;
+                if (j >= np) break;
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) v[r] -= lv[q] * __shfl(v[r], j);
+            }
         }
         if (on) {
 #pragma unroll
@@ -417,26 +423,39 @@ __device__ __forceinline__ void bwd_front_w(const SolveArgs &a, int slot, int ma
         // s_i -= sum_t U12[i, t] x_upd[t] (the update values sit in lanes np + t)
         T acc[SOLVE_RC] = {};
         const T *urow = F + p + (int64_t)np * nf;
-#pragma unroll 16
-        for (int t = 0; t < 64; t++) {
-            if (t >= nu) break;
-            const T u = p < np ? urow[(int64_t)t * nf] : T(0);
+        for (int t0 = 0; t0 < nu; t0 += 8) {
+            T uu[8];
 #pragma unroll
-            for (int r = 0; r < SOLVE_RC; r++) acc[r] += u * __shfl(v[r], np + t);
+            for (int q = 0; q < 8; q++) uu[q] = (p < np && t0 + q < nu) ? urow[(int64_t)(t0 + q) * nf] : T(0);
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int t = t0 + q;
+                if (t >= nu) break;
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) acc[r] += uu[q] * __shfl(v[r], np + t);
+            }
         }
 #pragma unroll
         for (int r = 0; r < SOLVE_RC; r++) v[r] -= acc[r];
         const T *col = F + p;
-#pragma unroll 16
-        for (int jj = 0; jj < 64; jj++) {
-            const int j = np - 1 - jj;
-            if (j < 0) break;
-            const T rjj = T(1) / F[j + (int64_t)j * nf];   // one division per step
-            const T u = p < j ? col[(int64_t)j * nf] : T(0);
+        for (int j1 = np - 1; j1 >= 0; j1 -= 8) {
+            T uc[8], dg[8];
 #pragma unroll
-            for (int r = 0; r < SOLVE_RC; r++) {
-                if (p == j) v[r] = v[r] * rjj;
-                v[r] -= u * __shfl(v[r], j);
+            for (int q = 0; q < 8; q++) {
+                const int j = j1 - q;
+                uc[q] = (j >= 0 && p < j) ? col[(int64_t)j * nf] : T(0);
+                dg[q] = j >= 0 ? F[j + (int64_t)j * nf] : T(1);
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int j = j1 - q;
+                if (j < 0) break;
+                const T rjj = T(1) / dg[q];   // one division per step
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) {
+                    if (p == j) v[r] = v[r] * rjj;
+                    v[r] -= uc[q] * __shfl(v[r], j);
+                }
             }
         }
         if (p < np) {
