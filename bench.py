@@ -193,6 +193,24 @@ def run_batch(hip, k, nmat, nrhs, precision, steps, warmup, tol, max_refine, gra
             "launches_per_factor": st["launches"]}
 
 
+def run_convdiff(hip, k, steps, warmup):
+    """Unsymmetric 3D convection-diffusion (upwind, velocity (1, 0.5, 0.25), cell Peclet 5),
+    COLAMD ordering, BF16 Schur complements with per-tile power-of-two scaling, FP32
+    getrf/trsm, FP64 refinement to backward error <= 1e-12; 1 RHS randn(5)."""
+    A = M.convdiff3d(k)
+    t0 = time.perf_counter()
+    S = I.symbolic_mf(A, ordering="colamd")
+    t_an = time.perf_counter() - t0
+    Rs = I.row_scaling(A, I.SCALE_SUM)
+    B = np.random.default_rng(5).standard_normal((A.shape[0], 1))
+    res, _ = run_gpu(hip, A, S, Rs, B, "bf16", steps, warmup, 1e-12, 60, graph=True)
+    res.pop("profile", None)
+    res.update({"workload": f"convdiff3d-{k} (n={A.shape[0]}), COLAMD, SCALE_SUM, 1 RHS, bf16 Schur + FP32 "
+                            f"getrf/trsm, refine to bwd err <= 1e-12", "analysis_s": t_an,
+                "nnz_LU": int(S.predicted_fill()), "nsuper": int(S.nsuper)})
+    return res
+
+
 def cpu_baseline(A, S, Rs, B, refine):
     """The oracle's C restatement of the reference multifrontal path (FP64, dense fronts through
     OpenBLAS), timed on the same matrix: numeric factor + solve(F, b; refine) per RHS column.
@@ -248,6 +266,9 @@ def main():
     ap.add_argument("--paths", default="fp64,fp32")
     ap.add_argument("--graph", type=int, default=1, help="replay the factor schedule as a hipGraph")
     ap.add_argument("--batch", default="512,16,64", help="nmat,k,nrhs of the same-pattern batch run ('' = skip)")
+    ap.add_argument("--convdiff", type=int, default=32,
+                    help="grid of the conv-diff COLAMD + BF16 run (0 = skip; 64 does not fit one GPU's HBM "
+                         "with every front resident)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration) and exit")
     args = ap.parse_args()
@@ -303,6 +324,11 @@ def main():
                                    graph=bool(args.graph))
         print(f"# batched: {out['batched']['seconds']:.4f} s/step, {out['batched']['matrices_per_s']:.0f} matrices/s, "
               f"bwd {out['batched']['backward_error_host_check']:.2e}", file=sys.stderr, flush=True)
+    if args.convdiff:
+        out["convdiff_colamd_bf16"] = run_convdiff(hip, args.convdiff, args.steps, args.warmup)
+        cd = out["convdiff_colamd_bf16"]
+        print(f"# convdiff {args.convdiff}^3 colamd bf16: {cd['seconds']:.4f} s/step, bwd {cd['backward_error']:.2e}, "
+              f"{cd['refine_iterations']} refine", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A, S, Rs, B, refine=1)
     print(json.dumps(out))

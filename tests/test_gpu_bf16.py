@@ -29,3 +29,15 @@ def test_bf16_refines_to_backward_error(name, make):
     G = I.splu(A, method="multifrontal", symbolic=S, precision="fp32", small_nf=1)
     d = np.abs(F.U.data - G.U.data).max() / np.abs(G.U.data).max()
     assert 1e-6 < d < 5e-2
+
+
+def test_bf16_colamd_convdiff_to_1e12():
+    """The conv-diff workload shape of the BF16 configuration at a small grid: COLAMD ordering,
+    BF16 Schur complements, FP32 getrf/trsm, FP64 refinement to backward error <= 1e-12."""
+    A = M.convdiff3d(14)
+    S = I.symbolic_mf(A, ordering="colamd")
+    b = np.random.default_rng(5).standard_normal(A.shape[0])
+    F = I.splu(A, method="multifrontal", symbolic=S, precision="bf16", small_nf=1)
+    x = I.solve(F, b, refine=60, tol=1e-12)
+    assert F.last_report["converged"] == 1
+    assert _bwd(A, x, b) <= 1e-12
