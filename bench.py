@@ -263,7 +263,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-14)
     ap.add_argument("--max-refine", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--paths", default="fp64,fp32")
+    ap.add_argument("--paths", default="fp64,fp32,bf16")
     ap.add_argument("--graph", type=int, default=1, help="replay the factor schedule as a hipGraph")
     ap.add_argument("--batch", default="512,16,64", help="nmat,k,nrhs of the same-pattern batch run ('' = skip)")
     ap.add_argument("--convdiff", type=int, default=32,
