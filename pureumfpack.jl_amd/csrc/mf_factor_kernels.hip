@@ -447,7 +447,7 @@ __device__ __forceinline__ void spec_diag_steps(T (&x)[KB], int lane, int kb, T 
 }
 
 template <typename T, int KB>
-__global__ __launch_bounds__(64) void k_spec_diag(FactorArgs a, int k, float tau)
+__global__ __launch_bounds__(KB) void k_spec_diag(FactorArgs a, int k, float tau)
 {
     // One wave: lane i owns row i of the block in registers; step j reads the pivot row from
     // lane j with v_readlane (no memory round trip).  Control flow depends only on the pivot,
@@ -1014,7 +1014,7 @@ hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int
         hipLaunchKernelGGL(k_inv_leaf<T>, grid, dim3(128), 0, st, a);
         break;
     case L_SPEC_DIAG:
-        hipLaunchKernelGGL((k_spec_diag<T, SPEC_KB>), grid, dim3(64), 0, st, a, L.k, L.tau);
+        hipLaunchKernelGGL((k_spec_diag<T, SPEC_KB>), grid, dim3(SPEC_KB), 0, st, a, L.k, L.tau);
         break;
     case L_SPEC_TRSM:
         hipLaunchKernelGGL((k_spec_trsm<T, SPEC_KB>), grid, dim3(4 * SPEC_ROWS), 0, st, a, L.k, L.tau,
