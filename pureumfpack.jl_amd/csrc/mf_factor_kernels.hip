@@ -447,7 +447,7 @@ __device__ __forceinline__ void spec_diag_steps(T (&x)[KB], int lane, int kb, T 
 }
 
 template <typename T, int KB>
-__global__ __launch_bounds__(KB) void k_spec_diag(FactorArgs a, int k, float tau)
+__device__ __forceinline__ void spec_diag_front(const FactorArgs &a, int k, float tau, int slot, T *sblk)
 {
     // One wave: lane i owns row i of the block in registers; step j reads the pivot row from
     // lane j with v_readlane (no memory round trip).  Control flow depends only on the pivot,
@@ -455,14 +455,13 @@ __global__ __launch_bounds__(KB) void k_spec_diag(FactorArgs a, int k, float tau
     // multiplier 0.  The factored block is also written, packed for the trsm, to this front's
     // slot of a.spec: U by rows, L by columns, the reciprocal pivots and a flag for a zero or
     // tiny pivot (which getf2 treats apart: no scaling, or a division).
-    __shared__ T sblk[KB * (KB + 1)];
     const int mat = blockIdx.y;
-    const SnDev d = a.sn[a.list[blockIdx.x]];
+    const SnDev d = a.sn[a.list[slot]];
     const int np = d.np, nf = d.nf;
     const int kb = min(KB, np - k);
     const int lane = threadIdx.x;
     T *F = reinterpret_cast<T *>(a.pool) + mat * a.pool_stride + d.foff;
-    T *S = reinterpret_cast<T *>(a.spec) + mat * a.spec_stride + (int64_t)blockIdx.x * SPEC_SLOT;
+    T *S = reinterpret_cast<T *>(a.spec) + mat * a.spec_stride + (int64_t)slot * SPEC_SLOT;
     T x[KB];
 #pragma unroll
     for (int c = 0; c < KB; c++) x[c] = (lane < kb && c < kb) ? F[(k + lane) + (int64_t)(k + c) * nf] : T(0);
@@ -506,6 +505,14 @@ __global__ __launch_bounds__(KB) void k_spec_diag(FactorArgs a, int k, float tau
     S[2 * KB * KB + lane] = lane < kb ? myrcp : T(1);
     if (lane == 0) S[2 * KB * KB + KB] = special ? T(1) : T(0);
 }
+
+template <typename T, int KB>
+__global__ __launch_bounds__(KB) void k_spec_diag(FactorArgs a, int k, float tau)
+{
+    __shared__ T sblk[KB * (KB + 1)];
+    spec_diag_front<T, KB>(a, k, tau, blockIdx.x, sblk);
+}
+
 
 // front entries (idx, k + c) / (k + c, idx) of matrix mat: base pointers of a row / column
 template <typename T>
@@ -719,18 +726,20 @@ template <> struct Mfma<float> {
 // stored to the other buffer, one barrier per slice.  The MFMA computes B^T A^T, so each lane
 // ends up with C elements of 16 consecutive rows: the read-modify-write of C is coalesced.
 // Every element of C is accumulated over k in the same order for every tile shape.
-template <typename T, int TM, int TN, int USE>
-__global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs,
-                                              const GemmTile *__restrict__ tiles, void *pool_v,
-                                              int64_t pool_stride)
+// LDS elements a gemm tile needs (two buffers of the A and B slices)
+template <typename T, int TM, int TN>
+constexpr int gemm_lds_elems() { return 2 * (sizeof(T) == 8 ? 16 : 32) * ((TM + 1) + (TN + 1)); }
+
+template <typename T, int TM, int TN>
+__device__ __forceinline__ void gemm_tile(const GemmProb *__restrict__ probs, const GemmTile tl, void *pool_v,
+                                          int64_t pool_stride, T *lds)
 {
     typedef typename Mfma<T>::acc_t acc_t;
     constexpr int BK = sizeof(T) == 8 ? 16 : 32;
     constexpr int WM = TM / 2, WN = TN / 2, FM = WM / 16, FN = WN / 16;
     constexpr int NA = TM * BK / 256, NB = TN * BK / 256;
-    __shared__ T As[2][BK][TM + 1];
-    __shared__ T Bs[2][BK][TN + 1];
-    const GemmTile tl = tiles[blockIdx.x];
+    T (*As)[BK][TM + 1] = reinterpret_cast<T (*)[BK][TM + 1]>(lds);
+    T (*Bs)[BK][TN + 1] = reinterpret_cast<T (*)[BK][TN + 1]>(lds + 2 * BK * (TM + 1));
     const GemmProb pr = probs[tl.prob];
     T *pool = reinterpret_cast<T *>(pool_v) + blockIdx.y * pool_stride;
     const T *A = pool + pr.aoff;
@@ -813,6 +822,15 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs
                 }
             }
         }
+}
+
+template <typename T, int TM, int TN, int USE>
+__global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs,
+                                              const GemmTile *__restrict__ tiles, void *pool_v,
+                                              int64_t pool_stride)
+{
+    __shared__ T lds[gemm_lds_elems<T, TM, TN>()];
+    gemm_tile<T, TM, TN>(probs, tiles[blockIdx.x], pool_v, pool_stride, lds);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -945,6 +963,25 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmProb *__restrict__ 
         }
 }
 
+// Look-ahead: the first ndiag workgroups factor the diagonal block of block k of front
+// list[blockIdx.x] (one wave; the other lanes leave at once), dispatched first; the rest apply
+// the bulk of block k-1's trailing update (64 x 64 gemm tiles the next panel does not read).
+// One LDS area serves either role.
+template <typename T, int KB>
+__global__ __launch_bounds__(256) void k_spec_diag_gemm(FactorArgs a, int k, float tau,
+                                                        const GemmProb *__restrict__ probs,
+                                                        const GemmTile *__restrict__ tiles, int ndiag)
+{
+    constexpr int NG = gemm_lds_elems<T, 64, 64>(), ND = KB * (KB + 1);
+    __shared__ T lds[NG > ND ? NG : ND];
+    if ((int)blockIdx.x >= ndiag) {
+        gemm_tile<T, 64, 64>(probs, tiles[blockIdx.x - ndiag], a.pool, a.pool_stride, lds);
+        return;
+    }
+    if (threadIdx.x >= KB) return;
+    spec_diag_front<T, KB>(a, k, tau, (int)blockIdx.x, lds);
+}
+
 template <typename T>
 hipError_t launch_gemm_tile(int tile, int use, const GemmProb *probs, const GemmTile *tiles, int count, void *pool,
                             int64_t pool_stride, int nmat, hipStream_t st)
@@ -1015,6 +1052,10 @@ hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int
         break;
     case L_SPEC_DIAG:
         hipLaunchKernelGGL((k_spec_diag<T, SPEC_KB>), grid, dim3(SPEC_KB), 0, st, a, L.k, L.tau);
+        break;
+    case L_SPEC_DIAG_GEMM:
+        hipLaunchKernelGGL((k_spec_diag_gemm<T, SPEC_KB>), grid, dim3(256), 0, st, a, L.k, L.tau, probs + L.prob_off,
+                           tiles + L.tile_off, L.count - L.ntiles);
         break;
     case L_SPEC_TRSM:
         hipLaunchKernelGGL((k_spec_trsm<T, SPEC_KB>), grid, dim3(4 * SPEC_ROWS), 0, st, a, L.k, L.tau,

@@ -60,6 +60,7 @@ enum LaunchKind : int32_t {
     L_SPEC_DIAG = 5,    // speculative schedule: factor the diagonal block without interchanges
     L_SPEC_TRSM = 6,    // speculative schedule: L rows / U columns of the block + pivot check
     L_INV_LEAF = 7,     // inverses of the 64x64 diagonal blocks of L11 and U11 (large fronts)
+    L_SPEC_DIAG_GEMM = 8,  // next diagonal block factorization alongside the bulk of the trailing update
 };
 
 struct Launch {
@@ -72,6 +73,8 @@ struct Launch {
     int32_t shmem;      // dynamic LDS bytes
     int32_t level;
     int32_t tile;       // L_GEMM: tile shape (GEMM_TILE_* index)
+    int32_t tile_off;   // L_SPEC_DIAG_GEMM: offset of its gemm tiles (its fronts are at list_off)
+    int32_t ntiles;     // L_SPEC_DIAG_GEMM: gemm tiles before the diagonal-block workgroups
     float tau;          // speculative schedule: multiplier bound 1/tau
     double flops;       // algorithmic flops of the launch (host-side bookkeeping)
     double bytes;       // algorithmic HBM bytes of the launch

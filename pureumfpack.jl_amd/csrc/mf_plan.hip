@@ -210,6 +210,7 @@ static int launch_class(const Launch &L)
     case L_PANEL_APPLY: return 2;
     case L_SPEC_DIAG: return 2;
     case L_SPEC_TRSM: return 2;
+    case L_SPEC_DIAG_GEMM: return 2;
     default: return L.variant == 3 ? 7 : L.variant == 1 ? 4 : 3;
     }
 }
@@ -601,56 +602,70 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             int64_t maxnp = 0;
             for (int64_t s : large) maxnp = std::max(maxnp, np[s]);
             if (sp == 1) {
-                // interchange-free blocks of SPEC_KB columns with the multiplier check
-                for (int64_t k = 0; k < maxnp; k += SPEC_KB) {
+                // Interchange-free blocks of SPEC_KB columns with the multiplier check, with a
+                // look-ahead: block k's trailing update is split into G1 (the next block's panel
+                // column and row, which its diagonal factorization and trsm read) and G2 (the
+                // rest); G2 runs in the same launch as the next diagonal factorization.  Every
+                // entry still receives the same products in the same order.
+                auto diag_fronts = [&](int64_t k) {
+                    std::vector<int64_t> v;
+                    for (int64_t s : large)
+                        if (np[s] > k) v.push_back(s);
+                    return v;
+                };
+                auto add_diag = [&](Launch &L, const std::vector<int64_t> &fr, int64_t k) {
+                    for (size_t q = 0; q < fr.size(); q++) {
+                        const int64_t s = fr[q];
+                        lists.push_back((int32_t)s);
+                        P.spec_slots = std::max<int64_t>(P.spec_slots, (int64_t)q + 1);
+                        const double b = (double)std::min<int64_t>(SPEC_KB, np[s] - k);
+                        L.flops += 2.0 * b * b * b / 3.0;
+                        L.bytes += 2.0 * b * b * elem;
+                    }
+                };
+                auto prob = [&](int64_t s, int64_t r0, int64_t c0, int64_t m, int64_t n, int64_t k, int64_t kb) {
+                    GemmProb g{};
+                    g.aoff = foff[s] + r0 + k * nf[s];
+                    g.boff = foff[s] + k + c0 * nf[s];
+                    g.coff = foff[s] + r0 + c0 * nf[s];
+                    g.m = (int32_t)m;
+                    g.n = (int32_t)n;
+                    g.k = (int32_t)kb;
+                    g.lda = g.ldb = g.ldc = (int32_t)nf[s];
+                    return g;
+                };
+                {
                     Launch Dg{};
                     Dg.kind = L_SPEC_DIAG;
-                    Dg.k = (int32_t)k;
+                    Dg.k = 0;
                     Dg.tau = P.pivot_tol;
                     Dg.level = (int32_t)lv;
                     Dg.list_off = (int32_t)lists.size();
-                    std::vector<GemmProb> pv;
+                    add_diag(Dg, diag_fronts(0), 0);
+                    Dg.count = (int32_t)(lists.size() - Dg.list_off);
+                    sched.push_back(Dg);
+                }
+                for (int64_t k = 0; k < maxnp; k += SPEC_KB) {
+                    const std::vector<int64_t> fr = diag_fronts(k);
                     std::vector<int32_t> items;
-                    for (int64_t s : large) {
-                        if (np[s] <= k) continue;
-                        const int32_t slot = (int32_t)(lists.size() - Dg.list_off);
-                        lists.push_back((int32_t)s);
-                        P.spec_slots = std::max<int64_t>(P.spec_slots, slot + 1);
-                        int64_t kb = std::min<int64_t>(SPEC_KB, np[s] - k), k2 = k + kb;
-                        double b = (double)kb;
-                        Dg.flops += 2.0 * b * b * b / 3.0;
-                        Dg.bytes += 2.0 * b * b * elem;
+                    std::vector<GemmProb> g1v, g2v;
+                    for (size_t q = 0; q < fr.size(); q++) {
+                        const int64_t s = fr[q];
+                        const int32_t slot = (int32_t)q;
+                        const int64_t kb = std::min<int64_t>(SPEC_KB, np[s] - k), k2 = k + kb;
                         for (int64_t r0 = 0; r0 < nf[s] - k2; r0 += SPEC_ROWS) {
                             items.push_back((int32_t)s); items.push_back(0); items.push_back((int32_t)r0); items.push_back(slot);
                         }
                         for (int64_t c0 = 0; c0 < nf[s] - k2; c0 += SPEC_ROWS) {
                             items.push_back((int32_t)s); items.push_back(1); items.push_back((int32_t)c0); items.push_back(slot);
                         }
-                        if (k2 < np[s]) {
-                            GemmProb g{};
-                            g.aoff = foff[s] + k2 + k * nf[s];
-                            g.boff = foff[s] + k + k2 * nf[s];
-                            g.coff = foff[s] + k2 + k2 * nf[s];
-                            g.m = (int32_t)(nf[s] - k2);
-                            g.n = (int32_t)(np[s] - k2);
-                            g.k = (int32_t)kb;
-                            g.lda = g.ldb = g.ldc = (int32_t)nf[s];
-                            pv.push_back(g);
-                            if (nu[s] > 0) {
-                                GemmProb h{};
-                                h.aoff = foff[s] + k2 + k * nf[s];
-                                h.boff = foff[s] + k + np[s] * nf[s];
-                                h.coff = foff[s] + k2 + np[s] * nf[s];
-                                h.m = (int32_t)(np[s] - k2);
-                                h.n = (int32_t)nu[s];
-                                h.k = (int32_t)kb;
-                                h.lda = h.ldb = h.ldc = (int32_t)nf[s];
-                                pv.push_back(h);
-                            }
-                        }
+                        if (k2 >= np[s]) continue;
+                        const int64_t k3 = k2 + std::min<int64_t>(SPEC_KB, np[s] - k2);
+                        g1v.push_back(prob(s, k2, k2, nf[s] - k2, k3 - k2, k, kb));                         // panel column
+                        if (nf[s] > k3) g1v.push_back(prob(s, k2, k3, k3 - k2, nf[s] - k3, k, kb));         // panel row
+                        if (np[s] > k3) g2v.push_back(prob(s, k3, k3, nf[s] - k3, np[s] - k3, k, kb));      // rest of the pivot columns
+                        if (np[s] > k3 && nu[s] > 0) g2v.push_back(prob(s, k3, np[s], np[s] - k3, nu[s], k, kb));  // rest of U12
                     }
-                    Dg.count = (int32_t)(lists.size() - Dg.list_off);
-                    sched.push_back(Dg);
                     if (!items.empty()) {
                         while (lists.size() & 3) lists.push_back(0);
                         Launch Tr{};
@@ -670,7 +685,31 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                         }
                         sched.push_back(Tr);
                     }
-                    add_gemm(pv, (int)lv, 0);
+                    add_gemm(g1v, (int)lv, 0);
+                    const int64_t next = k + SPEC_KB;
+                    const std::vector<int64_t> nfr = diag_fronts(next);
+                    if (nfr.empty() && g2v.empty()) continue;
+                    Launch C{};
+                    C.kind = L_SPEC_DIAG_GEMM;
+                    C.k = (int32_t)next;
+                    C.tau = P.pivot_tol;
+                    C.level = (int32_t)lv;
+                    C.list_off = (int32_t)lists.size();
+                    add_diag(C, nfr, next);
+                    C.prob_off = (int32_t)probs.size();
+                    C.tile_off = (int32_t)tiles.size();
+                    for (size_t q = 0; q < g2v.size(); q++) {
+                        const GemmProb &g = g2v[q];
+                        for (int tn = 0; tn < g.n; tn += 64)
+                            for (int tm = 0; tm < g.m; tm += 64) tiles.push_back({(int32_t)q, tm, tn, 0});
+                        probs.push_back(g);
+                        C.flops += 2.0 * g.m * (double)g.n * g.k;
+                        C.bytes += ((double)g.m * g.k + (double)g.k * g.n + 2.0 * g.m * (double)g.n) * ebytes;
+                    }
+                    xcd_order(tiles, C.tile_off);
+                    C.ntiles = (int32_t)(tiles.size() - C.tile_off);
+                    C.count = C.ntiles + (int32_t)nfr.size();
+                    sched.push_back(C);
                 }
             }
             for (int64_t k = 0; sp == 0 && k < maxnp; k += PANEL_KB) {
