@@ -203,7 +203,7 @@ def run_convdiff(hip, k, steps, warmup):
     t_an = time.perf_counter() - t0
     Rs = I.row_scaling(A, I.SCALE_SUM)
     B = np.random.default_rng(5).standard_normal((A.shape[0], 1))
-    res, _ = run_gpu(hip, A, S, Rs, B, "bf16", steps, warmup, 1e-12, 60, graph=True)
+    res, _ = run_gpu(hip, A, S, Rs, B, "bf16", steps, warmup, 1e-12, 60, graph=False)
     res.pop("profile", None)
     res.update({"workload": f"convdiff3d-{k} (n={A.shape[0]}), COLAMD, SCALE_SUM, 1 RHS, bf16 Schur + FP32 "
                             f"getrf/trsm, refine to bwd err <= 1e-12", "analysis_s": t_an,
