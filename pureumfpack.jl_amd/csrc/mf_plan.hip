@@ -24,7 +24,7 @@
 
 using namespace mf;
 
-static constexpr int64_t SOLVE_CK_HOST = 256;  // column chunk of the split-K solve products (k_rect_part)
+static constexpr int64_t SOLVE_CK_HOST = 64;   // column chunk of the split-K solve products (k_rect_part)
 static constexpr int64_t FRONT_MAXNF_HOST = 1024;  // largest front of the fused per-front sweeps
 
 static thread_local std::string g_err;
@@ -114,6 +114,7 @@ struct mf_plan {
     // per level: offsets/counts of the forward/backward split-K part and sum item lists
     std::vector<int32_t> lvl_fpo, lvl_fpc, lvl_fso, lvl_fsc, lvl_bpo, lvl_bpc, lvl_bso, lvl_bsc;
     std::vector<int32_t> lvl_ilpo, lvl_ilpc, lvl_ilso, lvl_ilsc, lvl_iupo, lvl_iupc, lvl_iuso, lvl_iusc;
+    std::vector<int32_t> lvl_ntri;   // large fronts of the level without stored inverses
     std::vector<int32_t> lvl_go, lvl_gc;   // forward gather items {front, pos0}
     // fronts swept by the fused per-front kernels (no stored inverse): nf <= 64 and
     // 64 < nf <= FRONT_MAXNF; the level lists above hold only the other fronts
@@ -921,7 +922,11 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             for (int64_t p0 = 0; p0 < nf[s]; p0 += 256) emit4((int32_t)s, (int32_t)p0, 0, 0);
         P.lvl_gc.push_back((int32_t)((slist.size() - P.lvl_go.back()) / 4));
         // dir 0: L21 (rows nu, cols np); 1: U12 (rows np, cols nu); 2: inv(L11) and 3: inv(U11)
-        // of the large fronts (rows np, cols np; chunks entirely outside the triangle skipped)
+        // of the large fronts (rows np, cols np; chunks entirely outside the triangle skipped).
+        // U12 of a root (nu = 0) has no chunks; its sum still sets x = y.
+        int32_t ntri = 0;
+        for (int64_t s : big) ntri += ioff[s] < 0 ? 1 : 0;
+        P.lvl_ntri.push_back(ntri);
         for (int dir = 0; dir < 4; dir++) {
             int64_t slot = 0;
             std::vector<int32_t> sums;
@@ -1356,9 +1361,10 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
                 return launch_solve_level<T>(9, a, sl, P.lvl_iuso[lv], P.lvl_iusc[lv], nmat, P.stream);
             }));
         }
-        HIPCHK(profiled(P, 6, 0.0, 0.0, [&] {
-            return launch_solve_level<T>(5, a, sl, P.lvl_off[lv], P.lvl_cnt[lv], nmat, P.stream);
-        }));
+        if (P.lvl_ntri[lv] > 0)
+            HIPCHK(profiled(P, 6, 0.0, 0.0, [&] {
+                return launch_solve_level<T>(5, a, sl, P.lvl_off[lv], P.lvl_cnt[lv], nmat, P.stream);
+            }));
     }
     HIPCHK(launch_finish_x<T>(P.d_x.as<T>(), P.d_qf.as<int32_t>(), P.d_X.as<double>(), n, n, nrhs, accumulate,
                               n * nrhs, nmat, P.stream));

@@ -27,8 +27,6 @@ hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st)
 constexpr int SOLVE_RC = 8;   // right-hand sides processed together per pass
 constexpr int SOLVE_NB = 64;  // triangular block size (one wave solves a diagonal block)
 
-// acc[r] += sum_{j < bs} row[j * ld] * s[r * SOLVE_NB + j]: one matrix row against a staged
-// block of SOLVE_RC vectors; 16 independent loads in flight per thread.
 // v[(r0 + r) * vld + i] -= sum_{j < bs} M[i + j * ld] * s[r * SOLVE_NB + j] for rows
 // i in [rbeg, rend), two rows per thread in flight.
 template <typename T>
@@ -62,26 +60,6 @@ __device__ __forceinline__ void rows_update(const T *M, int ld, int bs, int rbeg
                 v[(r0 + r) * vld + i0] -= acc0[r];
                 if (i1 < rend) v[(r0 + r) * vld + i1] -= acc1[r];
             }
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ void panel_dot(const T *row, int ld, int bs, const T *s, T (&acc)[SOLVE_RC])
-{
-    int j = 0;
-    for (; j + 16 <= bs; j += 16) {
-        T l[16];
-#pragma unroll
-        for (int q = 0; q < 16; q++) l[q] = row[(int64_t)(j + q) * ld];
-#pragma unroll
-        for (int q = 0; q < 16; q++)
-#pragma unroll
-            for (int r = 0; r < SOLVE_RC; r++) acc[r] += l[q] * s[r * SOLVE_NB + j + q];
-    }
-    for (; j < bs; j++) {
-        const T l = row[(int64_t)j * ld];
-#pragma unroll
-        for (int r = 0; r < SOLVE_RC; r++) acc[r] += l * s[r * SOLVE_NB + j];
     }
 }
 
@@ -186,18 +164,43 @@ __global__ __launch_bounds__(256) void k_fwd_tri(SolveArgs a)
 }
 
 // Rectangular products of the sweeps, split over (256-row block) x (SOLVE_CK-column chunk)
-// workgroups.  Each item writes its partial sums for SOLVE_RC right-hand sides at a time to
-// part[slot ...]; k_rect_sum then adds the chunks of a row block in chunk order (the result
-// does not depend on scheduling) and applies them:
-//   forward  (kind 0): uv[t] -= sum_a L21[t, a] y[c1 + a]        rows t < nu, columns a < np
-//   backward (kind 1): x[c1+i] = y[c1+i] - sum_t U12[i, t] x[upd[t]]   rows i < np, columns t < nu
-// Items are int4 {front, row0, col0, slot}; a partial slot holds 256 * nrhs values.
-constexpr int SOLVE_CK = 256;
+// workgroups:
+//   kind 0: uv[t] -= sum_a L21[t, a] y[c1 + a]                  rows t < nu, columns a < np
+//   kind 1: x[c1+i] = y[c1+i] - sum_t U12[i, t] x[upd[t]]      rows i < np, columns t < nu
+//           (fronts with inverses: into zbuf, inv(U11) follows)
+//   kind 2: y[c1+i] = sum_a inv(L11)[i, a] zbuf[c1 + a]         lower-triangle chunks only
+//   kind 3: x[c1+i] = sum_a inv(U11)[i, a] zbuf[c1 + a]         upper-triangle chunks only
+// k_rect_part items are int4 {front, row0, col0, slot}: each workgroup writes its partial sums
+// (256 rows x nrhs) to partial slot `slot`.  k_rect_sum items are {front, row0, nchunks,
+// slot0}: the row block's chunks own slots slot0 .. slot0 + nchunks - 1, added in chunk order
+// (the result does not depend on scheduling) and applied.
+constexpr int SOLVE_CK = 64;
+
+// acc[r] += sum_{j < bs} M[j * ld] * s[r * sld + j], G loads in flight
+template <typename T, int G>
+__device__ __forceinline__ void dot_rows(const T *M, int64_t ld, int bs, const T *s, int sld, T (&acc)[SOLVE_RC])
+{
+    int j = 0;
+    for (; j + G <= bs; j += G) {
+        T l[G];
+#pragma unroll
+        for (int q = 0; q < G; q++) l[q] = M[(int64_t)(j + q) * ld];
+#pragma unroll
+        for (int q = 0; q < G; q++)
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) acc[r] += l[q] * s[r * sld + j + q];
+    }
+    for (; j < bs; j++) {
+        const T l = M[(int64_t)j * ld];
+#pragma unroll
+        for (int r = 0; r < SOLVE_RC; r++) acc[r] += l * s[r * sld + j];
+    }
+}
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_rect_part(SolveArgs a, int kind)
 {
-    __shared__ T sv[SOLVE_NB * SOLVE_RC];
+    __shared__ T sv[SOLVE_CK * SOLVE_RC];
     const int mat = blockIdx.y;
     const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
     const SnDev d = a.sn[w.x];
@@ -206,7 +209,8 @@ __global__ __launch_bounds__(256) void k_rect_part(SolveArgs a, int kind)
     const int tid = threadIdx.x;
     const int row = w.y + tid;
     const int nrows = kind == 0 ? nu : np, ncols = kind == 1 ? nu : np;
-    const int cend = min(w.z + SOLVE_CK, ncols);
+    const int col0 = w.z;
+    const int bs = max(0, min(SOLVE_CK, ncols - col0));
     const int64_t vb = (int64_t)mat * a.nrhs;
     const T *y = reinterpret_cast<const T *>(a.y) + vb * a.n;
     const T *x = reinterpret_cast<const T *>(a.x) + vb * a.n;
@@ -227,29 +231,27 @@ __global__ __launch_bounds__(256) void k_rect_part(SolveArgs a, int kind)
     const int nrhs = (int)a.nrhs;
     for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
-        T acc[SOLVE_RC] = {};
-        for (int j0 = w.z; j0 < cend; j0 += SOLVE_NB) {
-            const int bs = min(SOLVE_NB, cend - j0);
-            __syncthreads();
-            for (int e = tid; e < SOLVE_NB * SOLVE_RC; e += blockDim.x) {
-                const int j = e % SOLVE_NB, r = e / SOLVE_NB;
-                T v = T(0);
-                if (j < bs && r < rc) {
-                    const int64_t ro = (int64_t)(r0 + r) * a.n;
-                    v = kind == 0 ? y[ro + c1 + j0 + j] : kind == 1 ? x[ro + upd[j0 + j]] : zb[ro + c1 + j0 + j];
-                }
-                sv[e] = v;
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < SOLVE_CK * SOLVE_RC / 256; h++) {
+            const int e = tid + h * 256;
+            const int j = e % SOLVE_CK, r = e / SOLVE_CK;
+            T v = T(0);
+            if (j < bs && r < rc) {
+                const int64_t ro = (int64_t)(r0 + r) * a.n;
+                v = kind == 0 ? y[ro + c1 + col0 + j] : kind == 1 ? x[ro + upd[col0 + j]] : zb[ro + c1 + col0 + j];
             }
-            __syncthreads();
-            if (row < nrows) panel_dot<T>(rowp + (int64_t)j0 * ld, ld, bs, sv, acc);
+            sv[e] = v;
         }
+        __syncthreads();
+        T acc[SOLVE_RC] = {};
+        if (row < nrows) dot_rows<T, 32>(rowp + (int64_t)col0 * ld, ld, bs, sv, SOLVE_CK, acc);
 #pragma unroll
         for (int r = 0; r < SOLVE_RC; r++)
             if (r < rc) part[(int64_t)(r0 + r) * 256 + tid] = acc[r];
     }
 }
 
-// Items int4 {front, row0, nchunks, slot0}; chunk c of the row block is partial slot slot0 + c.
 template <typename T>
 __global__ __launch_bounds__(256) void k_rect_sum(SolveArgs a, int kind)
 {
@@ -260,22 +262,47 @@ __global__ __launch_bounds__(256) void k_rect_sum(SolveArgs a, int kind)
     const int row = w.y + threadIdx.x;
     const int nrows = kind == 0 ? nu : np;
     if (row >= nrows) return;
+    const int nch = w.z;
     const int64_t vb = (int64_t)mat * a.nrhs;
     T *y = reinterpret_cast<T *>(a.y) + vb * a.n;
     T *x = reinterpret_cast<T *>(a.x) + vb * a.n;
     T *zb = reinterpret_cast<T *>(a.zbuf) + vb * a.n;
     T *uv = reinterpret_cast<T *>(a.uv) + vb * a.nupd;
-    const T *part = reinterpret_cast<const T *>(a.part) + mat * a.part_stride + (int64_t)w.w * 256 * a.nrhs;
-    const int64_t cs = (int64_t)256 * a.nrhs;
-    for (int r = 0; r < (int)a.nrhs; r++) {
-        T s = T(0);
-        for (int c = 0; c < w.z; c++) s += part[c * cs + (int64_t)r * 256 + threadIdx.x];
-        const int64_t ro = (int64_t)r * a.n + c1 + row;
-        switch (kind) {
-        case 0: uv[r * a.nupd + d.upd_off + row] -= s; break;
-        case 1: (d.ioff >= 0 ? zb : x)[ro] = y[ro] - s; break;   // large fronts: inv(U11) next
-        case 2: y[ro] = s; break;
-        default: x[ro] = s; break;
+    const int64_t cs = (int64_t)256 * a.nrhs;   // one partial slot
+    const T *part = reinterpret_cast<const T *>(a.part) + mat * a.part_stride + (int64_t)w.w * cs;
+    const int nrhs = (int)a.nrhs;
+    for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
+        const int rc = min(SOLVE_RC, nrhs - r0);
+        const T *pp = part + (int64_t)r0 * 256 + threadIdx.x;
+        // chunks in order, four chunks x SOLVE_RC right-hand sides of loads in flight
+        T s[SOLVE_RC] = {};
+        int c = 0;
+        for (; c + 4 <= nch; c += 4) {
+            T v[4][SOLVE_RC];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) v[q][r] = r < rc ? pp[(c + q) * cs + r * 256] : T(0);
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) s[r] += v[q][r];
+        }
+        for (; c < nch; c++) {
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++)
+                if (r < rc) s[r] += pp[c * cs + r * 256];
+        }
+#pragma unroll
+        for (int r = 0; r < SOLVE_RC; r++) {
+            if (r >= rc) continue;
+            const int64_t ro = (int64_t)(r0 + r) * a.n + c1 + row;
+            switch (kind) {
+            case 0: uv[(int64_t)(r0 + r) * a.nupd + d.upd_off + row] -= s[r]; break;
+            case 1: (d.ioff >= 0 ? zb : x)[ro] = y[ro] - s[r]; break;   // inv(U11) next
+            case 2: y[ro] = s[r]; break;
+            default: x[ro] = s[r]; break;
+            }
         }
     }
 }
@@ -492,27 +519,6 @@ __global__ __launch_bounds__(256) void k_bwd_front_w(SolveArgs a, int count, int
     }
 }
 
-// acc[r] += sum_{j < bs} M[j * ld] * s[r * sld + j], 16 loads in flight
-template <typename T>
-__device__ __forceinline__ void dot_rows16(const T *M, int64_t ld, int bs, const T *s, int sld, T (&acc)[SOLVE_RC])
-{
-    int j = 0;
-    for (; j + 16 <= bs; j += 16) {
-        T l[16];
-#pragma unroll
-        for (int q = 0; q < 16; q++) l[q] = M[(int64_t)(j + q) * ld];
-#pragma unroll
-        for (int q = 0; q < 16; q++)
-#pragma unroll
-            for (int r = 0; r < SOLVE_RC; r++) acc[r] += l[q] * s[r * sld + j + q];
-    }
-    for (; j < bs; j++) {
-        const T l = M[(int64_t)j * ld];
-#pragma unroll
-        for (int r = 0; r < SOLVE_RC; r++) acc[r] += l * s[r * sld + j];
-    }
-}
-
 template <typename T, int MAXNF>
 __global__ __launch_bounds__(256) void k_fwd_front_b(SolveArgs a, const int32_t *__restrict__ ecolptr,
                                                      const int2 *__restrict__ ext)
@@ -593,7 +599,7 @@ __global__ __launch_bounds__(256) void k_fwd_front_b(SolveArgs a, const int32_t 
             __syncthreads();
             for (int p = b0 + bs + tid; p < nf; p += NT) {
                 T acc[SOLVE_RC] = {};
-                dot_rows16<T>(F + p + (int64_t)b0 * nf, nf, bs, g + b0, MAXNF, acc);
+                dot_rows<T, 16>(F + p + (int64_t)b0 * nf, nf, bs, g + b0, MAXNF, acc);
 #pragma unroll
                 for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + p] -= acc[r];
             }
@@ -711,7 +717,7 @@ __global__ __launch_bounds__(256) void k_bwd_front_b(SolveArgs a)
             __syncthreads();
             for (int i = tid; i < b0; i += NT) {
                 T acc[SOLVE_RC] = {};
-                dot_rows16<T>(F + i + (int64_t)b0 * nf, nf, bs, g + b0, MAXNF, acc);
+                dot_rows<T, 16>(F + i + (int64_t)b0 * nf, nf, bs, g + b0, MAXNF, acc);
 #pragma unroll
                 for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + i] -= acc[r];
             }
@@ -924,9 +930,8 @@ hipError_t launch_solve_level(int which, const SolveArgs &a, const int32_t *list
 {
     SolveArgs b = a;
     b.list = lists + list_off;
-    // right-hand-side chunks of SOLVE_RC in grid z (the split-K sums take every chunk at once)
-    const bool sum = which == 2 || which == 4 || which == 7 || which == 9;
-    const int nz = sum ? 1 : (int)((a.nrhs + SOLVE_RC - 1) / SOLVE_RC);
+    // right-hand-side chunks of SOLVE_RC in grid z
+    const int nz = (int)((a.nrhs + SOLVE_RC - 1) / SOLVE_RC);
     dim3 g(count, nmat, nz), blk(256);
     switch (which) {
     case 0: hipLaunchKernelGGL(k_fwd_tri<T>, g, blk, 0, st, b); break;
