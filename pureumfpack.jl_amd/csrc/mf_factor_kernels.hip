@@ -379,32 +379,6 @@ __global__ __launch_bounds__(256) void k_panel_apply(FactorArgs a, int k)
 //     kind 1: columns c in [start, start+SPEC_ROWS) of [k+kb, nf): U[k:k+kb, c] = L_kk^-1 A[k:k+kb, c];
 //     slot: the front's position in the diagonal launch (its packed block in a.spec).
 // ------------------------------------------------------------------------------------------
-// value held by lane l of the wave (l uniform), via v_readlane
-__device__ __forceinline__ double lane_value(double v, int l)
-{
-    const unsigned long long u = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(unsigned)u, l);
-    const int hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
-    return __longlong_as_double(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
-}
-__device__ __forceinline__ float lane_value(float v, int l)
-{
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-
-// a value known to be equal in all lanes, moved to scalar registers
-__device__ __forceinline__ double wave_uniform(double v)
-{
-    const unsigned long long u = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readfirstlane((int)(unsigned)u);
-    const int hi = __builtin_amdgcn_readfirstlane((int)(unsigned)(u >> 32));
-    return __longlong_as_double(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
-}
-__device__ __forceinline__ float wave_uniform(float v)
-{
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
-
 // The KB steps of the diagonal block factorization (see k_spec_diag).  FAST: every pivot is
 // assumed normal (|u_jj| >= smallest normal): scaling by the reciprocal, branch-free, and
 // `special` records whether the assumption failed.  Otherwise getf2's rules per step: a zero

@@ -74,6 +74,32 @@ __device__ __forceinline__ void wave_argmax(T &best, int &bi)
     }
 }
 
+// value held by lane l of the wave (l uniform), via v_readlane
+__device__ __forceinline__ double lane_value(double v, int l)
+{
+    const unsigned long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const int hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __longlong_as_double(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float lane_value(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// a value known to be equal in all lanes, moved to scalar registers
+__device__ __forceinline__ double wave_uniform(double v)
+{
+    const unsigned long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(unsigned)u);
+    const int hi = __builtin_amdgcn_readfirstlane((int)(unsigned)(u >> 32));
+    return __longlong_as_double(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float wave_uniform(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 struct FactorArgs {
     const SnDev *sn;
     const int32_t *list;        // fronts of this launch (or columns as int2 for L_ASM_COLS)
@@ -106,7 +132,7 @@ struct SolveArgs {
     void *part;    // split-K partial sums of the rectangular products
     void *zbuf;    // large fronts: permuted pivot part (forward) / U12-corrected part (backward)
     const int32_t *ecolptr;  // extend-add lists of the factorization (per front position)
-    const int2 *ext;
+    const int32_t *eupd;     // their entries as offsets of the child's update rows in uv
     int64_t n, nupd, nrhs;
     int64_t pool_stride, n_stride, part_stride;
 };

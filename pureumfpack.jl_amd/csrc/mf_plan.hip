@@ -24,7 +24,8 @@
 
 using namespace mf;
 
-static constexpr int64_t SOLVE_CK_HOST = 64;   // column chunk of the split-K solve products (k_rect_part)
+static constexpr int64_t SOLVE_CK_HOST = 64;    // column chunk of the split-K solve products (k_rect_part)
+static constexpr int64_t SOLVE_FUSE_MIN = 128;  // fewer mid-size fronts in a level: split pipeline
 static constexpr int64_t FRONT_MAXNF_HOST = 1024;  // largest front of the fused per-front sweeps
 
 static thread_local std::string g_err;
@@ -124,7 +125,7 @@ struct mf_plan {
     int64_t part_slots = 0;  // partial-sum slots (256 rows x nrhs each) the solve needs
     std::vector<double> lvl_lflops, lvl_uflops, lvl_lbytes, lvl_ubytes, lvl_vec;
     // device
-    DevBuf d_sn, d_upd, d_rmap, d_child, d_orow, d_ocol, d_osrc, d_ocolptr, d_ecolptr, d_ext, d_arow;
+    DevBuf d_sn, d_upd, d_rmap, d_child, d_orow, d_ocol, d_osrc, d_ocolptr, d_ecolptr, d_ext, d_eupd, d_arow;
     DevBuf d_lists, d_probs, d_tiles, d_solvelist, d_qf;
     DevBuf d_csr_rp, d_csr_ci, d_csr_vmap;
     DevBuf d_pool, d_prow, d_info, d_ax, d_rs, d_pinfo, d_flag, d_spec;
@@ -464,6 +465,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             for (int64_t t = 0; t < nu[c]; t++) ecolptr[fcol[s] + rmap[uoff0[c] + t] + 1]++;
     for (int64_t q = 0; q < fcol[nsuper]; q++) ecolptr[q + 1] += ecolptr[q];
     std::vector<int2> ext(ecolptr[fcol[nsuper]]);
+    std::vector<int32_t> eupd(ext.size());   // the same entries as offsets into the update vectors
     {
         std::vector<int32_t> cur(ecolptr.begin(), ecolptr.end() - 1);
         for (int64_t s = 0; s < nsuper; s++)
@@ -471,6 +473,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                 int64_t c = *it;
                 for (int64_t t = 0; t < nu[c]; t++) {
                     int64_t col = fcol[s] + rmap[uoff0[c] + t];
+                    eupd[cur[col]] = (int32_t)(uoff0[c] + t);
                     ext[cur[col]++] = make_int2((int)c, (int)t);
                 }
             }
@@ -895,22 +898,32 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
         slist.push_back(a0); slist.push_back(a1); slist.push_back(a2); slist.push_back(a3);
     };
     for (int64_t lv = 0; lv < nlev; lv++) {
+        // The fused one-workgroup-per-front sweeps pay off when a level has many fronts.  A
+        // level that needs the split pipeline for a large front anyway (stored inverses or
+        // nf > FRONT_MAXNF), or has only a few fronts beyond one wave, sends all its fronts
+        // through the split pipeline, whose launches spread each front over many workgroups.
+        int64_t nmid = 0;
+        bool split = false;
+        for (int64_t s : bylevel[lv]) {
+            if (ioff[s] >= 0 || nf[s] > FRONT_MAXNF_HOST) split = true;
+            else if (nf[s] > 64) nmid++;
+        }
+        if (nmid > 0 && nmid < SOLVE_FUSE_MIN) split = true;
         std::vector<int64_t> big;
         P.lvl_s64o.push_back((int32_t)slist.size());
         for (int64_t s : bylevel[lv])
-            if (ioff[s] < 0 && nf[s] <= 64) slist.push_back((int32_t)s);
+            if (!split && nf[s] <= 64) slist.push_back((int32_t)s);
         P.lvl_s64c.push_back((int32_t)slist.size() - P.lvl_s64o.back());
         P.lvl_s1ko.push_back((int32_t)slist.size());
         int64_t mnf1k = 0;
         for (int64_t s : bylevel[lv])
-            if (ioff[s] < 0 && nf[s] > 64 && nf[s] <= FRONT_MAXNF_HOST) {
+            if (!split && nf[s] > 64) {
                 slist.push_back((int32_t)s);
                 mnf1k = std::max(mnf1k, nf[s]);
             }
         P.lvl_s1k_small.push_back(mnf1k <= 256 ? 1 : 0);
         P.lvl_s1kc.push_back((int32_t)slist.size() - P.lvl_s1ko.back());
-        for (int64_t s : bylevel[lv])
-            if (ioff[s] >= 0 || nf[s] > FRONT_MAXNF_HOST) big.push_back(s);
+        if (split) big = bylevel[lv];
         P.lvl_off[lv] = (int32_t)slist.size();
         for (int64_t s : big) slist.push_back((int32_t)s);
         P.lvl_cnt[lv] = (int32_t)big.size();
@@ -1028,6 +1041,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
     HIPCHK(upload(P.d_ocolptr, ocolptr));
     HIPCHK(upload(P.d_ecolptr, ecolptr));
     HIPCHK(upload(P.d_ext, ext));
+    HIPCHK(upload(P.d_eupd, eupd));
     HIPCHK(upload(P.d_arow, arow));
     HIPCHK(upload(P.d_lists, lists));
     HIPCHK(upload(P.d_probs, probs));
@@ -1290,7 +1304,7 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
     a.part = P.d_part.p;
     a.zbuf = P.d_zbuf.p;
     a.ecolptr = P.d_ecolptr.as<int32_t>();
-    a.ext = P.d_ext.as<int2>();
+    a.eupd = P.d_eupd.as<int32_t>();
     a.n = n;
     a.nupd = P.nupd;
     a.nrhs = nrhs;
@@ -1314,9 +1328,10 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
         HIPCHK(profiled(P, 5, 0.0, vb * nmat, [&] {
             return launch_solve_level<T>(10, a, sl, P.lvl_go[lv], P.lvl_gc[lv], nmat, P.stream);
         }));
-        HIPCHK(profiled(P, 5, 0.0, 0.0, [&] {
-            return launch_solve_level<T>(0, a, sl, P.lvl_off[lv], P.lvl_cnt[lv], nmat, P.stream);
-        }));
+        if (P.lvl_ntri[lv] > 0)
+            HIPCHK(profiled(P, 5, 0.0, 0.0, [&] {
+                return launch_solve_level<T>(0, a, sl, P.lvl_off[lv], P.lvl_cnt[lv], nmat, P.stream);
+            }));
         if (P.lvl_ilsc[lv] > 0) {
             HIPCHK(profiled(P, 5, 0.0, 0.0, [&] {
                 return launch_solve_level<T>(6, a, sl, P.lvl_ilpo[lv], P.lvl_ilpc[lv], nmat, P.stream);

@@ -63,6 +63,55 @@ __device__ __forceinline__ void rows_update(const T *M, int ld, int bs, int rbeg
     }
 }
 
+// One wave: v <- L^-1 v for the unit lower bs x bs block staged in sd (leading dimension
+// ldd, lane = row; rows and columns >= bs of sd and v are zero).  Columns in groups of 8,
+// read from LDS before their 8 steps, without branches: a step past bs has a zero column and
+// changes nothing.  The value of lane j reaches the other lanes by v_readlane (j is
+// wave-uniform).
+template <typename T>
+__device__ __forceinline__ void wave_lower_solve(const T *sd, int ldd, int bs, int lane, T (&v)[SOLVE_RC])
+{
+    for (int j0 = 0; j0 < bs; j0 += 8) {
+        T l[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const T t = sd[lane + (j0 + q) * ldd];
+            l[q] = lane > j0 + q ? t : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) v[r] -= l[q] * lane_value(v[r], j0 + q);
+    }
+}
+
+// One wave: v <- U^-1 v for the upper bs x bs block staged in sd, last column first; lane j
+// takes x_j = v_j / u_jj (one division per step), the lanes above subtract u_ij x_j.
+// Steps below column 0 (the last group) have u = 0 and 1/u_jj = 0 and change nothing.
+template <typename T>
+__device__ __forceinline__ void wave_upper_solve(const T *sd, int ldd, int bs, int lane, T (&v)[SOLVE_RC])
+{
+    for (int j1 = bs - 1; j1 >= 0; j1 -= 8) {
+        T u[8], rj[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int j = max(j1 - q, 0);
+            const T t = sd[lane + j * ldd];
+            u[q] = lane < j1 - q ? t : T(0);
+            rj[q] = j1 - q >= 0 ? T(1) / sd[j + j * ldd] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int j = j1 - q;
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) {
+                const T t = lane_value(v[r], max(j, 0)) * rj[q];
+                v[r] = lane == j ? t : v[r] - u[q] * t;
+            }
+        }
+    }
+}
+
 // Forward sweep, part 0: pull the children's update vectors into this supernode's front
 // vector, one thread per front position (items int4 {front, pos0}): pivot rows accumulate in
 // w0 (V-row order), update rows start this supernode's update vector uv.  The children of a
@@ -70,7 +119,7 @@ __device__ __forceinline__ void rows_update(const T *M, int ld, int bs, int rbeg
 // order, so the sums do not depend on scheduling.
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_gather(SolveArgs a, const int32_t *__restrict__ ecolptr,
-                                                    const int2 *__restrict__ ext)
+                                                    const int32_t *__restrict__ eupd)
 {
     const int mat = blockIdx.y;
     const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
@@ -88,8 +137,7 @@ __global__ __launch_bounds__(256) void k_fwd_gather(SolveArgs a, const int32_t *
         for (int r = 0; r < SOLVE_RC; r++)
             acc[r] = (r < rc && p < d.np) ? w0[(int64_t)(r0 + r) * a.n + d.c1 + p] : T(0);
         for (int q = q0; q < q1; q++) {
-            const int2 e = ext[q];
-            const int32_t off = a.sn[e.x].upd_off + e.y;
+            const int32_t off = eupd[q];
 #pragma unroll
             for (int r = 0; r < SOLVE_RC; r++)
                 if (r < rc) acc[r] += uv[(int64_t)(r0 + r) * a.nupd + off];
@@ -105,8 +153,9 @@ __global__ __launch_bounds__(256) void k_fwd_gather(SolveArgs a, const int32_t *
 
 // Forward sweep, part 1 (one workgroup per supernode of the level): permute the gathered
 // pivot part by the block pivoting into y (factor order), then solve with the unit lower L11
-// block by block.  The L21 product that finishes
-// the update vector is k_fwd_rect.
+// block by block.  The L21 product that finishes the update vector is k_rect_part kind 0.
+// Fronts with stored inverses skip this: k_rect_part kind 2 reads the gathered vector
+// through the pivoting itself.
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_tri(SolveArgs a)
 {
@@ -122,14 +171,18 @@ __global__ __launch_bounds__(256) void k_fwd_tri(SolveArgs a)
     T *w0 = reinterpret_cast<T *>(a.w0) + vb * a.n;
     T *y = reinterpret_cast<T *>(a.y) + vb * a.n;
     const int nrhs = (int)a.nrhs;
-    const bool inv = d.ioff >= 0;   // large front: inv(L11) is applied by k_rect_part kind 2
-    T *z = inv ? reinterpret_cast<T *>(a.zbuf) + vb * a.n : y;
-    for (int e = tid; e < np * nrhs; e += nt) {
-        const int i = e % np, r = e / np;
-        if ((r / SOLVE_RC) % (int)gridDim.z != (int)blockIdx.z) continue;   // another chunk's block
-        z[r * a.n + c1 + i] = w0[r * a.n + prow[c1 + i]];
+    if (d.ioff >= 0) return;   // large front: inv(L11) is applied by k_rect_part kind 2
+    for (int i = tid; i < np; i += nt) {
+        const int64_t src = prow[c1 + i];
+        for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
+            T v[SOLVE_RC];
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) v[r] = r0 + r < nrhs ? w0[(int64_t)(r0 + r) * a.n + src] : T(0);
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++)
+                if (r0 + r < nrhs) y[(int64_t)(r0 + r) * a.n + c1 + i] = v[r];
+        }
     }
-    if (inv) return;
     __syncthreads();
     for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
@@ -145,11 +198,7 @@ __global__ __launch_bounds__(256) void k_fwd_tri(SolveArgs a)
 #pragma unroll
                 for (int r = 0; r < SOLVE_RC; r++)
                     v[r] = (r < rc && lane < bs) ? y[(r0 + r) * a.n + c1 + b0 + lane] : T(0);
-                for (int j = 0; j < bs; j++) {
-                    const T l = (lane > j) ? sdiag[lane + j * (SOLVE_NB + 1)] : T(0);
-#pragma unroll
-                    for (int r = 0; r < SOLVE_RC; r++) v[r] -= l * __shfl(v[r], j);
-                }
+                wave_lower_solve<T>(sdiag, SOLVE_NB + 1, bs, lane, v);
 #pragma unroll
                 for (int r = 0; r < SOLVE_RC; r++) {
                     sblk[r * SOLVE_NB + lane] = v[r];
@@ -168,7 +217,7 @@ __global__ __launch_bounds__(256) void k_fwd_tri(SolveArgs a)
 //   kind 0: uv[t] -= sum_a L21[t, a] y[c1 + a]                  rows t < nu, columns a < np
 //   kind 1: x[c1+i] = y[c1+i] - sum_t U12[i, t] x[upd[t]]      rows i < np, columns t < nu
 //           (fronts with inverses: into zbuf, inv(U11) follows)
-//   kind 2: y[c1+i] = sum_a inv(L11)[i, a] zbuf[c1 + a]         lower-triangle chunks only
+//   kind 2: y[c1+i] = sum_a inv(L11)[i, a] w0[prow[c1 + a]]     lower-triangle chunks only
 //   kind 3: x[c1+i] = sum_a inv(U11)[i, a] zbuf[c1 + a]         upper-triangle chunks only
 // k_rect_part items are int4 {front, row0, col0, slot}: each workgroup writes its partial sums
 // (256 rows x nrhs) to partial slot `slot`.  k_rect_sum items are {front, row0, nchunks,
@@ -215,6 +264,8 @@ __global__ __launch_bounds__(256) void k_rect_part(SolveArgs a, int kind)
     const T *y = reinterpret_cast<const T *>(a.y) + vb * a.n;
     const T *x = reinterpret_cast<const T *>(a.x) + vb * a.n;
     const T *zb = reinterpret_cast<const T *>(a.zbuf) + vb * a.n;
+    const T *w0 = reinterpret_cast<const T *>(a.w0) + vb * a.n;
+    const int32_t *prow = a.prow + mat * a.n_stride;
     T *part = reinterpret_cast<T *>(a.part) + mat * a.part_stride + (int64_t)w.w * 256 * a.nrhs;
     const int32_t *upd = a.upd + d.upd_off;
     // row r of the operand: kind 0 L21 row (np + r) over columns a; kind 1 U12 row r over
@@ -239,7 +290,10 @@ __global__ __launch_bounds__(256) void k_rect_part(SolveArgs a, int kind)
             T v = T(0);
             if (j < bs && r < rc) {
                 const int64_t ro = (int64_t)(r0 + r) * a.n;
-                v = kind == 0 ? y[ro + c1 + col0 + j] : kind == 1 ? x[ro + upd[col0 + j]] : zb[ro + c1 + col0 + j];
+                v = kind == 0   ? y[ro + c1 + col0 + j]
+                    : kind == 1 ? x[ro + upd[col0 + j]]
+                    : kind == 2 ? w0[ro + prow[c1 + col0 + j]]
+                                : zb[ro + c1 + col0 + j];
             }
             sv[e] = v;
         }
@@ -336,15 +390,7 @@ __global__ __launch_bounds__(256) void k_bwd_tri(SolveArgs a)
 #pragma unroll
                 for (int r = 0; r < SOLVE_RC; r++)
                     v[r] = (r < rc && lane < bs) ? x[(r0 + r) * a.n + c1 + b0 + lane] : T(0);
-                for (int j = bs - 1; j >= 0; j--) {
-                    const T rjj = T(1) / sdiag[j + j * (SOLVE_NB + 1)];   // one division per step
-                    const T u = (lane < j) ? sdiag[lane + j * (SOLVE_NB + 1)] : T(0);
-#pragma unroll
-                    for (int r = 0; r < SOLVE_RC; r++) {
-                        if (lane == j) v[r] = v[r] * rjj;
-                        v[r] -= u * __shfl(v[r], j);
-                    }
-                }
+                wave_upper_solve<T>(sdiag, SOLVE_NB + 1, bs, lane, v);
 #pragma unroll
                 for (int r = 0; r < SOLVE_RC; r++) {
                     sblk[r * SOLVE_NB + lane] = v[r];
@@ -374,7 +420,7 @@ constexpr int FRONT_MAXNF = 1024;
 
 template <typename T>
 __device__ __forceinline__ void fwd_front_w(const SolveArgs &a, const int32_t *__restrict__ ecolptr,
-                                            const int2 *__restrict__ ext, int slot, int mat, int zc, int nz)
+                                            const int32_t *__restrict__ eupd, int slot, int mat, int zc, int nz)
 {
     const SnDev d = a.sn[a.list[slot]];
     const int nf = d.nf, np = d.np, c1 = d.c1;
@@ -396,8 +442,7 @@ __device__ __forceinline__ void fwd_front_w(const SolveArgs &a, const int32_t *_
         for (int r = 0; r < SOLVE_RC; r++)
             v[r] = (r < rc && p < np) ? w0[(int64_t)(r0 + r) * a.n + c1 + p] : T(0);
         for (int q = q0; q < q1; q++) {
-            const int2 e = ext[q];
-            const int32_t off = a.sn[e.x].upd_off + e.y;
+            const int32_t off = eupd[q];
 #pragma unroll
             for (int r = 0; r < SOLVE_RC; r++)
                 if (r < rc) v[r] += uv[(int64_t)(r0 + r) * a.nupd + off];
@@ -415,7 +460,7 @@ __device__ __forceinline__ void fwd_front_w(const SolveArgs &a, const int32_t *_
                 const int j = j0 + q;
                 if (j >= np) break;
 #pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++) v[r] -= lv[q] * __shfl(v[r], j);
+                for (int r = 0; r < SOLVE_RC; r++) v[r] -= lv[q] * lane_value(v[r], j);
             }
         }
         if (on) {
@@ -459,7 +504,7 @@ __device__ __forceinline__ void bwd_front_w(const SolveArgs &a, int slot, int ma
                 const int t = t0 + q;
                 if (t >= nu) break;
 #pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++) acc[r] += uu[q] * __shfl(v[r], np + t);
+                for (int r = 0; r < SOLVE_RC; r++) acc[r] += uu[q] * lane_value(v[r], np + t);
             }
         }
 #pragma unroll
@@ -480,8 +525,8 @@ __device__ __forceinline__ void bwd_front_w(const SolveArgs &a, int slot, int ma
                 const T rjj = T(1) / dg[q];   // one division per step
 #pragma unroll
                 for (int r = 0; r < SOLVE_RC; r++) {
-                    if (p == j) v[r] = v[r] * rjj;
-                    v[r] -= uc[q] * __shfl(v[r], j);
+                    const T t = lane_value(v[r], j) * rjj;
+                    v[r] = p == j ? t : v[r] - uc[q] * t;
                 }
             }
         }
@@ -498,13 +543,13 @@ __device__ __forceinline__ void bwd_front_w(const SolveArgs &a, int slot, int ma
 // times a batch of matrices is a few thousand workgroups, not millions.
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_front_w(SolveArgs a, const int32_t *__restrict__ ecolptr,
-                                                     const int2 *__restrict__ ext, int count, int nmat, int nz)
+                                                     const int32_t *__restrict__ eupd, int count, int nmat, int nz)
 {
     const int64_t items = (int64_t)count * nmat * nz;
     for (int64_t it = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); it < items; it += (int64_t)gridDim.x * 4) {
         const int slot = (int)(it % count);
         const int64_t rest = it / count;
-        fwd_front_w<T>(a, ecolptr, ext, slot, (int)(rest % nmat), (int)(rest / nmat), nz);
+        fwd_front_w<T>(a, ecolptr, eupd, slot, (int)(rest % nmat), (int)(rest / nmat), nz);
     }
 }
 
@@ -521,9 +566,9 @@ __global__ __launch_bounds__(256) void k_bwd_front_w(SolveArgs a, int count, int
 
 template <typename T, int MAXNF>
 __global__ __launch_bounds__(256) void k_fwd_front_b(SolveArgs a, const int32_t *__restrict__ ecolptr,
-                                                     const int2 *__restrict__ ext)
+                                                     const int32_t *__restrict__ eupd)
 {
-    constexpr int NT = 256, PT = MAXNF / NT, LDD = SOLVE_NB + 1;
+    constexpr int NT = 256, PT = MAXNF / NT, LDD = SOLVE_NB + 1, DPT = SOLVE_NB * SOLVE_NB / NT;
     __shared__ T g[SOLVE_RC * MAXNF];            // g[r * MAXNF + p]
     __shared__ T sd[SOLVE_NB * LDD];             // diagonal block of L11
     const int mat = blockIdx.y;
@@ -537,33 +582,67 @@ __global__ __launch_bounds__(256) void k_fwd_front_b(SolveArgs a, const int32_t 
     T *y = reinterpret_cast<T *>(a.y) + vb * a.n;
     T *uv = reinterpret_cast<T *>(a.uv) + vb * a.nupd;
     const int nrhs = (int)a.nrhs;
+    // the extend-add ranges of this thread's positions p = tid + k NT
+    int qb[PT], qe[PT];
+    int nstep = 0;
+#pragma unroll
+    for (int k = 0; k < PT; k++) {
+        const int p = tid + k * NT;
+        qb[k] = p < nf ? ecolptr[d.fcol + p] : 0;
+        qe[k] = p < nf ? ecolptr[d.fcol + p + 1] : 0;
+        nstep = max(nstep, qe[k] - qb[k]);
+    }
     for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
-        for (int p = tid; p < nf; p += NT) {
-            const int q0 = ecolptr[d.fcol + p], q1 = ecolptr[d.fcol + p + 1];
-            T acc[SOLVE_RC];
+        // the first diagonal block does not depend on the gather: its loads go out first
+        T dpre[DPT];
+        {
+            const int bs = min(SOLVE_NB, np);
+#pragma unroll
+            for (int h = 0; h < DPT; h++) {
+                const int e = tid + h * NT, i = e % SOLVE_NB, j = e / SOLVE_NB;
+                dpre[h] = (i < bs && j < bs) ? F[i + (int64_t)j * nf] : T(0);
+            }
+        }
+        // gather: own entries plus the children's update rows, child by child in list order,
+        // every position of the thread at once
+        T acc[PT][SOLVE_RC];
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            const int p = tid + k * NT;
 #pragma unroll
             for (int r = 0; r < SOLVE_RC; r++)
-                acc[r] = (r < rc && p < np) ? w0[(int64_t)(r0 + r) * a.n + c1 + p] : T(0);
-            for (int q = q0; q < q1; q++) {
-                const int2 e = ext[q];
-                const int32_t off = a.sn[e.x].upd_off + e.y;
+                acc[k][r] = (r < rc && p < np) ? w0[(int64_t)(r0 + r) * a.n + c1 + p] : T(0);
+        }
+        for (int st = 0; st < nstep; st++) {
+            int off[PT];
 #pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++)
-                    if (r < rc) acc[r] += uv[(int64_t)(r0 + r) * a.nupd + off];
+            for (int k = 0; k < PT; k++) off[k] = qb[k] + st < qe[k] ? eupd[qb[k] + st] : -1;
+#pragma unroll
+            for (int k = 0; k < PT; k++)
+                if (off[k] >= 0) {
+#pragma unroll
+                    for (int r = 0; r < SOLVE_RC; r++)
+                        if (r < rc) acc[k][r] += uv[(int64_t)(r0 + r) * a.nupd + off[k]];
+                }
+        }
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            const int p = tid + k * NT;
+            if (p < nf) {
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + p] = acc[k][r];
             }
-#pragma unroll
-            for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + p] = acc[r];
         }
         __syncthreads();
-        T t[PT][SOLVE_RC];
+        // pivot part into factor order (the block pivoting), through registers
 #pragma unroll
         for (int k = 0; k < PT; k++) {
             const int i = tid + k * NT;
             if (i < np) {
                 const int src = prow[c1 + i] - c1;
 #pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++) t[k][r] = g[r * MAXNF + src];
+                for (int r = 0; r < SOLVE_RC; r++) acc[k][r] = g[r * MAXNF + src];
             }
         }
         __syncthreads();
@@ -572,25 +651,29 @@ __global__ __launch_bounds__(256) void k_fwd_front_b(SolveArgs a, const int32_t 
             const int i = tid + k * NT;
             if (i < np) {
 #pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + i] = t[k][r];
+                for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + i] = acc[k][r];
             }
         }
         for (int b0 = 0; b0 < np; b0 += SOLVE_NB) {
             const int bs = min(SOLVE_NB, np - b0);
-            for (int e = tid; e < SOLVE_NB * SOLVE_NB; e += NT) {
-                const int i = e % SOLVE_NB, j = e / SOLVE_NB;
-                sd[i + j * LDD] = (i < bs && j < bs) ? F[(b0 + i) + (int64_t)(b0 + j) * nf] : T(0);
+            if (b0 > 0) {
+#pragma unroll
+                for (int h = 0; h < DPT; h++) {
+                    const int e = tid + h * NT, i = e % SOLVE_NB, j = e / SOLVE_NB;
+                    dpre[h] = (i < bs && j < bs) ? F[(b0 + i) + (int64_t)(b0 + j) * nf] : T(0);
+                }
+            }
+#pragma unroll
+            for (int h = 0; h < DPT; h++) {
+                const int e = tid + h * NT;
+                sd[(e % SOLVE_NB) + (e / SOLVE_NB) * LDD] = dpre[h];
             }
             __syncthreads();
             if (tid < 64) {
                 T v[SOLVE_RC];
 #pragma unroll
                 for (int r = 0; r < SOLVE_RC; r++) v[r] = lane < bs ? g[r * MAXNF + b0 + lane] : T(0);
-                for (int j = 0; j < bs; j++) {
-                    const T l = lane > j ? sd[lane + j * LDD] : T(0);
-#pragma unroll
-                    for (int r = 0; r < SOLVE_RC; r++) v[r] -= l * __shfl(v[r], j);
-                }
+                wave_lower_solve<T>(sd, LDD, bs, lane, v);
                 if (lane < bs) {
 #pragma unroll
                     for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + b0 + lane] = v[r];
@@ -598,10 +681,10 @@ __global__ __launch_bounds__(256) void k_fwd_front_b(SolveArgs a, const int32_t 
             }
             __syncthreads();
             for (int p = b0 + bs + tid; p < nf; p += NT) {
-                T acc[SOLVE_RC] = {};
-                dot_rows<T, 16>(F + p + (int64_t)b0 * nf, nf, bs, g + b0, MAXNF, acc);
+                T acc1[SOLVE_RC] = {};
+                dot_rows<T, 32>(F + p + (int64_t)b0 * nf, nf, bs, g + b0, MAXNF, acc1);
 #pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + p] -= acc[r];
+                for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + p] -= acc1[r];
             }
             __syncthreads();
         }
@@ -641,13 +724,31 @@ __global__ __launch_bounds__(256) void k_bwd_front_b(SolveArgs a)
     int npad = 1;
     while (npad < np && npad < NT) npad <<= 1;
     const int ts_n = NT / npad, ti = tid % npad, ts = tid / npad;
+    constexpr int PT = MAXNF / NT;
+    int64_t srcs[PT];
+#pragma unroll
+    for (int k = 0; k < PT; k++) {
+        const int p = tid + k * NT;
+        srcs[k] = p < np ? c1 + p : (p < nf ? upd[p - np] : 0);
+    }
     for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
-        for (int p = tid; p < nf; p += NT) {
-            const int64_t src = p < np ? c1 + p : upd[p - np];
+        // s = y of the pivot rows, x at the update columns: every position of the thread at once
+        T gv[PT][SOLVE_RC];
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            const int p = tid + k * NT;
 #pragma unroll
             for (int r = 0; r < SOLVE_RC; r++)
-                g[r * MAXNF + p] = r < rc ? (p < np ? y : x)[(int64_t)(r0 + r) * a.n + src] : T(0);
+                gv[k][r] = (r < rc && p < nf) ? (p < np ? y : x)[(int64_t)(r0 + r) * a.n + srcs[k]] : T(0);
+        }
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            const int p = tid + k * NT;
+            if (p < nf) {
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + p] = gv[k][r];
+            }
         }
         __syncthreads();
         for (int i0 = 0; i0 < np; i0 += npad) {
@@ -657,12 +758,21 @@ __global__ __launch_bounds__(256) void k_bwd_front_b(SolveArgs a)
                 const T *row = F + i + (int64_t)(np + ts) * nf;
                 const int cnt = nu > ts ? (nu - ts + ts_n - 1) / ts_n : 0;   // columns ts + k * ts_n
                 int k = 0;
-                for (; k + 8 <= cnt; k += 8) {
-                    T u[8];
+                for (; k + 16 <= cnt; k += 16) {
+                    T u[16];
 #pragma unroll
-                    for (int q = 0; q < 8; q++) u[q] = row[(int64_t)(k + q) * ts_n * nf];
+                    for (int q = 0; q < 16; q++) u[q] = row[(int64_t)(k + q) * ts_n * nf];
 #pragma unroll
-                    for (int q = 0; q < 8; q++)
+                    for (int q = 0; q < 16; q++)
+#pragma unroll
+                        for (int r = 0; r < SOLVE_RC; r++) acc[r] += u[q] * g[r * MAXNF + np + ts + (k + q) * ts_n];
+                }
+                for (; k + 4 <= cnt; k += 4) {
+                    T u[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) u[q] = row[(int64_t)(k + q) * ts_n * nf];
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
 #pragma unroll
                         for (int r = 0; r < SOLVE_RC; r++) acc[r] += u[q] * g[r * MAXNF + np + ts + (k + q) * ts_n];
                 }
@@ -700,15 +810,7 @@ __global__ __launch_bounds__(256) void k_bwd_front_b(SolveArgs a)
                 T v[SOLVE_RC];
 #pragma unroll
                 for (int r = 0; r < SOLVE_RC; r++) v[r] = lane < bs ? g[r * MAXNF + b0 + lane] : T(0);
-                for (int j = bs - 1; j >= 0; j--) {
-                    const T rjj = T(1) / sd[j + j * LDD];   // one division per step
-                    const T u = lane < j ? sd[lane + j * LDD] : T(0);
-#pragma unroll
-                    for (int r = 0; r < SOLVE_RC; r++) {
-                        if (lane == j) v[r] = v[r] * rjj;
-                        v[r] -= u * __shfl(v[r], j);
-                    }
-                }
+                wave_upper_solve<T>(sd, LDD, bs, lane, v);
                 if (lane < bs) {
 #pragma unroll
                     for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + b0 + lane] = v[r];
@@ -944,20 +1046,20 @@ hipError_t launch_solve_level(int which, const SolveArgs &a, const int32_t *list
     case 7: hipLaunchKernelGGL(k_rect_sum<T>, g, blk, 0, st, b, 2); break;
     case 8: hipLaunchKernelGGL(k_rect_part<T>, g, blk, 0, st, b, 3); break;
     case 9: hipLaunchKernelGGL(k_rect_sum<T>, g, blk, 0, st, b, 3); break;
-    case 10: hipLaunchKernelGGL(k_fwd_gather<T>, g, blk, 0, st, b, a.ecolptr, a.ext); break;
+    case 10: hipLaunchKernelGGL(k_fwd_gather<T>, g, blk, 0, st, b, a.ecolptr, a.eupd); break;
     case 11:
     case 13: {
         const int64_t items = (int64_t)count * nmat * nz;
         const unsigned wg = (unsigned)std::min<int64_t>((items + 3) / 4, 8192);
         if (which == 11)
-            hipLaunchKernelGGL(k_fwd_front_w<T>, dim3(wg), blk, 0, st, b, a.ecolptr, a.ext, count, nmat, nz);
+            hipLaunchKernelGGL(k_fwd_front_w<T>, dim3(wg), blk, 0, st, b, a.ecolptr, a.eupd, count, nmat, nz);
         else
             hipLaunchKernelGGL(k_bwd_front_w<T>, dim3(wg), blk, 0, st, b, count, nmat, nz);
         break;
     }
-    case 12: hipLaunchKernelGGL((k_fwd_front_b<T, FRONT_MAXNF>), g, blk, 0, st, b, a.ecolptr, a.ext); break;
+    case 12: hipLaunchKernelGGL((k_fwd_front_b<T, FRONT_MAXNF>), g, blk, 0, st, b, a.ecolptr, a.eupd); break;
     case 14: hipLaunchKernelGGL((k_bwd_front_b<T, FRONT_MAXNF>), g, blk, 0, st, b); break;
-    case 15: hipLaunchKernelGGL((k_fwd_front_b<T, 256>), g, blk, 0, st, b, a.ecolptr, a.ext); break;
+    case 15: hipLaunchKernelGGL((k_fwd_front_b<T, 256>), g, blk, 0, st, b, a.ecolptr, a.eupd); break;
     case 16: hipLaunchKernelGGL((k_bwd_front_b<T, 256>), g, blk, 0, st, b); break;
     default: return hipErrorInvalidValue;
     }
