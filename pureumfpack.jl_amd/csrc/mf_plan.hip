@@ -956,8 +956,11 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                         emit4((int32_t)s, (int32_t)r0, (int32_t)c0, (int32_t)(slot + nch));
                         nch++;
                     }
-                    sums.push_back((int32_t)s); sums.push_back((int32_t)r0); sums.push_back((int32_t)nch);
-                    sums.push_back((int32_t)slot);
+                    const int64_t dst = (dir == 0 ? uoff0[s] : c1[s]) + r0;
+                    const int64_t rows = std::min<int64_t>(256, nrows - r0);
+                    const int64_t to_zb = dir == 1 && ioff[s] >= 0 ? 1 : 0;
+                    sums.push_back((int32_t)dst); sums.push_back((int32_t)(rows | to_zb << 16));
+                    sums.push_back((int32_t)nch); sums.push_back((int32_t)slot);
                     slot += nch;
                 }
             }

@@ -27,42 +27,6 @@ hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st)
 constexpr int SOLVE_RC = 8;   // right-hand sides processed together per pass
 constexpr int SOLVE_NB = 64;  // triangular block size (one wave solves a diagonal block)
 
-// v[(r0 + r) * vld + i] -= sum_{j < bs} M[i + j * ld] * s[r * SOLVE_NB + j] for rows
-// i in [rbeg, rend), two rows per thread in flight.
-template <typename T>
-__device__ __forceinline__ void rows_update(const T *M, int ld, int bs, int rbeg, int rend, const T *s, T *v,
-                                            int64_t vld, int rc, int r0)
-{
-    const int nt = blockDim.x;
-    for (int i0 = rbeg + threadIdx.x; i0 < rend; i0 += 2 * nt) {
-        const int i1 = i0 + nt;
-        T acc0[SOLVE_RC] = {}, acc1[SOLVE_RC] = {};
-        for (int j0 = 0; j0 < bs; j0 += 16) {
-            T l0[16], l1[16];
-#pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const bool ok = j0 + q < bs;
-                l0[q] = ok ? M[i0 + (int64_t)(j0 + q) * ld] : T(0);
-                l1[q] = (ok && i1 < rend) ? M[i1 + (int64_t)(j0 + q) * ld] : T(0);
-            }
-#pragma unroll
-            for (int q = 0; q < 16; q++)
-#pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++) {
-                    const T sv = s[r * SOLVE_NB + j0 + q];
-                    acc0[r] += l0[q] * sv;
-                    acc1[r] += l1[q] * sv;
-                }
-        }
-#pragma unroll
-        for (int r = 0; r < SOLVE_RC; r++)
-            if (r < rc) {
-                v[(r0 + r) * vld + i0] -= acc0[r];
-                if (i1 < rend) v[(r0 + r) * vld + i1] -= acc1[r];
-            }
-    }
-}
-
 // One wave: v <- L^-1 v for the unit lower bs x bs block staged in sd (leading dimension
 // ldd, lane = row; rows and columns >= bs of sd and v are zero).  Columns in groups of 8,
 // read from LDS before their 8 steps, without branches: a step past bs has a zero column and
@@ -151,64 +115,107 @@ __global__ __launch_bounds__(256) void k_fwd_gather(SolveArgs a, const int32_t *
     }
 }
 
-// Forward sweep, part 1 (one workgroup per supernode of the level): permute the gathered
-// pivot part by the block pivoting into y (factor order), then solve with the unit lower L11
-// block by block.  The L21 product that finishes the update vector is k_rect_part kind 0.
-// Fronts with stored inverses skip this: k_rect_part kind 2 reads the gathered vector
-// through the pivoting itself.
+// Forward sweep, part 1, fronts without stored inverses (np <= INV_NP, two diagonal blocks at
+// most): one workgroup per front permutes the gathered pivot part by the block pivoting into
+// factor order (in LDS), solves with the unit lower L11 and writes y.  The L21 product that
+// finishes the update vector is k_rect_part kind 0.  Fronts with inverses skip this:
+// k_rect_part kind 2 reads the gathered vector through the pivoting itself.  Every factor
+// value the workgroup needs is loaded before the vector arrives.
+constexpr int TRI_NP = INV_NP;
+static_assert(TRI_NP == 2 * SOLVE_NB, "the triangle kernels handle two diagonal blocks");
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_tri(SolveArgs a)
 {
-    __shared__ T sblk[SOLVE_NB * SOLVE_RC];
-    __shared__ T sdiag[SOLVE_NB * (SOLVE_NB + 1)];
+    constexpr int NT = 256, LDD = SOLVE_NB + 1, DPT = SOLVE_NB * SOLVE_NB / NT;
+    __shared__ T g[SOLVE_RC * TRI_NP];
+    __shared__ T sd[SOLVE_NB * LDD];
+    __shared__ T red[SOLVE_RC * NT];
     const int mat = blockIdx.y;
     const SnDev d = a.sn[a.list[blockIdx.x]];
+    if (d.ioff >= 0) return;   // large front: inv(L11) is applied by k_rect_part kind 2
     const int nf = d.nf, np = d.np, c1 = d.c1;
     const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
     const int32_t *prow = a.prow + mat * a.n_stride;
-    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
+    const int tid = threadIdx.x, lane = tid & 63;
     const int64_t vb = (int64_t)mat * a.nrhs;
-    T *w0 = reinterpret_cast<T *>(a.w0) + vb * a.n;
+    const T *w0 = reinterpret_cast<const T *>(a.w0) + vb * a.n;
     T *y = reinterpret_cast<T *>(a.y) + vb * a.n;
     const int nrhs = (int)a.nrhs;
-    if (d.ioff >= 0) return;   // large front: inv(L11) is applied by k_rect_part kind 2
-    for (int i = tid; i < np; i += nt) {
-        const int64_t src = prow[c1 + i];
-        for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
-            T v[SOLVE_RC];
+    const int bs0 = min(SOLVE_NB, np), bs1 = np - bs0;   // bs1 > 0: a second block
+    // diagonal blocks of L11 and the block L11[64 + ur, 16 uq .. 16 uq + 15] below the first
+    T d0[DPT], d1[DPT], lr[16];
+    const int ur = tid & 63, uq = tid >> 6;
 #pragma unroll
-            for (int r = 0; r < SOLVE_RC; r++) v[r] = r0 + r < nrhs ? w0[(int64_t)(r0 + r) * a.n + src] : T(0);
-#pragma unroll
-            for (int r = 0; r < SOLVE_RC; r++)
-                if (r0 + r < nrhs) y[(int64_t)(r0 + r) * a.n + c1 + i] = v[r];
-        }
+    for (int h = 0; h < DPT; h++) {
+        const int e = tid + h * NT, i = e % SOLVE_NB, j = e / SOLVE_NB;
+        d0[h] = (i < bs0 && j < bs0) ? F[i + (int64_t)j * nf] : T(0);
+        d1[h] = (i < bs1 && j < bs1) ? F[(SOLVE_NB + i) + (int64_t)(SOLVE_NB + j) * nf] : T(0);
     }
-    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+        lr[q] = ur < bs1 ? F[(SOLVE_NB + ur) + (int64_t)(uq * 16 + q) * nf] : T(0);
+    const int64_t src = tid < np ? prow[c1 + tid] : 0;
     for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
-        for (int b0 = 0; b0 < np; b0 += SOLVE_NB) {
-            const int bs = min(SOLVE_NB, np - b0);
-            for (int e = tid; e < SOLVE_NB * SOLVE_NB; e += nt) {
-                const int i = e % SOLVE_NB, j = e / SOLVE_NB;
-                sdiag[i + j * (SOLVE_NB + 1)] = (i < bs && j < bs) ? F[(b0 + i) + (int64_t)(b0 + j) * nf] : T(0);
+        if (tid < np) {
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) g[r * TRI_NP + tid] = r < rc ? w0[(int64_t)(r0 + r) * a.n + src] : T(0);
+        }
+#pragma unroll
+        for (int h = 0; h < DPT; h++) {
+            const int e = tid + h * NT;
+            sd[(e % SOLVE_NB) + (e / SOLVE_NB) * LDD] = d0[h];
+        }
+        __syncthreads();
+        if (tid < 64) {
+            T v[SOLVE_RC];
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) v[r] = lane < bs0 ? g[r * TRI_NP + lane] : T(0);
+            wave_lower_solve<T>(sd, LDD, bs0, lane, v);
+            if (lane < bs0) {
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) g[r * TRI_NP + lane] = v[r];
+            }
+        }
+        __syncthreads();
+        if (bs1 > 0) {
+            // rows 64.. -= L11[64.., 0:64] x_0, four partial sums per row added in order
+            T acc[SOLVE_RC] = {};
+#pragma unroll
+            for (int q = 0; q < 16; q++)
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) acc[r] += lr[q] * g[r * TRI_NP + uq * 16 + q];
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) red[r * NT + tid] = acc[r];
+#pragma unroll
+            for (int h = 0; h < DPT; h++) {
+                const int e = tid + h * NT;
+                sd[(e % SOLVE_NB) + (e / SOLVE_NB) * LDD] = d1[h];
             }
             __syncthreads();
             if (tid < 64) {
                 T v[SOLVE_RC];
 #pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++)
-                    v[r] = (r < rc && lane < bs) ? y[(r0 + r) * a.n + c1 + b0 + lane] : T(0);
-                wave_lower_solve<T>(sdiag, SOLVE_NB + 1, bs, lane, v);
-#pragma unroll
                 for (int r = 0; r < SOLVE_RC; r++) {
-                    sblk[r * SOLVE_NB + lane] = v[r];
-                    if (r < rc && lane < bs) y[(r0 + r) * a.n + c1 + b0 + lane] = v[r];
+                    const T sum = ((red[r * NT + lane] + red[r * NT + 64 + lane]) + red[r * NT + 128 + lane]) +
+                                  red[r * NT + 192 + lane];
+                    v[r] = lane < bs1 ? g[r * TRI_NP + SOLVE_NB + lane] - sum : T(0);
+                }
+                wave_lower_solve<T>(sd, LDD, bs1, lane, v);
+                if (lane < bs1) {
+#pragma unroll
+                    for (int r = 0; r < SOLVE_RC; r++) g[r * TRI_NP + SOLVE_NB + lane] = v[r];
                 }
             }
             __syncthreads();
-            rows_update<T>(F + (int64_t)b0 * nf, nf, bs, b0 + bs, np, sblk, y + c1, a.n, rc, r0);
-            __syncthreads();
         }
+        if (tid < np) {
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++)
+                if (r < rc) y[(int64_t)(r0 + r) * a.n + c1 + tid] = g[r * TRI_NP + tid];
+        }
+        __syncthreads();
     }
 }
 
@@ -220,9 +227,10 @@ __global__ __launch_bounds__(256) void k_fwd_tri(SolveArgs a)
 //   kind 2: y[c1+i] = sum_a inv(L11)[i, a] w0[prow[c1 + a]]     lower-triangle chunks only
 //   kind 3: x[c1+i] = sum_a inv(U11)[i, a] zbuf[c1 + a]         upper-triangle chunks only
 // k_rect_part items are int4 {front, row0, col0, slot}: each workgroup writes its partial sums
-// (256 rows x nrhs) to partial slot `slot`.  k_rect_sum items are {front, row0, nchunks,
-// slot0}: the row block's chunks own slots slot0 .. slot0 + nchunks - 1, added in chunk order
-// (the result does not depend on scheduling) and applied.
+// (256 rows x nrhs) to partial slot `slot`.  k_rect_sum items are {dst, rows | zb << 16,
+// nchunks, slot0}: the row block's chunks own slots slot0 .. slot0 + nchunks - 1, added in
+// chunk order (the result does not depend on scheduling) and applied to `rows` entries from
+// dst (c1 + row0, or upd_off + row0 for kind 0; zb: kind 1 writes zbuf).
 constexpr int SOLVE_CK = 64;
 
 // acc[r] += sum_{j < bs} M[j * ld] * s[r * sld + j], G loads in flight
@@ -311,11 +319,9 @@ __global__ __launch_bounds__(256) void k_rect_sum(SolveArgs a, int kind)
 {
     const int mat = blockIdx.y;
     const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
-    const SnDev d = a.sn[w.x];
-    const int np = d.np, nu = d.nu, c1 = d.c1;
-    const int row = w.y + threadIdx.x;
-    const int nrows = kind == 0 ? nu : np;
-    if (row >= nrows) return;
+    if ((int)threadIdx.x >= (w.y & 0xffff)) return;
+    const int64_t dst = (int64_t)w.x + threadIdx.x;
+    const bool to_zb = (w.y >> 16) != 0;
     const int nch = w.z;
     const int64_t vb = (int64_t)mat * a.nrhs;
     T *y = reinterpret_cast<T *>(a.y) + vb * a.n;
@@ -350,10 +356,10 @@ __global__ __launch_bounds__(256) void k_rect_sum(SolveArgs a, int kind)
 #pragma unroll
         for (int r = 0; r < SOLVE_RC; r++) {
             if (r >= rc) continue;
-            const int64_t ro = (int64_t)(r0 + r) * a.n + c1 + row;
+            const int64_t ro = (int64_t)(r0 + r) * a.n + dst;
             switch (kind) {
-            case 0: uv[(int64_t)(r0 + r) * a.nupd + d.upd_off + row] -= s[r]; break;
-            case 1: (d.ioff >= 0 ? zb : x)[ro] = y[ro] - s[r]; break;   // inv(U11) next
+            case 0: uv[(int64_t)(r0 + r) * a.nupd + dst] -= s[r]; break;
+            case 1: (to_zb ? zb : x)[ro] = y[ro] - s[r]; break;   // inv(U11) next
             case 2: y[ro] = s[r]; break;
             default: x[ro] = s[r]; break;
             }
@@ -361,46 +367,101 @@ __global__ __launch_bounds__(256) void k_rect_sum(SolveArgs a, int kind)
     }
 }
 
-// Backward sweep, part 2 (one workgroup per supernode): x_s <- U11^-1 x_s, bottom block first.
+// Backward sweep, part 2, fronts without stored inverses (np <= TRI_NP): x_s <- U11^-1 x_s,
+// bottom diagonal block first, in LDS; the factor values are loaded before the vector.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_tri(SolveArgs a)
 {
-    __shared__ T sblk[SOLVE_NB * SOLVE_RC];
-    __shared__ T sdiag[SOLVE_NB * (SOLVE_NB + 1)];
+    constexpr int NT = 256, LDD = SOLVE_NB + 1, DPT = SOLVE_NB * SOLVE_NB / NT;
+    __shared__ T g[SOLVE_RC * TRI_NP];
+    __shared__ T sd[SOLVE_NB * LDD];
+    __shared__ T red[SOLVE_RC * NT];
     const int mat = blockIdx.y;
     const SnDev d = a.sn[a.list[blockIdx.x]];
     if (d.ioff >= 0) return;  // large front: inv(U11) was applied by k_rect_part kind 3
     const int nf = d.nf, np = d.np, c1 = d.c1;
     const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
-    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
+    const int tid = threadIdx.x, lane = tid & 63;
     const int64_t vb = (int64_t)mat * a.nrhs;
     T *x = reinterpret_cast<T *>(a.x) + vb * a.n;
     const int nrhs = (int)a.nrhs;
+    const int bs0 = min(SOLVE_NB, np), bs1 = np - bs0;
+    // diagonal blocks of U11 and U11[ur, 64 + 16 uq .. 64 + 16 uq + 15] above the second
+    T d0[DPT], d1[DPT], ur16[16];
+    const int ur = tid & 63, uq = tid >> 6;
+#pragma unroll
+    for (int h = 0; h < DPT; h++) {
+        const int e = tid + h * NT, i = e % SOLVE_NB, j = e / SOLVE_NB;
+        d0[h] = (i < bs0 && j < bs0) ? F[i + (int64_t)j * nf] : T(0);
+        d1[h] = (i < bs1 && j < bs1) ? F[(SOLVE_NB + i) + (int64_t)(SOLVE_NB + j) * nf] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int c = uq * 16 + q;
+        ur16[q] = (ur < bs0 && c < bs1) ? F[ur + (int64_t)(SOLVE_NB + c) * nf] : T(0);
+    }
     for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
-        for (int b0 = ((np - 1) / SOLVE_NB) * SOLVE_NB; b0 >= 0; b0 -= SOLVE_NB) {
-            const int bs = min(SOLVE_NB, np - b0);
-            for (int e = tid; e < SOLVE_NB * SOLVE_NB; e += nt) {
-                const int i = e % SOLVE_NB, j = e / SOLVE_NB;
-                sdiag[i + j * (SOLVE_NB + 1)] = (i < bs && j < bs) ? F[(b0 + i) + (int64_t)(b0 + j) * nf] : T(0);
+        if (tid < np) {
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) g[r * TRI_NP + tid] = r < rc ? x[(int64_t)(r0 + r) * a.n + c1 + tid] : T(0);
+        }
+        if (bs1 > 0) {
+#pragma unroll
+            for (int h = 0; h < DPT; h++) {
+                const int e = tid + h * NT;
+                sd[(e % SOLVE_NB) + (e / SOLVE_NB) * LDD] = d1[h];
             }
             __syncthreads();
             if (tid < 64) {
                 T v[SOLVE_RC];
 #pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++)
-                    v[r] = (r < rc && lane < bs) ? x[(r0 + r) * a.n + c1 + b0 + lane] : T(0);
-                wave_upper_solve<T>(sdiag, SOLVE_NB + 1, bs, lane, v);
+                for (int r = 0; r < SOLVE_RC; r++) v[r] = lane < bs1 ? g[r * TRI_NP + SOLVE_NB + lane] : T(0);
+                wave_upper_solve<T>(sd, LDD, bs1, lane, v);
+                if (lane < bs1) {
 #pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++) {
-                    sblk[r * SOLVE_NB + lane] = v[r];
-                    if (r < rc && lane < bs) x[(r0 + r) * a.n + c1 + b0 + lane] = v[r];
+                    for (int r = 0; r < SOLVE_RC; r++) g[r * TRI_NP + SOLVE_NB + lane] = v[r];
                 }
             }
             __syncthreads();
-            rows_update<T>(F + (int64_t)b0 * nf, nf, bs, 0, b0, sblk, x + c1, a.n, rc, r0);
-            __syncthreads();
+            // rows 0..63 -= U11[0:64, 64..] x_1, four partial sums per row added in order
+            T acc[SOLVE_RC] = {};
+#pragma unroll
+            for (int q = 0; q < 16; q++)
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) acc[r] += ur16[q] * g[r * TRI_NP + SOLVE_NB + uq * 16 + q];
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) red[r * NT + tid] = acc[r];
         }
+#pragma unroll
+        for (int h = 0; h < DPT; h++) {
+            const int e = tid + h * NT;
+            sd[(e % SOLVE_NB) + (e / SOLVE_NB) * LDD] = d0[h];
+        }
+        __syncthreads();
+        if (tid < 64) {
+            T v[SOLVE_RC];
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) {
+                T val = lane < bs0 ? g[r * TRI_NP + lane] : T(0);
+                if (bs1 > 0 && lane < bs0)
+                    val -= ((red[r * NT + lane] + red[r * NT + 64 + lane]) + red[r * NT + 128 + lane]) +
+                           red[r * NT + 192 + lane];
+                v[r] = val;
+            }
+            wave_upper_solve<T>(sd, LDD, bs0, lane, v);
+            if (lane < bs0) {
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) g[r * TRI_NP + lane] = v[r];
+            }
+        }
+        __syncthreads();
+        if (tid < np) {
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++)
+                if (r < rc) x[(int64_t)(r0 + r) * a.n + c1 + tid] = g[r * TRI_NP + tid];
+        }
+        __syncthreads();
     }
 }
 
