@@ -421,14 +421,14 @@ __device__ __forceinline__ void spec_diag_steps(T (&x)[KB], int lane, int kb, T 
 }
 
 template <typename T, int KB>
-__device__ __forceinline__ void spec_diag_front(const FactorArgs &a, int k, float tau, int slot, T *sblk)
+__device__ __forceinline__ void spec_diag_front(const FactorArgs &a, int k, float tau, int slot)
 {
     // One wave: lane i owns row i of the block in registers; step j reads the pivot row from
     // lane j with v_readlane (no memory round trip).  Control flow depends only on the pivot,
     // which is uniform: rows a step must not change (finished rows, the pivot row) get
-    // multiplier 0.  The factored block is also written, packed for the trsm, to this front's
-    // slot of a.spec: U by rows, L by columns, the reciprocal pivots and a flag for a zero or
-    // tiny pivot (which getf2 treats apart: no scaling, or a division).
+    // multiplier 0.  The factored block goes back to the front, where the trsm reads it; this
+    // front's slot of a.spec takes the reciprocal pivots and a flag for a zero or tiny pivot
+    // (which getf2 treats apart: no scaling, or a division).
     const int mat = blockIdx.y;
     const SnDev d = a.sn[a.list[slot]];
     const int np = d.np, nf = d.nf;
@@ -463,28 +463,14 @@ __device__ __forceinline__ void spec_diag_front(const FactorArgs &a, int k, floa
         for (int c = 0; c < KB; c++)
             if (c < kb) F[(k + lane) + (int64_t)(k + c) * nf] = x[c];
     }
-    // packed copy through LDS (row i at sblk[i * (KB + 1)]: conflict-free both ways)
-#pragma unroll
-    for (int c = 0; c < KB; c++) sblk[lane * (KB + 1) + c] = x[c];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int j = 0; j < KB; j++) {
-        // row j of U (columns lane >= j) and column j of L (rows lane > j)
-        const T u = (lane >= j && lane < kb && j < kb) ? sblk[j * (KB + 1) + lane] : T(0);
-        const T l = (lane > j && lane < kb && j < kb) ? sblk[lane * (KB + 1) + j] : T(0);
-        S[j * KB + lane] = u;
-        S[KB * KB + j * KB + lane] = l;
-    }
-    S[2 * KB * KB + lane] = lane < kb ? myrcp : T(1);
-    if (lane == 0) S[2 * KB * KB + KB] = special ? T(1) : T(0);
+    S[lane] = lane < kb ? myrcp : T(1);
+    if (lane == 0) S[KB] = special ? T(1) : T(0);
 }
 
 template <typename T, int KB>
 __global__ __launch_bounds__(KB) void k_spec_diag(FactorArgs a, int k, float tau)
 {
-    __shared__ T sblk[KB * (KB + 1)];
-    spec_diag_front<T, KB>(a, k, tau, blockIdx.x, sblk);
+    spec_diag_front<T, KB>(a, k, tau, blockIdx.x);
 }
 
 
@@ -511,6 +497,7 @@ template <typename T, int KB, bool LROWS, bool SPECIAL>
 __device__ __forceinline__ int spec_solve_group(T (&x)[KB / 4], const T *M, const T *rc, int g, int gbase,
                                                 bool cand, T itau)
 {
+    constexpr int LDM = KB + 1;   // M[j][c] at M[j * LDM + c]
     int bad = 0;
 #pragma unroll
     for (int jc = 0; jc < KB / 32; jc++)
@@ -519,7 +506,7 @@ __device__ __forceinline__ int spec_solve_group(T (&x)[KB / 4], const T *M, cons
         const int j = 32 * jc + jj, owner = j & 3, tj = j >> 2;
         T v = x[tj];
         if (LROWS) {
-            const T u = M[j * KB + j];
+            const T u = M[j * LDM + j];
             bad |= (int)(g == owner && cand && fabs(v) > fabs(u) * itau);
             if (SPECIAL) {
                 if (u == T(0)) v = cand ? v : v * rc[j];
@@ -535,7 +522,7 @@ __device__ __forceinline__ int spec_solve_group(T (&x)[KB / 4], const T *M, cons
         for (int t = 0; t < KB / 4; t++)
             if (4 * t + 3 > j) {
                 const int c = 4 * t + g;
-                if (c > j) x[t] -= v * M[j * KB + c];
+                if (c > j) x[t] -= v * M[j * LDM + c];
             }
     }
     return bad;
@@ -546,9 +533,12 @@ __global__ __launch_bounds__(4 * SPEC_ROWS) void k_spec_trsm(FactorArgs a, int k
                                                              const T *__restrict__ spec)
 {
     // SPEC_ROWS rows (kind 0) or columns (kind 1) per workgroup, 4 lanes each (see
-    // spec_solve_group), M[j][c] = U(j, c) for kind 0 and L(c, j) for kind 1 staged in LDS
-    // from the packed block k_spec_diag left in this front's slot.
-    __shared__ T sm[KB * KB + 2 * KB];
+    // spec_solve_group), M[j][c] = U(j, c) for kind 0 and L(c, j) for kind 1 (zero outside the
+    // triangle) staged in LDS (row stride KB + 1) from the factored diagonal block in the
+    // front, read down its columns; the reciprocal pivots and the special-pivot flag come
+    // from this front's slot of a.spec.
+    constexpr int LDM = KB + 1;
+    __shared__ T sm[KB * LDM + 2 * KB];
     const int mat = blockIdx.y;
     const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
     const SnDev d = a.sn[w.x];
@@ -557,12 +547,19 @@ __global__ __launch_bounds__(4 * SPEC_ROWS) void k_spec_trsm(FactorArgs a, int k
     const bool lrows = w.y == 0;
     {
         const T *S = spec + mat * a.spec_stride + (int64_t)w.w * SPEC_SLOT;
-        const T *Mg = S + (lrows ? 0 : KB * KB);
-        for (int e = threadIdx.x; e < KB * KB; e += 4 * SPEC_ROWS) sm[e] = Mg[e];
-        for (int e = threadIdx.x; e < 2 * KB; e += 4 * SPEC_ROWS) sm[KB * KB + e] = S[2 * KB * KB + e];
+        const T *Fd = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff + k + (int64_t)k * nf;
+        for (int e = threadIdx.x; e < KB * KB; e += 4 * SPEC_ROWS) {
+            const int i = e % KB, jj = e / KB;   // block entry (i, jj)
+            const T v = (i < kb && jj < kb) ? Fd[i + (int64_t)jj * nf] : T(0);
+            if (lrows)
+                sm[i * LDM + jj] = jj >= i ? v : T(0);   // U(i, jj)
+            else
+                sm[jj * LDM + i] = i > jj ? v : T(0);    // L(i, jj)
+        }
+        for (int e = threadIdx.x; e < 2 * KB; e += 4 * SPEC_ROWS) sm[KB * LDM + e] = S[e];
     }
     __syncthreads();
-    const T *rc = sm + KB * KB;
+    const T *rc = sm + KB * LDM;
     const int g = threadIdx.x & 3, gbase = threadIdx.x & 60;   // lane of the group, group's first lane in the wave
     const int idx = k + kb + w.z + (threadIdx.x >> 2);        // row (kind 0) or column (kind 1) in the front
     const bool on = idx < nf;
@@ -946,14 +943,13 @@ __global__ __launch_bounds__(256) void k_spec_diag_gemm(FactorArgs a, int k, flo
                                                         const GemmProb *__restrict__ probs,
                                                         const GemmTile *__restrict__ tiles, int ndiag)
 {
-    constexpr int NG = gemm_lds_elems<T, 64, 64>(), ND = KB * (KB + 1);
-    __shared__ T lds[NG > ND ? NG : ND];
+    __shared__ T lds[gemm_lds_elems<T, 64, 64>()];
     if ((int)blockIdx.x >= ndiag) {
         gemm_tile<T, 64, 64>(probs, tiles[blockIdx.x - ndiag], a.pool, a.pool_stride, lds);
         return;
     }
     if (threadIdx.x >= KB) return;
-    spec_diag_front<T, KB>(a, k, tau, (int)blockIdx.x, lds);
+    spec_diag_front<T, KB>(a, k, tau, (int)blockIdx.x);
 }
 
 template <typename T>

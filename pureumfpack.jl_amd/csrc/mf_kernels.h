@@ -18,9 +18,9 @@ namespace mf {
 constexpr int PANEL_KB = 16;   // panel width of the large-front getrf (pivoted schedule)
 constexpr int SPEC_KB = 64;    // block width of the speculative (interchange-free) schedule
 constexpr int SPEC_ROWS = 64;  // rows (or columns) per workgroup of the speculative trsm
-// per front of a speculative diagonal launch: U_kk by rows, L_kk by columns, 1/u_jj, a flag
-// for zero or tiny pivots (read by the trsm through the scalar cache)
-constexpr int SPEC_SLOT = 2 * SPEC_KB * SPEC_KB + 2 * SPEC_KB;
+// per front of a speculative diagonal launch: 1/u_jj and a flag for zero or tiny pivots (the
+// trsm reads the factored block itself from the front)
+constexpr int SPEC_SLOT = 2 * SPEC_KB;
 constexpr int INV_LEAF = 64;   // diagonal block of the recursive pivot-block inversion
 constexpr int INV_NP = 128;    // fronts with np > INV_NP keep inv(L11), inv(U11) for the solves
 constexpr int PINFO_STRIDE = 1 + 4 * PANEL_KB;  // per-front panel permutation record

@@ -21,6 +21,10 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
          # indices); the default pragma threshold stops short of that
          "-mllvm", "-pragma-unroll-threshold=200000",
          "-Wall", "-Wno-unused-result"]
+# The solves' register-resident substitutions broadcast with v_readlane; the default scheduler
+# pairs each readlane with its consumer through one SGPR (a hazard stall per pair), the ILP
+# strategy batches them into distinct SGPRs.
+EXTRA = {"mf_solve_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
 
 
 def build(verbose: bool = False) -> str:
@@ -31,7 +35,7 @@ def build(verbose: bool = False) -> str:
     for src in SOURCES:
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        cmds.append([HIPCC, *FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC, *FLAGS, *EXTRA.get(src, []), *lang, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     # the sources compile independently: run them side by side
     procs = []
