@@ -486,15 +486,40 @@ __device__ __forceinline__ T *F_col_base(const FactorArgs &a, int mat, const SnD
     return reinterpret_cast<T *>(a.pool) + mat * a.pool_stride + d.foff + (int64_t)idx * d.nf + k;
 }
 
+// value of lane o of each group of 4 lanes (a DPP quad_perm broadcast; o a literal)
+template <int O>
+__device__ __forceinline__ float quad_lane(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), O * 0x55, 0xf, 0xf, false));
+}
+template <int O>
+__device__ __forceinline__ double quad_lane(double v)
+{
+    const unsigned long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, O * 0x55, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), O * 0x55, 0xf, 0xf, false);
+    return __longlong_as_double(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+template <typename T>
+__device__ __forceinline__ T quad_lane(T v, int o)
+{
+    switch (o) {   // o is a constant in the unrolled steps
+    case 0: return quad_lane<0>(v);
+    case 1: return quad_lane<1>(v);
+    case 2: return quad_lane<2>(v);
+    default: return quad_lane<3>(v);
+    }
+}
+
 // Right-looking substitution on one row (kind 0) or column (kind 1) of the block, shared by a
 // group of 4 lanes: lane g of the group holds the entries c = 4 t + g (t = 0..15) in x[t].  Step
-// j: the owner lane of column j finishes x_j (kind 0: multiplier check and scaling), shuffles
-// it to its group, and every lane updates its entries c > j with M[j][c] from LDS.  Each x_c
+// j: the owner lane of column j finishes x_j (kind 0: multiplier check and scaling), the group
+// takes it by a DPP broadcast, and every lane updates its entries c > j with M[j][c] from LDS.  Each x_c
 // receives its updates in the order j = 0, 1, ..., as in the row-by-row dot-product form.
 // SPECIAL: some pivot of the block is zero or below the smallest normal number, where getf2
 // leaves the column unscaled or divides instead of scaling by the reciprocal.
 template <typename T, int KB, bool LROWS, bool SPECIAL>
-__device__ __forceinline__ int spec_solve_group(T (&x)[KB / 4], const T *M, const T *rc, int g, int gbase,
+__device__ __forceinline__ int spec_solve_group(T (&x)[KB / 4], const T *M, const T *rc, int g,
                                                 bool cand, T itau)
 {
     constexpr int LDM = KB + 1;   // M[j][c] at M[j * LDM + c]
@@ -517,7 +542,7 @@ __device__ __forceinline__ int spec_solve_group(T (&x)[KB / 4], const T *M, cons
             }
             if (g == owner) x[tj] = v;
         }
-        v = __shfl(v, gbase + owner);
+        v = quad_lane(v, owner);   // from the group's owner lane
 #pragma unroll
         for (int t = 0; t < KB / 4; t++)
             if (4 * t + 3 > j) {
@@ -560,7 +585,7 @@ __global__ __launch_bounds__(4 * SPEC_ROWS) void k_spec_trsm(FactorArgs a, int k
     }
     __syncthreads();
     const T *rc = sm + KB * LDM;
-    const int g = threadIdx.x & 3, gbase = threadIdx.x & 60;   // lane of the group, group's first lane in the wave
+    const int g = threadIdx.x & 3;   // lane of the group
     const int idx = k + kb + w.z + (threadIdx.x >> 2);        // row (kind 0) or column (kind 1) in the front
     const bool on = idx < nf;
     const bool cand = lrows && idx < np;
@@ -577,11 +602,11 @@ __global__ __launch_bounds__(4 * SPEC_ROWS) void k_spec_trsm(FactorArgs a, int k
     }
     int bad = 0;
     if (!lrows)
-        spec_solve_group<T, KB, false, false>(x, sm, rc, g, gbase, false, itau);
+        spec_solve_group<T, KB, false, false>(x, sm, rc, g, false, itau);
     else if (rc[KB] == T(0))
-        bad = spec_solve_group<T, KB, true, false>(x, sm, rc, g, gbase, cand, itau);
+        bad = spec_solve_group<T, KB, true, false>(x, sm, rc, g, cand, itau);
     else
-        bad = spec_solve_group<T, KB, true, true>(x, sm, rc, g, gbase, cand, itau);
+        bad = spec_solve_group<T, KB, true, true>(x, sm, rc, g, cand, itau);
     if (bad) atomicOr(a.flag + mat, 1);
     if (on) {
 #pragma unroll
