@@ -898,32 +898,31 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
         slist.push_back(a0); slist.push_back(a1); slist.push_back(a2); slist.push_back(a3);
     };
     for (int64_t lv = 0; lv < nlev; lv++) {
-        // The fused one-workgroup-per-front sweeps pay off when a level has many fronts.  A
-        // level that needs the split pipeline for a large front anyway (stored inverses or
-        // nf > FRONT_MAXNF), or has only a few fronts beyond one wave, sends all its fronts
-        // through the split pipeline, whose launches spread each front over many workgroups.
+        // The fused one-workgroup-per-front sweeps pay off when a level has many fronts (times
+        // the matrices of a batch).  A level with only a few fronts beyond one wave sends all
+        // its fronts through the split pipeline, whose launches spread each front over many
+        // workgroups; large fronts (stored inverses or nf > FRONT_MAXNF) always take it.
         int64_t nmid = 0;
-        bool split = false;
-        for (int64_t s : bylevel[lv]) {
-            if (ioff[s] >= 0 || nf[s] > FRONT_MAXNF_HOST) split = true;
-            else if (nf[s] > 64) nmid++;
-        }
-        if (nmid > 0 && nmid < SOLVE_FUSE_MIN) split = true;
+        for (int64_t s : bylevel[lv])
+            if (ioff[s] < 0 && nf[s] > 64 && nf[s] <= FRONT_MAXNF_HOST) nmid++;
+        const bool split = nmid > 0 && nmid * std::max(P.nbatch, 1) < SOLVE_FUSE_MIN;
         std::vector<int64_t> big;
+        auto is_big = [&](int64_t s) { return ioff[s] >= 0 || nf[s] > FRONT_MAXNF_HOST; };
         P.lvl_s64o.push_back((int32_t)slist.size());
         for (int64_t s : bylevel[lv])
-            if (!split && nf[s] <= 64) slist.push_back((int32_t)s);
+            if (!split && !is_big(s) && nf[s] <= 64) slist.push_back((int32_t)s);
         P.lvl_s64c.push_back((int32_t)slist.size() - P.lvl_s64o.back());
         P.lvl_s1ko.push_back((int32_t)slist.size());
         int64_t mnf1k = 0;
         for (int64_t s : bylevel[lv])
-            if (!split && nf[s] > 64) {
+            if (!split && !is_big(s) && nf[s] > 64) {
                 slist.push_back((int32_t)s);
                 mnf1k = std::max(mnf1k, nf[s]);
             }
         P.lvl_s1k_small.push_back(mnf1k <= 256 ? 1 : 0);
         P.lvl_s1kc.push_back((int32_t)slist.size() - P.lvl_s1ko.back());
-        if (split) big = bylevel[lv];
+        for (int64_t s : bylevel[lv])
+            if (split || is_big(s)) big.push_back(s);
         P.lvl_off[lv] = (int32_t)slist.size();
         for (int64_t s : big) slist.push_back((int32_t)s);
         P.lvl_cnt[lv] = (int32_t)big.size();

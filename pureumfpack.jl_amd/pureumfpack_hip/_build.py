@@ -21,10 +21,8 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
          # indices); the default pragma threshold stops short of that
          "-mllvm", "-pragma-unroll-threshold=200000",
          "-Wall", "-Wno-unused-result"]
-# The solves' register-resident substitutions broadcast with v_readlane; the default scheduler
-# pairs each readlane with its consumer through one SGPR (a hazard stall per pair), the ILP
-# strategy batches them into distinct SGPRs.
-EXTRA = {"mf_solve_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
+# per-source extra flags
+EXTRA = {}
 
 
 def build(verbose: bool = False) -> str:
