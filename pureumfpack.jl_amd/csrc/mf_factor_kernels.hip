@@ -416,47 +416,93 @@ __device__ __forceinline__ void spec_diag_steps(T (&x)[KB], int lane, int kb, T 
         x[j] = act ? v : x[j];
 #pragma unroll
         for (int c = 0; c < KB; c++)
-            if (c > j) x[c] -= l * lane_value(x[c], j);
+            if (c > j) x[c] = fma(-l, lane_value(x[c], j), x[c]);
+    }
+}
+
+// Four waves on a full block with normal pivots: wave w holds columns 16w .. 16w+15 of every
+// row (lane = row).  Step j: the owner wave (j / 16) takes the pivot from lane j, checks and
+// scales its multipliers, publishes them in LDS (two buffers by step parity, one barrier per
+// step) and updates its own later columns; the waves to its right update all their columns.
+// Every element receives the same updates (explicit fused multiply-adds) in the same order as
+// in spec_diag_steps, so the results are bit-identical to the one-wave form.
+template <typename T, int KB>
+__device__ __forceinline__ void spec_diag_steps4(T (&x)[KB / 4], int lane, int w, T itau, int &bad, int &special,
+                                                 T &myrcp, T *lbuf)
+{
+    constexpr int CW = KB / 4;
+#pragma unroll
+    for (int j = 0; j < KB; j++) {
+        const int ow = j / CW, jl = j % CW;
+        if (w == ow) {
+            const T piv = lane_value(x[jl], j);
+            const bool act = lane > j;
+            T v = x[jl];
+            bad |= (int)(act && fabs(v) > fabs(piv) * itau);
+            const T rp = T(1) / piv;
+            special |= (int)!(fabs(piv) >= Tiny<T>::v());
+            v = v * rp;
+            myrcp = lane == j ? rp : myrcp;
+            const T l = act ? v : T(0);
+            x[jl] = act ? v : x[jl];
+            lbuf[(j & 1) * KB + lane] = l;
+#pragma unroll
+            for (int c = jl + 1; c < CW; c++) x[c] = fma(-l, lane_value(x[c], j), x[c]);
+        }
+        __syncthreads();
+        if (w > ow) {
+            const T l = lbuf[(j & 1) * KB + lane];
+#pragma unroll
+            for (int c = 0; c < CW; c++) x[c] = fma(-l, lane_value(x[c], j), x[c]);
+        }
     }
 }
 
 template <typename T, int KB>
-__device__ __forceinline__ void spec_diag_front(const FactorArgs &a, int k, float tau, int slot)
+__device__ __forceinline__ void spec_diag_front(const FactorArgs &a, int k, float tau, int slot, T *lbuf)
 {
-    // One wave: lane i owns row i of the block in registers; step j reads the pivot row from
-    // lane j with v_readlane (no memory round trip).  Control flow depends only on the pivot,
-    // which is uniform: rows a step must not change (finished rows, the pivot row) get
-    // multiplier 0.  The factored block goes back to the front, where the trsm reads it; this
-    // front's slot of a.spec takes the reciprocal pivots and a flag for a zero or tiny pivot
-    // (which getf2 treats apart: no scaling, or a division).
+    // One block per 256-thread workgroup.  One wave: lane i owns row i of the block in
+    // registers; step j reads the pivot row from lane j with v_readlane (no memory round
+    // trip).  Control flow depends only on the pivot, which is uniform: rows a step must not
+    // change (finished rows, the pivot row) get multiplier 0.  A full block is first factored
+    // by four waves assuming normal pivots (spec_diag_steps4); a zero or tiny pivot, or a
+    // partial block, goes through the one-wave form with getf2's rules.  The factored block
+    // goes back to the front, where the trsm reads it; this front's slot of a.spec takes the
+    // reciprocal pivots and a flag for a zero or tiny pivot (which getf2 treats apart: no
+    // scaling, or a division).
     const int mat = blockIdx.y;
     const SnDev d = a.sn[a.list[slot]];
     const int np = d.np, nf = d.nf;
     const int kb = min(KB, np - k);
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     T *F = reinterpret_cast<T *>(a.pool) + mat * a.pool_stride + d.foff;
     T *S = reinterpret_cast<T *>(a.spec) + mat * a.spec_stride + (int64_t)slot * SPEC_SLOT;
+    const T itau = T(1) / T(tau);
+    if (kb == KB) {
+        constexpr int CW = KB / 4;
+        T xs[CW];
+#pragma unroll
+        for (int c = 0; c < CW; c++) xs[c] = F[(k + lane) + (int64_t)(k + CW * w + c) * nf];
+        int bad = 0, special = 0;
+        T myrcp = T(1);
+        spec_diag_steps4<T, KB>(xs, lane, w, itau, bad, special, myrcp, lbuf);
+        if (!__syncthreads_or(special)) {
+            if (bad) atomicOr(a.flag + mat, 1);
+#pragma unroll
+            for (int c = 0; c < CW; c++) F[(k + lane) + (int64_t)(k + CW * w + c) * nf] = xs[c];
+            if ((lane >> 4) == w) S[lane] = myrcp;   // lane j's pivot belongs to wave j / 16
+            if (threadIdx.x == 0) S[KB] = T(0);
+            return;
+        }
+        // a zero or tiny pivot: wave 0 redoes the block below (F still holds the input)
+    }
+    if (w != 0) return;
     T x[KB];
 #pragma unroll
     for (int c = 0; c < KB; c++) x[c] = (lane < kb && c < kb) ? F[(k + lane) + (int64_t)(k + c) * nf] : T(0);
-    const T itau = T(1) / T(tau);
     int bad = 0, special = 0;
     T myrcp = T(1);
-    if (kb == KB) {
-        // full block, first assuming normal pivots: no branch in the steps, so the compiler
-        // can overlap a step's reciprocal with the previous step's updates
-        spec_diag_steps<T, KB, true>(x, lane, kb, itau, bad, special, myrcp, nullptr, 0);
-        if (special) {
-            // a zero or tiny pivot: redo the block from its input with getf2's rules
-#pragma unroll
-            for (int c = 0; c < KB; c++) x[c] = F[(k + lane) + (int64_t)(k + c) * nf];
-            bad = 0;
-            myrcp = T(1);
-            spec_diag_steps<T, KB, false>(x, lane, kb, itau, bad, special, myrcp, a.info + mat, d.c1 + k);
-        }
-    } else {
-        spec_diag_steps<T, KB, false>(x, lane, kb, itau, bad, special, myrcp, a.info + mat, d.c1 + k);
-    }
+    spec_diag_steps<T, KB, false>(x, lane, kb, itau, bad, special, myrcp, a.info + mat, d.c1 + k);
     if (bad) atomicOr(a.flag + mat, 1);
     if (lane < kb) {
 #pragma unroll
@@ -468,9 +514,10 @@ __device__ __forceinline__ void spec_diag_front(const FactorArgs &a, int k, floa
 }
 
 template <typename T, int KB>
-__global__ __launch_bounds__(KB) void k_spec_diag(FactorArgs a, int k, float tau)
+__global__ __launch_bounds__(256) void k_spec_diag(FactorArgs a, int k, float tau)
 {
-    spec_diag_front<T, KB>(a, k, tau, blockIdx.x);
+    __shared__ T lbuf[2 * KB];
+    spec_diag_front<T, KB>(a, k, tau, blockIdx.x, lbuf);
 }
 
 
@@ -973,8 +1020,7 @@ __global__ __launch_bounds__(256) void k_spec_diag_gemm(FactorArgs a, int k, flo
         gemm_tile<T, 64, 64>(probs, tiles[blockIdx.x - ndiag], a.pool, a.pool_stride, lds);
         return;
     }
-    if (threadIdx.x >= KB) return;
-    spec_diag_front<T, KB>(a, k, tau, (int)blockIdx.x);
+    spec_diag_front<T, KB>(a, k, tau, (int)blockIdx.x, lds);
 }
 
 template <typename T>
@@ -1046,7 +1092,7 @@ hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int
         hipLaunchKernelGGL(k_inv_leaf<T>, grid, dim3(128), 0, st, a);
         break;
     case L_SPEC_DIAG:
-        hipLaunchKernelGGL((k_spec_diag<T, SPEC_KB>), grid, dim3(SPEC_KB), 0, st, a, L.k, L.tau);
+        hipLaunchKernelGGL((k_spec_diag<T, SPEC_KB>), grid, dim3(256), 0, st, a, L.k, L.tau);
         break;
     case L_SPEC_DIAG_GEMM:
         hipLaunchKernelGGL((k_spec_diag_gemm<T, SPEC_KB>), grid, dim3(256), 0, st, a, L.k, L.tau, probs + L.prob_off,
