@@ -617,28 +617,14 @@ __global__ __launch_bounds__(4 * SPEC_ROWS) void k_spec_trsm(FactorArgs a, int k
     const int np = d.np, nf = d.nf;
     const int kb = min(KB, np - k);
     const bool lrows = w.y == 0;
-    {
-        const T *S = spec + mat * a.spec_stride + (int64_t)w.w * SPEC_SLOT;
-        const T *Fd = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff + k + (int64_t)k * nf;
-        for (int e = threadIdx.x; e < KB * KB; e += 4 * SPEC_ROWS) {
-            const int i = e % KB, jj = e / KB;   // block entry (i, jj)
-            const T v = (i < kb && jj < kb) ? Fd[i + (int64_t)jj * nf] : T(0);
-            if (lrows)
-                sm[i * LDM + jj] = jj >= i ? v : T(0);   // U(i, jj)
-            else
-                sm[jj * LDM + i] = i > jj ? v : T(0);    // L(i, jj)
-        }
-        for (int e = threadIdx.x; e < 2 * KB; e += 4 * SPEC_ROWS) sm[KB * LDM + e] = S[e];
-    }
-    __syncthreads();
-    const T *rc = sm + KB * LDM;
     const int g = threadIdx.x & 3;   // lane of the group
     const int idx = k + kb + w.z + (threadIdx.x >> 2);        // row (kind 0) or column (kind 1) in the front
     const bool on = idx < nf;
     const bool cand = lrows && idx < np;
     const T itau = T(1) / T(tau);
     // kind 0 reads row idx across the block columns (stride nf); kind 1 reads column idx
-    // down the block rows (contiguous)
+    // down the block rows (contiguous).  These loads do not depend on the staged block: they
+    // go out first.
     T *base = lrows ? F_row_base<T>(a, mat, d, idx, k) : F_col_base<T>(a, mat, d, idx, k);
     const int64_t stride = lrows ? nf : 1;
     T x[KB / 4];
@@ -647,6 +633,28 @@ __global__ __launch_bounds__(4 * SPEC_ROWS) void k_spec_trsm(FactorArgs a, int k
         const int c = 4 * t + g;
         x[t] = (on && c < kb) ? base[c * stride] : T(0);
     }
+    {
+        const T *S = spec + mat * a.spec_stride + (int64_t)w.w * SPEC_SLOT;
+        const T *Fd = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff + k + (int64_t)k * nf;
+        constexpr int PT = KB * KB / (4 * SPEC_ROWS);
+        T v[PT];
+#pragma unroll
+        for (int h = 0; h < PT; h++) {
+            const int e = threadIdx.x + h * 4 * SPEC_ROWS, i = e % KB, jj = e / KB;   // block entry (i, jj)
+            v[h] = (i < kb && jj < kb) ? Fd[i + (int64_t)jj * nf] : T(0);
+        }
+#pragma unroll
+        for (int h = 0; h < PT; h++) {
+            const int e = threadIdx.x + h * 4 * SPEC_ROWS, i = e % KB, jj = e / KB;
+            if (lrows)
+                sm[i * LDM + jj] = jj >= i ? v[h] : T(0);   // U(i, jj)
+            else
+                sm[jj * LDM + i] = i > jj ? v[h] : T(0);    // L(i, jj)
+        }
+        for (int e = threadIdx.x; e < 2 * KB; e += 4 * SPEC_ROWS) sm[KB * LDM + e] = S[e];
+    }
+    __syncthreads();
+    const T *rc = sm + KB * LDM;
     int bad = 0;
     if (!lrows)
         spec_solve_group<T, KB, false, false>(x, sm, rc, g, false, itau);
