@@ -148,7 +148,8 @@ hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int
 int panel_variant_rows(int v);
 // which: 0 forward triangular, 1/2 forward rectangular partials/sum (L21), 3/4 backward
 // rectangular partials/sum (U12), 5 backward triangular, 6/7 inv(L11) partials/sum, 8/9
-// inv(U11) partials/sum, 10 forward gather, 11-16 fused per-front sweeps
+// inv(U11) partials/sum, 10 forward gather, 11-16 fused per-front sweeps, 17 forward L21
+// product without split-K
 template <typename T>
 hipError_t launch_solve_level(int which, const SolveArgs &a, const int32_t *lists, int list_off, int count, int nmat,
                               hipStream_t st);

@@ -116,6 +116,7 @@ struct mf_plan {
     std::vector<int32_t> lvl_fpo, lvl_fpc, lvl_fso, lvl_fsc, lvl_bpo, lvl_bpc, lvl_bso, lvl_bsc;
     std::vector<int32_t> lvl_ilpo, lvl_ilpc, lvl_ilso, lvl_ilsc, lvl_iupo, lvl_iupc, lvl_iuso, lvl_iusc;
     std::vector<int32_t> lvl_ntri;   // large fronts of the level without stored inverses
+    std::vector<int32_t> lvl_fdo, lvl_fdc;   // forward L21 product without split-K (k_rect_direct)
     std::vector<int32_t> lvl_go, lvl_gc;   // forward gather items {front, pos0}
     // fronts swept by the fused per-front kernels (no stored inverse): nf <= 64 and
     // 64 < nf <= FRONT_MAXNF; the level lists above hold only the other fronts
@@ -937,8 +938,20 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
         // of the large fronts (rows np, cols np; chunks entirely outside the triangle skipped).
         // U12 of a root (nu = 0) has no chunks; its sum still sets x = y.
         int32_t ntri = 0;
-        for (int64_t s : big) ntri += ioff[s] < 0 ? 1 : 0;
+        int64_t maxnp = 0;
+        for (int64_t s : big) {
+            ntri += ioff[s] < 0 ? 1 : 0;
+            maxnp = std::max(maxnp, np[s]);
+        }
         P.lvl_ntri.push_back(ntri);
+        // L21 of a level whose pivot blocks fit four column chunks: one launch, no split-K
+        const bool fdirect = maxnp <= 4 * SOLVE_CK_HOST;
+        while (slist.size() & 3) slist.push_back(0);
+        P.lvl_fdo.push_back((int32_t)slist.size());
+        if (fdirect)
+            for (int64_t s : big)
+                for (int64_t r0 = 0; r0 < nu[s]; r0 += 64) emit4((int32_t)s, (int32_t)r0, 0, 0);
+        P.lvl_fdc.push_back((int32_t)((slist.size() - P.lvl_fdo.back()) / 4));
         for (int dir = 0; dir < 4; dir++) {
             int64_t slot = 0;
             std::vector<int32_t> sums;
@@ -946,6 +959,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             int32_t po = (int32_t)slist.size();
             for (int64_t s : big) {
                 if (dir >= 2 && ioff[s] < 0) continue;
+                if (dir == 0 && fdirect) continue;
                 int64_t nrows = dir == 0 ? nu[s] : np[s], ncols = dir == 1 ? nu[s] : np[s];
                 for (int64_t r0 = 0; r0 < nrows; r0 += 256) {
                     int64_t nch = 0;
@@ -1342,6 +1356,10 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
                 return launch_solve_level<T>(7, a, sl, P.lvl_ilso[lv], P.lvl_ilsc[lv], nmat, P.stream);
             }));
         }
+        if (P.lvl_fdc[lv] > 0)
+            HIPCHK(profiled(P, 5, P.lvl_lflops[lv] * nrhs * nmat, P.lvl_lbytes[lv] * nmat, [&] {
+                return launch_solve_level<T>(17, a, sl, P.lvl_fdo[lv], P.lvl_fdc[lv], nmat, P.stream);
+            }));
         if (P.lvl_fpc[lv] > 0) {
             HIPCHK(profiled(P, 5, P.lvl_lflops[lv] * nrhs * nmat, P.lvl_lbytes[lv] * nmat, [&] {
                 return launch_solve_level<T>(1, a, sl, P.lvl_fpo[lv], P.lvl_fpc[lv], nmat, P.stream);

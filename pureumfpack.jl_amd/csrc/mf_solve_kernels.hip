@@ -367,6 +367,57 @@ __global__ __launch_bounds__(256) void k_rect_sum(SolveArgs a, int kind)
     }
 }
 
+// Forward L21 product without split-K, for levels whose split fronts all have np <= 4 *
+// SOLVE_CK: uv[t] -= sum_a L21[t, a] y[c1 + a].  Items int4 {front, row0}: 64 rows per
+// 256-thread workgroup, wave w takes columns 64 w .. 64 w + 63 (32 loads in flight), and the
+// four partial sums of a row are added in wave order in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void k_rect_direct(SolveArgs a)
+{
+    constexpr int NW = 4, CK = SOLVE_CK;
+    __shared__ T sv[NW * CK * SOLVE_RC];   // y, columns 0 .. 255, by right-hand side
+    __shared__ T red[NW * SOLVE_RC * 64];
+    const int mat = blockIdx.y;
+    const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
+    const SnDev d = a.sn[w.x];
+    const int nf = d.nf, np = d.np, nu = d.nu, c1 = d.c1;
+    const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int row = w.y + lane;
+    const int64_t vb = (int64_t)mat * a.nrhs;
+    const T *y = reinterpret_cast<const T *>(a.y) + vb * a.n;
+    T *uv = reinterpret_cast<T *>(a.uv) + vb * a.nupd;
+    const int col0 = wv * CK;
+    const int bs = max(0, min(CK, np - col0));
+    const int nrhs = (int)a.nrhs;
+    for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
+        const int rc = min(SOLVE_RC, nrhs - r0);
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < NW * CK * SOLVE_RC / 256; h++) {
+            const int e = tid + h * 256;
+            const int j = e % (NW * CK), r = e / (NW * CK);
+            sv[e] = (j < np && r < rc) ? y[(int64_t)(r0 + r) * a.n + c1 + j] : T(0);
+        }
+        __syncthreads();
+        T acc[SOLVE_RC] = {};
+        if (row < nu) dot_rows<T, 32>(F + (np + row) + (int64_t)col0 * nf, nf, bs, sv + col0, NW * CK, acc);
+#pragma unroll
+        for (int r = 0; r < SOLVE_RC; r++) red[(wv * SOLVE_RC + r) * 64 + lane] = acc[r];
+        __syncthreads();
+        if (wv == 0 && row < nu) {
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) {
+                if (r >= rc) continue;
+                const T sum = ((red[(0 * SOLVE_RC + r) * 64 + lane] + red[(1 * SOLVE_RC + r) * 64 + lane]) +
+                               red[(2 * SOLVE_RC + r) * 64 + lane]) +
+                              red[(3 * SOLVE_RC + r) * 64 + lane];
+                uv[(int64_t)(r0 + r) * a.nupd + d.upd_off + row] -= sum;
+            }
+        }
+    }
+}
+
 // Backward sweep, part 2, fronts without stored inverses (np <= TRI_NP): x_s <- U11^-1 x_s,
 // bottom diagonal block first, in LDS; the factor values are loaded before the vector.
 template <typename T>
@@ -1122,6 +1173,7 @@ hipError_t launch_solve_level(int which, const SolveArgs &a, const int32_t *list
     case 14: hipLaunchKernelGGL((k_bwd_front_b<T, FRONT_MAXNF>), g, blk, 0, st, b); break;
     case 15: hipLaunchKernelGGL((k_fwd_front_b<T, 256>), g, blk, 0, st, b, a.ecolptr, a.eupd); break;
     case 16: hipLaunchKernelGGL((k_bwd_front_b<T, 256>), g, blk, 0, st, b); break;
+    case 17: hipLaunchKernelGGL(k_rect_direct<T>, g, blk, 0, st, b); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
