@@ -1021,12 +1021,17 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmProb *__restrict__ 
 template <typename T, int KB>
 __global__ __launch_bounds__(256) void k_spec_diag_gemm(FactorArgs a, int k, float tau,
                                                         const GemmProb *__restrict__ probs,
-                                                        const GemmTile *__restrict__ tiles, int ndiag)
+                                                        const GemmTile *__restrict__ tiles, int ndiag, int dprob)
 {
     __shared__ T lds[gemm_lds_elems<T, 64, 64>()];
     if ((int)blockIdx.x >= ndiag) {
         gemm_tile<T, 64, 64>(probs, tiles[blockIdx.x - ndiag], a.pool, a.pool_stride, lds);
         return;
+    }
+    if (dprob >= 0) {
+        // this diagonal block's own share of the previous block's update, then its factorization
+        gemm_tile<T, 64, 64>(probs, GemmTile{dprob + (int)blockIdx.x, 0, 0, 0}, a.pool, a.pool_stride, lds);
+        __syncthreads();
     }
     spec_diag_front<T, KB>(a, k, tau, (int)blockIdx.x, lds);
 }
@@ -1104,7 +1109,7 @@ hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int
         break;
     case L_SPEC_DIAG_GEMM:
         hipLaunchKernelGGL((k_spec_diag_gemm<T, SPEC_KB>), grid, dim3(256), 0, st, a, L.k, L.tau, probs + L.prob_off,
-                           tiles + L.tile_off, L.count - L.ntiles);
+                           tiles + L.tile_off, L.count - L.ntiles, L.dprob);
         break;
     case L_SPEC_TRSM:
         hipLaunchKernelGGL((k_spec_trsm<T, SPEC_KB>), grid, dim3(4 * SPEC_ROWS), 0, st, a, L.k, L.tau,

@@ -75,6 +75,8 @@ struct Launch {
     int32_t tile;       // L_GEMM: tile shape (GEMM_TILE_* index)
     int32_t tile_off;   // L_SPEC_DIAG_GEMM: offset of its gemm tiles (its fronts are at list_off)
     int32_t ntiles;     // L_SPEC_DIAG_GEMM: gemm tiles before the diagonal-block workgroups
+    int32_t dprob;      // L_SPEC_DIAG_GEMM: problem (from prob_off) of the first diagonal block's
+                        // own panel update, one per diagonal workgroup; -1: none
     float tau;          // speculative schedule: multiplier bound 1/tau
     double flops;       // algorithmic flops of the launch (host-side bookkeeping)
     double bytes;       // algorithmic HBM bytes of the launch

@@ -608,10 +608,11 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             for (int64_t s : large) maxnp = std::max(maxnp, np[s]);
             if (sp == 1) {
                 // Interchange-free blocks of SPEC_KB columns with the multiplier check, with a
-                // look-ahead: block k's trailing update is split into G1 (the next block's panel
-                // column and row, which its diagonal factorization and trsm read) and G2 (the
-                // rest); G2 runs in the same launch as the next diagonal factorization.  Every
-                // entry still receives the same products in the same order.
+                // look-ahead: after block k's trsm, one launch holds the next diagonal blocks
+                // (each workgroup first applies block k's update to its own diagonal block,
+                // then factors it), the rest of the next block's panel column and row, and the
+                // remaining trailing update.  Every entry still receives the same products in
+                // the same order.
                 auto diag_fronts = [&](int64_t k) {
                     std::vector<int64_t> v;
                     for (int64_t s : large)
@@ -653,7 +654,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                 for (int64_t k = 0; k < maxnp; k += SPEC_KB) {
                     const std::vector<int64_t> fr = diag_fronts(k);
                     std::vector<int32_t> items;
-                    std::vector<GemmProb> g1v, g2v;
+                    std::vector<GemmProb> g1v, g2v, dgv;
                     for (size_t q = 0; q < fr.size(); q++) {
                         const int64_t s = fr[q];
                         const int32_t slot = (int32_t)q;
@@ -666,7 +667,8 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                         }
                         if (k2 >= np[s]) continue;
                         const int64_t k3 = k2 + std::min<int64_t>(SPEC_KB, np[s] - k2);
-                        g1v.push_back(prob(s, k2, k2, nf[s] - k2, k3 - k2, k, kb));                         // panel column
+                        dgv.push_back(prob(s, k2, k2, k3 - k2, k3 - k2, k, kb));                            // next diagonal block
+                        if (nf[s] > k3) g1v.push_back(prob(s, k3, k2, nf[s] - k3, k3 - k2, k, kb));         // panel column below it
                         if (nf[s] > k3) g1v.push_back(prob(s, k2, k3, k3 - k2, nf[s] - k3, k, kb));         // panel row
                         if (np[s] > k3) g2v.push_back(prob(s, k3, k3, nf[s] - k3, np[s] - k3, k, kb));      // rest of the pivot columns
                         if (np[s] > k3 && nu[s] > 0) g2v.push_back(prob(s, k3, np[s], np[s] - k3, nu[s], k, kb));  // rest of U12
@@ -690,10 +692,11 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                         }
                         sched.push_back(Tr);
                     }
-                    add_gemm(g1v, (int)lv, 0);
                     const int64_t next = k + SPEC_KB;
                     const std::vector<int64_t> nfr = diag_fronts(next);
                     if (nfr.empty() && g2v.empty()) continue;
+                    if (dgv.size() != nfr.size()) return fail(MF_BADARG, "internal: look-ahead blocks out of step");
+                    g1v.insert(g1v.end(), g2v.begin(), g2v.end());
                     Launch C{};
                     C.kind = L_SPEC_DIAG_GEMM;
                     C.k = (int32_t)next;
@@ -703,10 +706,16 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                     add_diag(C, nfr, next);
                     C.prob_off = (int32_t)probs.size();
                     C.tile_off = (int32_t)tiles.size();
-                    for (size_t q = 0; q < g2v.size(); q++) {
-                        const GemmProb &g = g2v[q];
+                    for (size_t q = 0; q < g1v.size(); q++) {
+                        const GemmProb &g = g1v[q];
                         for (int tn = 0; tn < g.n; tn += 64)
                             for (int tm = 0; tm < g.m; tm += 64) tiles.push_back({(int32_t)q, tm, tn, 0});
+                        probs.push_back(g);
+                        C.flops += 2.0 * g.m * (double)g.n * g.k;
+                        C.bytes += ((double)g.m * g.k + (double)g.k * g.n + 2.0 * g.m * (double)g.n) * ebytes;
+                    }
+                    C.dprob = dgv.empty() ? -1 : (int32_t)g1v.size();
+                    for (const GemmProb &g : dgv) {
                         probs.push_back(g);
                         C.flops += 2.0 * g.m * (double)g.n * g.k;
                         C.bytes += ((double)g.m * g.k + (double)g.k * g.n + 2.0 * g.m * (double)g.n) * ebytes;
