@@ -51,10 +51,12 @@ __device__ __forceinline__ void wave_lower_solve(const T *sd, int ldd, int bs, i
 
 // One wave: v <- U^-1 v for the upper bs x bs block staged in sd, last column first; lane j
 // takes x_j = v_j / u_jj (one division per step), the lanes above subtract u_ij x_j.
-// Steps below column 0 (the last group) have u = 0 and 1/u_jj = 0 and change nothing.
+// Each lane divides once (its own 1/u_jj); step j takes lane j's by v_readlane.  Steps below
+// column 0 (the last group) have u = 0 and 1/u_jj = 0 and change nothing.
 template <typename T>
 __device__ __forceinline__ void wave_upper_solve(const T *sd, int ldd, int bs, int lane, T (&v)[SOLVE_RC])
 {
+    const T rd = lane < bs ? T(1) / sd[lane + lane * ldd] : T(1);
     for (int j1 = bs - 1; j1 >= 0; j1 -= 8) {
         T u[8], rj[8];
 #pragma unroll
@@ -62,7 +64,7 @@ __device__ __forceinline__ void wave_upper_solve(const T *sd, int ldd, int bs, i
             const int j = max(j1 - q, 0);
             const T t = sd[lane + j * ldd];
             u[q] = lane < j1 - q ? t : T(0);
-            rj[q] = j1 - q >= 0 ? T(1) / sd[j + j * ldd] : T(0);
+            rj[q] = j1 - q >= 0 ? lane_value(rd, j) : T(0);
         }
 #pragma unroll
         for (int q = 0; q < 8; q++) {
