@@ -153,6 +153,11 @@ int panel_variant_rows(int v);
 template <typename T>
 hipError_t launch_solve_level(int which, const SolveArgs &a, const int32_t *lists, int list_off, int count, int nmat,
                               hipStream_t st);
+// one launch for a level's one-wave fronts (woff, wcount) and workgroup fronts (boff, bcount) of
+// the fused sweeps; small: every workgroup front has nf <= 256
+template <typename T>
+hipError_t launch_solve_merged(int fwd, int small, const SolveArgs &a, const int32_t *lists, int boff, int bcount,
+                               int woff, int wcount, int nmat, hipStream_t st);
 template <typename T>
 hipError_t launch_prep_rhs(const double *B, int64_t ldb, const double *rs, int64_t rs_stride, const int32_t *qf,
                            T *w0, int64_t n, int64_t nrhs, int64_t b_mat_stride, int nmat, hipStream_t st);

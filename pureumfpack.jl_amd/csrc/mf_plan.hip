@@ -1338,17 +1338,28 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
     a.part_stride = std::max<int64_t>(P.part_slots, 1) * 256 * nrhs;
     const double e = (double)P.esize();
     const int32_t *sl = P.d_solvelist.as<int32_t>();
+    // one launch for a level's one-wave and workgroup fronts when the one-wave items are few
+    // (many of them are better off without the workgroup fronts' LDS footprint)
+    const int64_t nzc = (nrhs + 7) / 8;
+    auto merge_ok = [&](int64_t wcount) { return wcount * nmat * nzc <= 4096; };
     for (int64_t lv = 0; lv < P.nlevels; lv++) {
         const double vb = 3.0 * e * P.lvl_vec[lv] * nrhs;
-        if (P.lvl_s64c[lv] > 0)
+        if (P.lvl_s64c[lv] > 0 && P.lvl_s1kc[lv] > 0 && merge_ok(P.lvl_s64c[lv])) {
             HIPCHK(profiled(P, 5, 0.0, 0.0, [&] {
-                return launch_solve_level<T>(11, a, sl, P.lvl_s64o[lv], P.lvl_s64c[lv], nmat, P.stream);
+                return launch_solve_merged<T>(1, P.lvl_s1k_small[lv], a, sl, P.lvl_s1ko[lv], P.lvl_s1kc[lv],
+                                              P.lvl_s64o[lv], P.lvl_s64c[lv], nmat, P.stream);
             }));
-        if (P.lvl_s1kc[lv] > 0)
-            HIPCHK(profiled(P, 5, 0.0, 0.0, [&] {
-                return launch_solve_level<T>(P.lvl_s1k_small[lv] ? 15 : 12, a, sl, P.lvl_s1ko[lv], P.lvl_s1kc[lv], nmat,
-                                             P.stream);
-            }));
+        } else {
+            if (P.lvl_s64c[lv] > 0)
+                HIPCHK(profiled(P, 5, 0.0, 0.0, [&] {
+                    return launch_solve_level<T>(11, a, sl, P.lvl_s64o[lv], P.lvl_s64c[lv], nmat, P.stream);
+                }));
+            if (P.lvl_s1kc[lv] > 0)
+                HIPCHK(profiled(P, 5, 0.0, 0.0, [&] {
+                    return launch_solve_level<T>(P.lvl_s1k_small[lv] ? 15 : 12, a, sl, P.lvl_s1ko[lv],
+                                                 P.lvl_s1kc[lv], nmat, P.stream);
+                }));
+        }
         if (P.lvl_cnt[lv] == 0) continue;
         HIPCHK(profiled(P, 5, 0.0, vb * nmat, [&] {
             return launch_solve_level<T>(10, a, sl, P.lvl_go[lv], P.lvl_gc[lv], nmat, P.stream);
@@ -1380,15 +1391,22 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
     }
     for (int64_t lv = P.nlevels - 1; lv >= 0; lv--) {
         const double vb = 3.0 * e * P.lvl_vec[lv] * nrhs;
-        if (P.lvl_s64c[lv] > 0)
+        if (P.lvl_s64c[lv] > 0 && P.lvl_s1kc[lv] > 0 && merge_ok(P.lvl_s64c[lv])) {
             HIPCHK(profiled(P, 6, 0.0, 0.0, [&] {
-                return launch_solve_level<T>(13, a, sl, P.lvl_s64o[lv], P.lvl_s64c[lv], nmat, P.stream);
+                return launch_solve_merged<T>(0, P.lvl_s1k_small[lv], a, sl, P.lvl_s1ko[lv], P.lvl_s1kc[lv],
+                                              P.lvl_s64o[lv], P.lvl_s64c[lv], nmat, P.stream);
             }));
-        if (P.lvl_s1kc[lv] > 0)
-            HIPCHK(profiled(P, 6, 0.0, 0.0, [&] {
-                return launch_solve_level<T>(P.lvl_s1k_small[lv] ? 16 : 14, a, sl, P.lvl_s1ko[lv], P.lvl_s1kc[lv], nmat,
-                                             P.stream);
-            }));
+        } else {
+            if (P.lvl_s64c[lv] > 0)
+                HIPCHK(profiled(P, 6, 0.0, 0.0, [&] {
+                    return launch_solve_level<T>(13, a, sl, P.lvl_s64o[lv], P.lvl_s64c[lv], nmat, P.stream);
+                }));
+            if (P.lvl_s1kc[lv] > 0)
+                HIPCHK(profiled(P, 6, 0.0, 0.0, [&] {
+                    return launch_solve_level<T>(P.lvl_s1k_small[lv] ? 16 : 14, a, sl, P.lvl_s1ko[lv],
+                                                 P.lvl_s1kc[lv], nmat, P.stream);
+                }));
+        }
         if (P.lvl_cnt[lv] == 0) continue;
         if (P.lvl_bpc[lv] > 0)
             HIPCHK(profiled(P, 6, P.lvl_uflops[lv] * nrhs * nmat, (P.lvl_ubytes[lv] + vb) * nmat, [&] {

@@ -679,14 +679,14 @@ __global__ __launch_bounds__(256) void k_bwd_front_w(SolveArgs a, int count, int
 }
 
 template <typename T, int MAXNF>
-__global__ __launch_bounds__(256) void k_fwd_front_b(SolveArgs a, const int32_t *__restrict__ ecolptr,
-                                                     const int32_t *__restrict__ eupd)
+__device__ __forceinline__ void fwd_front_b(const SolveArgs &a, const int32_t *__restrict__ ecolptr,
+                                            const int32_t *__restrict__ eupd, int slot, int mat_, int zc, int nz)
 {
     constexpr int NT = 256, PT = MAXNF / NT, LDD = SOLVE_NB + 1, DPT = SOLVE_NB * SOLVE_NB / NT;
     __shared__ T g[SOLVE_RC * MAXNF];            // g[r * MAXNF + p]
     __shared__ T sd[SOLVE_NB * LDD];             // diagonal block of L11
-    const int mat = blockIdx.y;
-    const SnDev d = a.sn[a.list[blockIdx.x]];
+    const int mat = mat_;
+    const SnDev d = a.sn[a.list[slot]];
     const int nf = d.nf, np = d.np, c1 = d.c1;
     const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
     const int32_t *prow = a.prow + mat * a.n_stride;
@@ -706,7 +706,7 @@ __global__ __launch_bounds__(256) void k_fwd_front_b(SolveArgs a, const int32_t 
         qe[k] = p < nf ? ecolptr[d.fcol + p + 1] : 0;
         nstep = max(nstep, qe[k] - qb[k]);
     }
-    for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
+    for (int r0 = zc * SOLVE_RC; r0 < nrhs; r0 += nz * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
         // the first diagonal block does not depend on the gather: its loads go out first
         T dpre[DPT];
@@ -815,7 +815,14 @@ __global__ __launch_bounds__(256) void k_fwd_front_b(SolveArgs a, const int32_t 
 }
 
 template <typename T, int MAXNF>
-__global__ __launch_bounds__(256) void k_bwd_front_b(SolveArgs a)
+__global__ __launch_bounds__(256) void k_fwd_front_b(SolveArgs a, const int32_t *__restrict__ ecolptr,
+                                                     const int32_t *__restrict__ eupd)
+{
+    fwd_front_b<T, MAXNF>(a, ecolptr, eupd, blockIdx.x, blockIdx.y, blockIdx.z, gridDim.z);
+}
+
+template <typename T, int MAXNF>
+__device__ __forceinline__ void bwd_front_b(const SolveArgs &a, int slot, int mat_, int zc, int nz)
 {
     constexpr int NT = 256, LDD = SOLVE_NB + 1;
     constexpr int SDN = SOLVE_NB * LDD > NT * SOLVE_RC ? SOLVE_NB * LDD : NT * SOLVE_RC;
@@ -823,8 +830,8 @@ __global__ __launch_bounds__(256) void k_bwd_front_b(SolveArgs a)
     __shared__ T sdred[SDN];                     // U12 product partial sums, then diagonal blocks of U11
     T *const sd = sdred;
     T *const red = sdred;
-    const int mat = blockIdx.y;
-    const SnDev d = a.sn[a.list[blockIdx.x]];
+    const int mat = mat_;
+    const SnDev d = a.sn[a.list[slot]];
     const int nf = d.nf, np = d.np, nu = d.nu, c1 = d.c1;
     const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
     const int32_t *upd = a.upd + d.upd_off;
@@ -845,7 +852,7 @@ __global__ __launch_bounds__(256) void k_bwd_front_b(SolveArgs a)
         const int p = tid + k * NT;
         srcs[k] = p < np ? c1 + p : (p < nf ? upd[p - np] : 0);
     }
-    for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
+    for (int r0 = zc * SOLVE_RC; r0 < nrhs; r0 += nz * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
         // s = y of the pivot rows, x at the update columns: every position of the thread at once
         T gv[PT][SOLVE_RC];
@@ -946,6 +953,12 @@ __global__ __launch_bounds__(256) void k_bwd_front_b(SolveArgs a)
         }
         __syncthreads();
     }
+}
+
+template <typename T, int MAXNF>
+__global__ __launch_bounds__(256) void k_bwd_front_b(SolveArgs a)
+{
+    bwd_front_b<T, MAXNF>(a, blockIdx.x, blockIdx.y, blockIdx.z, gridDim.z);
 }
 
 __global__ __launch_bounds__(256) void k_copy16(int4 *__restrict__ dst, const int4 *__restrict__ src, int64_t n16)
@@ -1140,6 +1153,70 @@ __global__ __launch_bounds__(256) void k_export(const SnDev *__restrict__ sn, co
 }
 
 // ---- host launchers ----
+// A level's one-wave fronts and workgroup fronts in one launch: the first nb * nmat * nz
+// workgroups take the larger fronts (front, matrix, right-hand-side chunk), the rest run the
+// one-wave items of wlist in a grid-stride loop.
+template <typename T, int MAXNF>
+__global__ __launch_bounds__(256) void k_fwd_front_wb(SolveArgs a, const int32_t *__restrict__ ecolptr,
+                                                      const int32_t *__restrict__ eupd, const int32_t *__restrict__ wlist,
+                                                      int nb, int wcount, int nmat, int nz)
+{
+    const int nbw = nb * nmat * nz;
+    if ((int)blockIdx.x < nbw) {
+        const int rest = (int)blockIdx.x / nb;
+        fwd_front_b<T, MAXNF>(a, ecolptr, eupd, (int)blockIdx.x % nb, rest % nmat, rest / nmat, nz);
+        return;
+    }
+    SolveArgs aw = a;
+    aw.list = wlist;
+    const int64_t items = (int64_t)wcount * nmat * nz, nwg = (int64_t)gridDim.x - nbw;
+    for (int64_t it = ((int64_t)blockIdx.x - nbw) * 4 + (threadIdx.x >> 6); it < items; it += nwg * 4) {
+        const int64_t rest = it / wcount;
+        fwd_front_w<T>(aw, ecolptr, eupd, (int)(it % wcount), (int)(rest % nmat), (int)(rest / nmat), nz);
+    }
+}
+
+template <typename T, int MAXNF>
+__global__ __launch_bounds__(256) void k_bwd_front_wb(SolveArgs a, const int32_t *__restrict__ wlist, int nb, int wcount,
+                                                      int nmat, int nz)
+{
+    const int nbw = nb * nmat * nz;
+    if ((int)blockIdx.x < nbw) {
+        const int rest = (int)blockIdx.x / nb;
+        bwd_front_b<T, MAXNF>(a, (int)blockIdx.x % nb, rest % nmat, rest / nmat, nz);
+        return;
+    }
+    SolveArgs aw = a;
+    aw.list = wlist;
+    const int64_t items = (int64_t)wcount * nmat * nz, nwg = (int64_t)gridDim.x - nbw;
+    for (int64_t it = ((int64_t)blockIdx.x - nbw) * 4 + (threadIdx.x >> 6); it < items; it += nwg * 4) {
+        const int64_t rest = it / wcount;
+        bwd_front_w<T>(aw, (int)(it % wcount), (int)(rest % nmat), (int)(rest / nmat), nz);
+    }
+}
+
+template <typename T>
+hipError_t launch_solve_merged(int fwd, int small, const SolveArgs &a, const int32_t *lists, int boff, int bcount,
+                               int woff, int wcount, int nmat, hipStream_t st)
+{
+    SolveArgs b = a;
+    b.list = lists + boff;
+    const int nz = (int)((a.nrhs + SOLVE_RC - 1) / SOLVE_RC);
+    const int64_t witems = (int64_t)wcount * nmat * nz;
+    const int64_t grid = (int64_t)bcount * nmat * nz + std::min<int64_t>((witems + 3) / 4, 8192);
+    if (grid > INT32_MAX) return hipErrorInvalidValue;
+    const dim3 g((unsigned)grid), blk(256);
+    const int32_t *wl = lists + woff;
+    if (fwd) {
+        if (small) hipLaunchKernelGGL((k_fwd_front_wb<T, 256>), g, blk, 0, st, b, a.ecolptr, a.eupd, wl, bcount, wcount, nmat, nz);
+        else hipLaunchKernelGGL((k_fwd_front_wb<T, FRONT_MAXNF>), g, blk, 0, st, b, a.ecolptr, a.eupd, wl, bcount, wcount, nmat, nz);
+    } else {
+        if (small) hipLaunchKernelGGL((k_bwd_front_wb<T, 256>), g, blk, 0, st, b, wl, bcount, wcount, nmat, nz);
+        else hipLaunchKernelGGL((k_bwd_front_wb<T, FRONT_MAXNF>), g, blk, 0, st, b, wl, bcount, wcount, nmat, nz);
+    }
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_solve_level(int which, const SolveArgs &a, const int32_t *lists, int list_off, int count, int nmat,
                               hipStream_t st)
@@ -1241,6 +1318,8 @@ hipError_t launch_export(const SnDev *sn, int64_t nsuper, const T *pool, const i
 
 #define MF_INST(T)                                                                                                \
     template hipError_t launch_solve_level<T>(int, const SolveArgs &, const int32_t *, int, int, int, hipStream_t); \
+    template hipError_t launch_solve_merged<T>(int, int, const SolveArgs &, const int32_t *, int, int, int, int, int, \
+                                               hipStream_t);                                                        \
     template hipError_t launch_prep_rhs<T>(const double *, int64_t, const double *, int64_t, const int32_t *,   \
                                            T *, int64_t, int64_t, int64_t, int, hipStream_t);                   \
     template hipError_t launch_finish_x<T>(const T *, const int32_t *, double *, int64_t, int64_t, int64_t,     \

@@ -61,6 +61,24 @@ def test_batch_mixed_refine_to_bwd_tol():
     assert B.last_report["backward_error"] <= 1e-14
 
 
+def test_batch_many_rhs_separate_wave_and_workgroup_launches():
+    """Enough matrices x right-hand sides that levels with one-wave and workgroup fronts run
+    them in separate launches (the one-wave items are too many to share a launch); every
+    matrix's FP64 solve must still be accurate."""
+    k, nmat, nrhs = 14, 64, 64   # 14^3: a level of 50 one-wave and 2 workgroup fronts
+    P, Ax = M.poisson3d_batch_values(k, nmat, seed=41)
+    n = P.shape[0]
+    print(f"[case] batch fp64 k={k} nmat={nmat} nrhs={nrhs}", flush=True)
+    Bv = np.random.default_rng(8).standard_normal((nmat, n, nrhs))
+    B = I.BatchedLU(P, nmat, precision="fp64", use_graph=False)
+    B.refactor(Ax)
+    X = B.solve(Bv, refine=0)
+    for m in range(0, nmat, 7):
+        A = B.matrix(Ax[m])
+        for j in (0, nrhs - 1):
+            assert _bwd_err(A, X[m][:, j], Bv[m][:, j]) <= 1e-13
+
+
 def test_batch_matches_single_matrix():
     """Matrix m of a batch solves exactly like a one-matrix plan on A_m (same schedule)."""
     k, nmat = 8, 4
