@@ -331,6 +331,15 @@ def main():
               f"{cd['refine_iterations']} refine", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A, S, Rs, B, refine=1)
+    # accuracy of every leg against its target: a timing of wrong answers is no timing
+    failed = [f"{p}: bwd {r['backward_error']:.2e}" for p, r in results.items() if r["backward_error"] > args.tol]
+    if "batched" in out and out["batched"]["backward_error_host_check"] > args.tol:
+        failed.append(f"batched: bwd {out['batched']['backward_error_host_check']:.2e}")
+    if "convdiff_colamd_bf16" in out and out["convdiff_colamd_bf16"]["backward_error"] > 1e-12:
+        failed.append(f"convdiff: bwd {out['convdiff_colamd_bf16']['backward_error']:.2e}")
+    out["accuracy_ok"] = not failed
+    if failed:
+        print("# ACCURACY CHECK FAILED: " + "; ".join(failed), file=sys.stderr, flush=True)
     print(json.dumps(out))
 
 
