@@ -230,9 +230,10 @@ __global__ __launch_bounds__(256) void k_fwd_tri(SolveArgs a)
 //   kind 3: x[c1+i] = sum_a inv(U11)[i, a] zbuf[c1 + a]         upper-triangle chunks only
 // k_rect_part items are int4 {front, row0, col0, slot}: each workgroup writes its partial sums
 // (256 rows x nrhs) to partial slot `slot`.  k_rect_sum items are {dst, rows | zb << 16,
-// nchunks, slot0}: the row block's chunks own slots slot0 .. slot0 + nchunks - 1, added in
-// chunk order (the result does not depend on scheduling) and applied to `rows` entries from
-// dst (c1 + row0, or upd_off + row0 for kind 0; zb: kind 1 writes zbuf).
+// nchunks, slot0}: the row block's chunks own slots slot0 .. slot0 + nchunks - 1, added in a
+// fixed order (four in-order quarters, then the quarters in order: the result does not depend
+// on scheduling) and applied to `rows` entries from dst (c1 + row0, or upd_off + row0 for
+// kind 0; zb: kind 1 writes zbuf).
 constexpr int SOLVE_CK = 64;
 
 // acc[r] += sum_{j < bs} M[j * ld] * s[r * sld + j], G loads in flight
@@ -319,12 +320,17 @@ __global__ __launch_bounds__(256) void k_rect_part(SolveArgs a, int kind)
 template <typename T>
 __global__ __launch_bounds__(256) void k_rect_sum(SolveArgs a, int kind)
 {
+    // four workgroups per item (64 rows each); four threads per row, thread quarter g adding
+    // chunks [g nch / 4, (g + 1) nch / 4) in order; the quarters are added in order in LDS
+    __shared__ T red[4 * SOLVE_RC * 64];
     const int mat = blockIdx.y;
-    const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
-    if ((int)threadIdx.x >= (w.y & 0xffff)) return;
-    const int64_t dst = (int64_t)w.x + threadIdx.x;
+    const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x >> 2];
+    const int g = threadIdx.x & 3, rr = ((int)blockIdx.x & 3) * 64 + (threadIdx.x >> 2);
+    const bool on = rr < (w.y & 0xffff);
+    const int64_t dst = (int64_t)w.x + rr;
     const bool to_zb = (w.y >> 16) != 0;
     const int nch = w.z;
+    const int c0 = g * nch / 4, c1 = (g + 1) * nch / 4;
     const int64_t vb = (int64_t)mat * a.nrhs;
     T *y = reinterpret_cast<T *>(a.y) + vb * a.n;
     T *x = reinterpret_cast<T *>(a.x) + vb * a.n;
@@ -335,35 +341,46 @@ __global__ __launch_bounds__(256) void k_rect_sum(SolveArgs a, int kind)
     const int nrhs = (int)a.nrhs;
     for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
-        const T *pp = part + (int64_t)r0 * 256 + threadIdx.x;
-        // chunks in order, four chunks x SOLVE_RC right-hand sides of loads in flight
+        const T *pp = part + (int64_t)r0 * 256 + rr;
         T s[SOLVE_RC] = {};
-        int c = 0;
-        for (; c + 4 <= nch; c += 4) {
-            T v[4][SOLVE_RC];
+        if (on) {
+            int c = c0;
+            for (; c + 4 <= c1; c += 4) {
+                T v[4][SOLVE_RC];
 #pragma unroll
-            for (int q = 0; q < 4; q++)
+                for (int q = 0; q < 4; q++)
 #pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++) v[q][r] = r < rc ? pp[(c + q) * cs + r * 256] : T(0);
+                    for (int r = 0; r < SOLVE_RC; r++) v[q][r] = r < rc ? pp[(c + q) * cs + r * 256] : T(0);
 #pragma unroll
-            for (int q = 0; q < 4; q++)
+                for (int q = 0; q < 4; q++)
 #pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++) s[r] += v[q][r];
+                    for (int r = 0; r < SOLVE_RC; r++) s[r] += v[q][r];
+            }
+            for (; c < c1; c++) {
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++)
+                    if (r < rc) s[r] += pp[c * cs + r * 256];
+            }
         }
-        for (; c < nch; c++) {
+        __syncthreads();
 #pragma unroll
-            for (int r = 0; r < SOLVE_RC; r++)
-                if (r < rc) s[r] += pp[c * cs + r * 256];
-        }
+        for (int r = 0; r < SOLVE_RC; r++) red[(g * SOLVE_RC + r) * 64 + (threadIdx.x >> 2)] = s[r];
+        __syncthreads();
+        if (g == 0 && on) {
+            const int l = threadIdx.x >> 2;
 #pragma unroll
-        for (int r = 0; r < SOLVE_RC; r++) {
-            if (r >= rc) continue;
-            const int64_t ro = (int64_t)(r0 + r) * a.n + dst;
-            switch (kind) {
-            case 0: uv[(int64_t)(r0 + r) * a.nupd + dst] -= s[r]; break;
-            case 1: (to_zb ? zb : x)[ro] = y[ro] - s[r]; break;   // inv(U11) next
-            case 2: y[ro] = s[r]; break;
-            default: x[ro] = s[r]; break;
+            for (int r = 0; r < SOLVE_RC; r++) {
+                if (r >= rc) continue;
+                const T sum = ((red[(0 * SOLVE_RC + r) * 64 + l] + red[(1 * SOLVE_RC + r) * 64 + l]) +
+                               red[(2 * SOLVE_RC + r) * 64 + l]) +
+                              red[(3 * SOLVE_RC + r) * 64 + l];
+                const int64_t ro = (int64_t)(r0 + r) * a.n + dst;
+                switch (kind) {
+                case 0: uv[(int64_t)(r0 + r) * a.nupd + dst] -= sum; break;
+                case 1: (to_zb ? zb : x)[ro] = y[ro] - sum; break;   // inv(U11) next
+                case 2: y[ro] = sum; break;
+                default: x[ro] = sum; break;
+                }
             }
         }
     }
@@ -1225,7 +1242,8 @@ hipError_t launch_solve_level(int which, const SolveArgs &a, const int32_t *list
     b.list = lists + list_off;
     // right-hand-side chunks of SOLVE_RC in grid z
     const int nz = (int)((a.nrhs + SOLVE_RC - 1) / SOLVE_RC);
-    dim3 g(count, nmat, nz), blk(256);
+    const bool sum = which == 2 || which == 4 || which == 7 || which == 9;   // four workgroups per item
+    dim3 g(sum ? 4 * count : count, nmat, nz), blk(256);
     switch (which) {
     case 0: hipLaunchKernelGGL(k_fwd_tri<T>, g, blk, 0, st, b); break;
     case 1: hipLaunchKernelGGL(k_rect_part<T>, g, blk, 0, st, b, 0); break;
