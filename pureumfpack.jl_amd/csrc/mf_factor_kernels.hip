@@ -372,18 +372,19 @@ __global__ __launch_bounds__(256) void k_panel_apply(FactorArgs a, int k)
 // pivoted one.  These kernels factor a 64-column block at a time without interchanges and
 // check that condition on every multiplier; a violation sets a per-matrix flag and the
 // factorization is redone on the pivoted schedule (k_panel / k_panel_apply).
-//   k_spec_diag: one wave per front factors the kb x kb diagonal block in registers.
+//   k_spec_diag: one 256-thread workgroup per front factors the kb x kb diagonal block in
+//                registers (four waves of 16 columns; one wave for special pivots).
 //   k_spec_trsm: many workgroups; item int4 {front, kind, start, slot}:
 //     kind 0: rows r in [start, start+SPEC_ROWS) of [k+kb, nf): L[r, k:k+kb] = A[r, k:k+kb] U_kk^-1,
 //             checking the multiplier bound on the candidate rows r < np;
 //     kind 1: columns c in [start, start+SPEC_ROWS) of [k+kb, nf): U[k:k+kb, c] = L_kk^-1 A[k:k+kb, c];
-//     slot: the front's position in the diagonal launch (its packed block in a.spec).
+//     slot: the front's position in the diagonal launch (its reciprocal pivots and special-pivot
+//     flag in a.spec).
 // ------------------------------------------------------------------------------------------
-// The KB steps of the diagonal block factorization (see k_spec_diag).  FAST: every pivot is
-// assumed normal (|u_jj| >= smallest normal): scaling by the reciprocal, branch-free, and
-// `special` records whether the assumption failed.  Otherwise getf2's rules per step: a zero
-// pivot leaves its column unscaled (and is reported through info), a tiny one divides.
-template <typename T, int KB, bool FAST>
+// The kb steps of the diagonal block factorization by one wave (see k_spec_diag), with getf2's
+// rules per step: a normal pivot scales by its reciprocal, a zero pivot leaves its column
+// unscaled (and is reported through info), a tiny one divides.
+template <typename T, int KB>
 __device__ __forceinline__ void spec_diag_steps(T (&x)[KB], int lane, int kb, T itau, int &bad, int &special,
                                                 T &myrcp, int32_t *info, int infoc)
 {
@@ -392,16 +393,13 @@ __device__ __forceinline__ void spec_diag_steps(T (&x)[KB], int lane, int kb, T 
 #pragma unroll
     for (int jj = 0; jj < 32; jj++) {
         const int j = 32 * jc + jj;
-        if (!FAST && j >= kb) continue;
+        if (j >= kb) continue;
         const T piv = lane_value(x[j], j);
-        const bool act = lane > j && (FAST || lane < kb);
+        const bool act = lane > j && lane < kb;
         T v = x[j];
         bad |= (int)(act && fabs(v) > fabs(piv) * itau);
         const T rp = T(1) / piv;
-        if (FAST) {
-            special |= (int)!(fabs(piv) >= Tiny<T>::v());
-            v = v * rp;
-        } else if (fabs(piv) >= Tiny<T>::v()) {
+        if (fabs(piv) >= Tiny<T>::v()) {
             v = v * rp;
         } else {
             special = 1;
@@ -425,7 +423,7 @@ __device__ __forceinline__ void spec_diag_steps(T (&x)[KB], int lane, int kb, T 
 // scales its multipliers, publishes them in LDS (two buffers by step parity, one barrier per
 // step) and updates its own later columns; the waves to its right update all their columns.
 // Every element receives the same updates (explicit fused multiply-adds) in the same order as
-// in spec_diag_steps, so the results are bit-identical to the one-wave form.
+// in spec_diag_steps, so with normal pivots the results are bit-identical to the one-wave form.
 template <typename T, int KB>
 __device__ __forceinline__ void spec_diag_steps4(T (&x)[KB / 4], int lane, int w, T itau, int &bad, int &special,
                                                  T &myrcp, T *lbuf)
@@ -502,7 +500,7 @@ __device__ __forceinline__ void spec_diag_front(const FactorArgs &a, int k, floa
     for (int c = 0; c < KB; c++) x[c] = (lane < kb && c < kb) ? F[(k + lane) + (int64_t)(k + c) * nf] : T(0);
     int bad = 0, special = 0;
     T myrcp = T(1);
-    spec_diag_steps<T, KB, false>(x, lane, kb, itau, bad, special, myrcp, a.info + mat, d.c1 + k);
+    spec_diag_steps<T, KB>(x, lane, kb, itau, bad, special, myrcp, a.info + mat, d.c1 + k);
     if (bad) atomicOr(a.flag + mat, 1);
     if (lane < kb) {
 #pragma unroll

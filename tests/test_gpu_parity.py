@@ -161,6 +161,29 @@ def test_singular_pivot():
     assert F.U.shape == (3, 3)
 
 
+def test_subnormal_pivot_in_interchange_free_block():
+    """A dense 100 x 100 matrix (one front, a full 64-column block) whose factor column 5 is
+    zero except a subnormal diagonal: the interchange-free block meets a tiny pivot with no
+    multiplier above the bound, so the block is redone with getf2's rules (division by the
+    tiny pivot), and the factors must still match the oracle."""
+    rng = np.random.default_rng(17)
+    n = 100
+    D = rng.uniform(-1.0, 1.0, (n, n)) + np.diag(np.full(n, 2.0 * n))
+    A0 = sp.csc_matrix(D)
+    S = I.symbolic_mf(A0)
+    t = 5
+    qc = int(S.qf[t])
+    D[:, qc] = 0.0
+    D[qc, qc] = 1e-310
+    rows, cols = np.nonzero(np.ones_like(D))   # the dense pattern, zeros kept as stored entries
+    A = sp.csc_matrix((D[rows, cols], (rows, cols)), shape=D.shape)
+    assert A.nnz == n * n
+    print(f"[case] dense n={n} subnormal pivot in factor column {t}", flush=True)
+    F = I.splu(A, method="multifrontal", scale=I.SCALE_NONE, symbolic=S)
+    R = oracle_splu(A, S, scale=OS.SCALE_NONE)
+    _check_parity(A, F, R)
+
+
 def test_deterministic():
     A = M.randmat(500, 6)
     S = I.symbolic_mf(A)
