@@ -214,20 +214,28 @@ def backward_error(A, x, b):
     return np.abs(r).max() / (Ainf * np.abs(x).max() + np.abs(b).max())
 
 
-def dense_restricted_lu(V, sstart):
+def dense_restricted_lu(V, sstart, tie="first", stats=None):
     """Dense elimination of V with pivots chosen, column by column, as the first row of
     maximum magnitude among the not-yet-pivoted rows of the column's own supernode.
 
     Returns (prow, L, U) with V[prow, :] = L U (dense).  Brute-force check of mfo_factor.
+    tie="last" takes the last row of maximum magnitude instead (a deliberately wrong rule the
+    tests use to show that ties decide interchanges); `stats` (a dict) receives the number of
+    steps whose maximum magnitude is held by more than one candidate row ("ties") and the
+    number of steps that interchange rows ("swaps").
     """
     W = np.array(V.toarray() if sp.issparse(V) else V, dtype=np.float64)
     n = W.shape[0]
     perm = np.arange(n)
+    ties = swaps = 0
     for s in range(len(sstart) - 1):
         c1, c2 = sstart[s], sstart[s + 1] - 1
         for j in range(c1, c2 + 1):
             seg = np.abs(W[j:c2 + 1, j])
-            p = j + int(np.argmax(seg))
+            at_max = np.flatnonzero(seg == seg.max())
+            ties += int(at_max.size > 1)
+            p = j + int(at_max[0] if tie == "first" else at_max[-1])
+            swaps += int(p != j)
             if p != j:
                 W[[j, p], :] = W[[p, j], :]
                 perm[[j, p]] = perm[[p, j]]
@@ -236,4 +244,7 @@ def dense_restricted_lu(V, sstart):
             W[j + 1:, j + 1:] -= np.outer(W[j + 1:, j], W[j, j + 1:])
     L = np.tril(W, -1) + np.eye(n)
     U = np.triu(W)
+    if stats is not None:
+        stats["ties"] = ties
+        stats["swaps"] = swaps
     return perm, L, U

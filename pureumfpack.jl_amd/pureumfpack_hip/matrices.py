@@ -157,3 +157,49 @@ def spd6(seed: int = 2024) -> sp.csc_matrix:
     rng = np.random.default_rng(seed)
     M = rng.standard_normal((6, 6))
     return sp.csc_matrix(M @ M.T + 6.0 * np.eye(6))
+
+
+# ---- matrices that make the in-block partial pivoting interchange rows ----
+
+def sprand_identity(n: int, density: float | None = None, seed: int = 1) -> sp.csc_matrix:
+    """sprand(n, n, 12/n) + I, the unsymmetric family of test/pureumfpack_test.jl:106: random
+    uniform[0,1) entries at density 12/n plus the identity.  The diagonal is no larger than
+    the off-diagonal entries, so the pivoting moves most rows."""
+    rng = np.random.default_rng(seed)
+    d = 12.0 / n if density is None else density
+    R = sp.random(n, n, density=d, format="csc", random_state=rng, dtype=np.float64)
+    A = (R + sp.identity(n, format="csc")).tocsc()
+    A.sum_duplicates()
+    A.sort_indices()
+    return A
+
+
+def dense_random(n: int, seed: int = 3) -> sp.csc_matrix:
+    """Dense standard-normal n x n matrix stored sparse: one supernode, one front."""
+    rng = np.random.default_rng(seed)
+    return sp.csc_matrix(rng.standard_normal((n, n)))
+
+
+def signs(n: int, density: float, seed: int) -> sp.csc_matrix:
+    """Random +-1 matrix (diagonal always stored).  Under SCALE_NONE the candidate magnitudes of
+    a pivot column are small integers, so exact ties occur at many steps and the first-maximum
+    rule (BLAS idamax) decides the interchanges."""
+    rng = np.random.default_rng(seed)
+    D = np.where(rng.random((n, n)) < density, rng.choice([-1.0, 1.0], (n, n)), 0.0)
+    np.fill_diagonal(D, rng.choice([-1.0, 1.0], n))
+    rows, cols = np.nonzero(D)
+    return _csc(rows, cols, D[rows, cols], n)
+
+
+def poisson3d_weak(k: int, seed: int = 7, lo: float = 2.0, hi: float = 3.0) -> sp.csc_matrix:
+    """The batched workload's unsymmetric values on the poisson3d(k) pattern (off-diagonals
+    -U[0.5,1.5]) with the diagonal weakened to U[lo, hi]: not diagonally dominant, so the
+    large fronts interchange rows."""
+    P, Ax = poisson3d_batch_values(k, 1, seed=seed)
+    A = P.copy()
+    A.data = Ax[0].copy()
+    rng = np.random.default_rng(seed + 100)
+    A.setdiag(rng.uniform(lo, hi, A.shape[0]))
+    A = A.tocsc()
+    A.sort_indices()
+    return A

@@ -69,3 +69,48 @@ def test_row_scaling_matches_oracle():
     for A in (M.randmat(300, 6), M.convdiff3d(6)):
         for k in (OS.SCALE_NONE, OS.SCALE_SUM, OS.SCALE_MAX):
             assert np.array_equal(I.row_scaling(A, k), OS.row_scaling(A, k))
+
+
+def _threshold_edge_matrix(n, seed):
+    """Sparse random matrix whose first rows hold exactly thr - 1, thr, thr + 1 and thr + 2
+    entries, thr = floor(min(n - 2, max(16, 10 sqrt(n)))) the dense-row threshold of
+    colamd_order (src/amd.jl:96)."""
+    import math
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    thr = int(math.floor(min(n - 2, max(16.0, 10.0 * math.sqrt(n)))))
+    D = (rng.random((n, n)) < 3.0 / n).astype(float)
+    np.fill_diagonal(D, 1.0)
+    for r, cnt in enumerate((thr - 1, thr, thr + 1, thr + 2)):
+        D[r, :] = 0.0
+        D[r, rng.choice(n, cnt, replace=False)] = 1.0
+    rows, cols = np.nonzero(D)
+    return sp.csc_matrix((rng.standard_normal(rows.size), (rows, cols)), shape=(n, n)), thr
+
+
+@pytest.mark.parametrize("n,seed", [(36, 1), (40, 2), (150, 3), (400, 4)])
+def test_colamd_ata_pattern_brute_force(n, seed):
+    """The A^T A pattern of colamd_order (src/amd.jl:91-104): a dense boolean product over the
+    rows of A with at most thr entries, diagonal dropped.  Pins the oracle's construction; the
+    library's (mf_analysis.cpp) then gives the same COLAMD q as the oracle on these matrices."""
+    A, thr = _threshold_edge_matrix(n, seed)
+    rc = np.diff(A.tocsr().indptr)
+    assert rc[0] == thr - 1 and rc[1] == thr and rc[2] == thr + 1 and rc[3] == thr + 2
+    P = (A.toarray() != 0).astype(np.int64)
+    keep = rc <= thr
+    C = (P[keep].T @ P[keep]) > 0
+    np.fill_diagonal(C, False)
+    cp, ci = OA.ata_pattern(A)
+    got = np.zeros((n, n), bool)
+    for j in range(n):
+        got[ci[cp[j]:cp[j + 1]], j] = True
+        assert np.all(np.diff(ci[cp[j]:cp[j + 1]]) > 0)
+    assert np.array_equal(got, C)
+    # the boundary matters: dropping the row with exactly thr entries (an off-by-one threshold)
+    # or keeping the row with thr + 1 gives another pattern
+    for wrong in (rc <= thr - 1, rc <= thr + 1):
+        W = (P[wrong].T @ P[wrong]) > 0
+        np.fill_diagonal(W, False)
+        assert not np.array_equal(W, C)
+    G = I.symbolic_mf(A, ordering="colamd")
+    assert np.array_equal(G.qf, OY.symbolic_mf(A, q=OA.colamd_order(A)).qf)

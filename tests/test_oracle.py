@@ -200,3 +200,103 @@ def test_planted_tiebreak_is_caught():
     V = np.array([[1.0, 2.0, 0.0], [-1.0, 3.0, 1.0], [1.0, 0.0, 4.0]])
     perm, _, _ = ON.dense_restricted_lu(V, [0, 3])
     assert perm[0] == 0                     # first of the tied rows 0, 1, 2
+
+
+# ---- pivoting: matrices whose in-block partial pivoting interchanges rows ----
+# (the reference's sprand(400, 400, 12/400) + I family of test/pureumfpack_test.jl:106, dense
+# random fronts, +-1 matrices with exact magnitude ties, and the batched workload's unsymmetric
+# values with a weakened diagonal)
+
+PIVOT_MATS = {
+    "sprand-400": (lambda: M.sprand_identity(400, seed=1), OS.SCALE_SUM),
+    "sprand-1000": (lambda: M.sprand_identity(1000, seed=2), OS.SCALE_SUM),
+    "dense-40": (lambda: M.dense_random(40, seed=5), OS.SCALE_SUM),
+    "dense-100": (lambda: M.dense_random(100, seed=3), OS.SCALE_SUM),
+    "signs-60": (lambda: M.signs(60, 0.3, 1), OS.SCALE_NONE),
+    "signs-150": (lambda: M.signs(150, 0.5, 3), OS.SCALE_NONE),
+    "weak-8": (lambda: M.poisson3d_weak(8), OS.SCALE_SUM),
+}
+
+
+def _restricted(A, scale, **kw):
+    F = ON.splu_mf(A, scale=scale)
+    As = A if scale == OS.SCALE_NONE else OS.apply_row_scaling(A, F.Rs)
+    S = OY.symbolic_mf(As)
+    V = As[S.qf, :][:, S.qf]
+    return F, As, V, S, ON.dense_restricted_lu(V, S.sstart, **kw)
+
+
+@pytest.mark.parametrize("name", list(PIVOT_MATS))
+def test_pivoting_matches_dense_restricted_elimination(name):
+    """The oracle's prow equals the brute-force restricted elimination's on matrices where the
+    pivoting moves rows (src/multifrontal.jl:247-275,363-375), and its L (relabelled through
+    invperm(prow), :347-350) and U equal the dense factors."""
+    A, scale = PIVOT_MATS[name]
+    A = A()
+    stats = {}
+    F, As, V, S, (perm, Ld, Ud) = _restricted(A, scale, stats=stats)
+    n = A.shape[0]
+    moved = int((F.prow != np.arange(n)).sum())
+    assert moved > 0 and stats["swaps"] > 0
+    assert np.array_equal(perm, F.prow)
+    assert np.array_equal(F.p, np.asarray(S.qf)[perm])
+    # pivoting restricted to a supernode's rows bounds neither L21 nor the growth of U (weak-8:
+    # max |L| ~ 110); the blocked and the dense elimination round differently, so the values
+    # agree relative to the largest factor entry
+    assert np.abs(F.L.toarray() - Ld).max() <= 1e-12 * np.abs(Ld).max()
+    assert np.abs(F.U.toarray() - Ud).max() <= 1e-12 * np.abs(Ud).max()
+    rec = abs(As[F.p, :][:, F.q] - F.L @ F.U).max() / abs(As).max()
+    assert rec <= 1e-14
+
+
+@pytest.mark.parametrize("name", list(PIVOT_MATS))
+def test_pivoting_blas_backend_matches_loops(name):
+    """The OpenBLAS backend (the reference's own getrf/trsm/gemm calls) and the loop backend give
+    the same interchanges on matrices that pivot."""
+    A, scale = PIVOT_MATS[name]
+    A = A()
+    F0 = ON.splu_mf(A, scale=scale, backend=0)
+    F1 = ON.splu_mf(A, scale=scale, backend=1)
+    assert (F0.prow != np.arange(A.shape[0])).any()
+    assert np.array_equal(F0.p, F1.p) and np.array_equal(F0.prow, F1.prow)
+    assert np.array_equal(F0.L.indices, F1.L.indices) and np.array_equal(F0.U.indices, F1.U.indices)
+    assert np.abs(F0.L.data - F1.L.data).max() <= 1e-12 * np.abs(F0.L.data).max()
+    assert np.abs(F0.U.data - F1.U.data).max() <= 1e-12 * np.abs(F0.U.data).max()
+
+
+@pytest.mark.parametrize("name", ["signs-60", "signs-150"])
+def test_pivoting_ties_take_the_first_maximum(name):
+    """+-1 matrices under SCALE_NONE: the pivot columns hold small integers, so exact ties of the
+    maximum magnitude occur and the elimination is exact.  The oracle must take the first
+    row of maximum magnitude (BLAS idamax, src/multifrontal.jl:248 lu!(A11, RowMaximum()))."""
+    A, scale = PIVOT_MATS[name]
+    A = A()
+    stats = {}
+    F, As, V, S, (perm, Ld, Ud) = _restricted(A, scale, stats=stats)
+    assert stats["ties"] > 0
+    assert np.array_equal(perm, F.prow)
+    assert np.array_equal(F.L.toarray(), Ld) and np.array_equal(F.U.toarray(), Ud)
+    # planted mistake: the last-maximum rule changes the interchanges, so the tests above see it
+    perm_last, _, _ = ON.dense_restricted_lu(V, S.sstart, tie="last")
+    assert not np.array_equal(perm_last, F.prow)
+
+
+def test_pivoting_planted_unswapped_r12_is_caught():
+    """Planted mistake: dropping the R12 row interchanges (src/multifrontal.jl:256-263) breaks the
+    reconstruction on a matrix that pivots."""
+    A = M.sprand_identity(400, seed=1)
+    F = ON.splu_mf(A)
+    As = OS.apply_row_scaling(A, F.Rs)
+    S = OY.symbolic_mf(As)
+    V = As[S.qf, :][:, S.qf].toarray()
+    W = V.copy()
+    n = W.shape[0]
+    for s in range(S.nsuper):
+        c1, c2 = S.sstart[s], S.sstart[s + 1] - 1
+        for j in range(c1, c2 + 1):
+            p = j + int(np.argmax(np.abs(W[j:c2 + 1, j])))
+            if p != j:
+                W[[j, p], :c2 + 1] = W[[p, j], :c2 + 1]      # the columns beyond c2 keep their rows
+            W[j + 1:, j] /= W[j, j]
+            W[j + 1:, j + 1:] -= np.outer(W[j + 1:, j], W[j, j + 1:])
+    assert np.abs(np.triu(W) - F.U.toarray()).max() > 1e-3

@@ -60,3 +60,22 @@ def test_tile_policy_argument_checks():
     got = np.ones(2 * nb[0], np.int32)
     assert lib.mf_get_gemm_tiles(1, got.ctypes.data_as(_lib.c_i32p)) == 0
     assert np.array_equal(got, ok)
+
+
+def test_oracle_adam_closed_form_steps():
+    """Adam's first step with bias correction is p - lr g / (|g| + eps) for every parameter
+    (m_hat = g, v_hat = g^2).  The second step, from the two gradients g1 and g2, uses
+    m_hat = (0.09 g1 + 0.1 g2) / (1 - 0.9^2) and v_hat = (0.000999 g1^2 + 0.001 g2^2) /
+    (1 - 0.999^2).  An Adam without the bias correction moves the first step by
+    0.1 lr / sqrt(0.001), about 3.2 lr, and fails."""
+    X, t = _data(din=5)
+    p0 = np.random.default_rng(4).standard_normal(OM.num_params(5, 6)) * 0.4
+    lr, eps = 1e-3, 1e-8
+    _, g1 = OM.loss_and_grad(p0, X, t, 5, 6)
+    p1, _ = OM.adam(p0, X, t, 5, 6, epochs=1, lr=lr)
+    assert np.allclose(p1, p0 - lr * g1 / (np.abs(g1) + eps), rtol=0, atol=1e-15)
+    _, g2 = OM.loss_and_grad(p1, X, t, 5, 6)
+    p2, _ = OM.adam(p0, X, t, 5, 6, epochs=2, lr=lr)
+    m2 = (0.9 * 0.1 * g1 + 0.1 * g2) / (1 - 0.9 ** 2)
+    v2 = (0.999 * 0.001 * g1 * g1 + 0.001 * g2 * g2) / (1 - 0.999 ** 2)
+    assert np.allclose(p2, p1 - lr * m2 / (np.sqrt(v2) + eps), rtol=0, atol=1e-14)
