@@ -161,19 +161,32 @@ int mf_export_factors_batch(mf_plan *P, int64_t mat, int64_t *Lcolptr, int64_t *
                             int64_t *Ucolptr, int64_t *Urowval, double *Unzval, int64_t *p, int64_t *q);
 
 /* Statistics of the plan: factor flops, Schur-gemm flops, flops in fronts with nf >= 1024,
- * front storage bytes, number of levels, kernel launches per factorization. */
+ * device bytes of the fronts, number of levels, kernel launches per factorization. */
 typedef struct mf_plan_stats {
-    int64_t last_schedule;   /* 0: pivoted panels ran last, 1: interchange-free blocks held */
+    int64_t last_schedule;   /* 0: some matrix of the last call ran the pivoted panels,
+                                1: the interchange-free blocks held for every matrix */
     double factor_flops;
     double gemm_flops;
     double gemm_flops_big;   /* Schur gemm flops of fronts with nf >= 1024 */
-    double front_bytes;
+    double front_bytes;      /* store_bytes + arena_bytes */
     int64_t nsuper;
     int64_t nlevels;
     int64_t launches;
     int64_t max_nf;
+    double store_bytes;      /* factor store: L, U12 and the pivot-block inverses (all matrices) */
+    double arena_bytes;      /* workspace arena: fronts and contribution blocks, liveness-planned */
+    double resident_front_bytes; /* what every nf x nf front kept resident would need */
+    int64_t last_redone;     /* matrices the last call factored on the pivoted panels */
 } mf_plan_stats;
 int mf_plan_get_stats(const mf_plan *P, mf_plan_stats *st);
+/* The statistics mf_plan_create would give (sizes of the factor store and the workspace
+ * arena, flops, launches), computed on the host without touching the GPU: for sizing a
+ * problem before committing device memory.  last_schedule reports the schedule a call would
+ * try first.  No reference counterpart. */
+int mf_plan_estimate(int64_t n, const int64_t *qf, int64_t nsuper, const int64_t *sstart,
+                     const int64_t *upd_ptr, const int64_t *upd_idx, const int64_t *child_ptr,
+                     const int64_t *child_idx, const int64_t *Ap, const int64_t *Ai,
+                     const mf_options *opt, mf_plan_stats *st);
 
 /* Per-kernel-class timing with HIP events on the plan's stream (instrumentation; no
  * reference counterpart).  While profiling is on, mf_factor* and mf_solve* record an event

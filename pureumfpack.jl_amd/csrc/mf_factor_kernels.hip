@@ -10,7 +10,8 @@ namespace mf {
 // extend-add the children's Schur blocks in reverse child order (:235-243, :45-57), then
 // right-looking elimination over all nf rows with the pivot search confined to the np pivot
 // rows (:247-252).  Eliminating the L21 rows and the R12 columns in the same sweep gives
-// L21 = A21 U11^-1, U12 = L11^-1 P A12 and F22 - L21 U12 (:254-268).
+// L21 = A21 U11^-1, U12 = L11^-1 P A12 and F22 - L21 U12 (:254-268).  The panel and U12 go
+// to the factor store, F22 to this front's contribution block in the arena (ld nu).
 // ------------------------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_fused_small(FactorArgs a)
@@ -18,12 +19,14 @@ __global__ __launch_bounds__(256) void k_fused_small(FactorArgs a)
     extern __shared__ unsigned char smem_raw[];
     T *F = reinterpret_cast<T *>(smem_raw);
     const int mat = blockIdx.y;
+    if (skip_matrix(a.only, mat)) return;
     const SnDev d = a.sn[a.list[blockIdx.x]];
-    const int nf = d.nf, np = d.np;
+    const int nf = d.nf, np = d.np, nu = d.nu;
     int *orig = reinterpret_cast<int *>(F + nf * nf);
     __shared__ int s_piv;
     const int tid = threadIdx.x, nt = blockDim.x;
-    T *pool = reinterpret_cast<T *>(a.pool) + mat * a.pool_stride;
+    T *ws = reinterpret_cast<T *>(a.ws) + mat * a.ws_stride;
+    T *store = reinterpret_cast<T *>(a.store) + mat * a.store_stride;
     const double *ax = a.ax + mat * a.ax_stride;
     const double *rs = a.rs ? a.rs + mat * a.rs_stride : nullptr;
 
@@ -38,11 +41,11 @@ __global__ __launch_bounds__(256) void k_fused_small(FactorArgs a)
     for (int ci = 0; ci < d.nchild; ci++) {
         const SnDev c = a.sn[a.child_list[d.child_off + ci]];
         const int m = c.nu;
-        const T *cb = pool + c.foff + c.np + (int64_t)c.np * c.nf;
+        const T *cb = ws + c.cboff;
         const int32_t *rm = a.rmap + c.upd_off;
         for (int idx = tid; idx < m * m; idx += nt) {
             int r = idx % m, s = idx / m;
-            F[rm[r] + rm[s] * nf] += cb[r + (int64_t)s * c.nf];
+            F[rm[r] + rm[s] * nf] += cb[r + (int64_t)s * c.cbld];
         }
         __syncthreads();
     }
@@ -91,8 +94,12 @@ __global__ __launch_bounds__(256) void k_fused_small(FactorArgs a)
         }
         __syncthreads();
     }
-    T *Fg = pool + d.foff;
-    for (int i = tid; i < nf * nf; i += nt) Fg[i] = F[i];
+    T *P = store + d.foff;             // [L11\U11; L21]: the first np columns, ld nf
+    for (int i = tid; i < nf * np; i += nt) P[i] = F[i];
+    T *U = store + d.u12off;           // U12, ld np
+    for (int i = tid; i < np * nu; i += nt) U[i] = F[(i % np) + (np + i / np) * nf];
+    T *CB = ws + d.cboff;              // F22, ld nu
+    for (int i = tid; i < nu * nu; i += nt) CB[i] = F[(np + i % nu) + (np + i / nu) * nf];
     int32_t *prow = a.prow + mat * a.n_stride;
     for (int i = tid; i < np; i += nt) prow[d.c1 + i] = orig[i];
 }
@@ -109,11 +116,12 @@ __global__ __launch_bounds__(256) void k_asm_cols(FactorArgs a)
     extern __shared__ unsigned char smem_raw[];
     T *col = reinterpret_cast<T *>(smem_raw);
     const int mat = blockIdx.y;
+    if (skip_matrix(a.only, mat)) return;
     const int2 w = reinterpret_cast<const int2 *>(a.list)[blockIdx.x];
     const SnDev d = a.sn[w.x];
     const int c = w.y, nf = d.nf;
     const int tid = threadIdx.x, nt = blockDim.x;
-    T *pool = reinterpret_cast<T *>(a.pool) + mat * a.pool_stride;
+    T *ws = reinterpret_cast<T *>(a.ws) + mat * a.ws_stride;
     const double *ax = a.ax + mat * a.ax_stride;
     const double *rs = a.rs ? a.rs + mat * a.rs_stride : nullptr;
 
@@ -126,12 +134,12 @@ __global__ __launch_bounds__(256) void k_asm_cols(FactorArgs a)
     for (int q = a.ext_colptr[fc]; q < a.ext_colptr[fc + 1]; q++) {
         const int2 e = a.ext[q];
         const SnDev ch = a.sn[e.x];
-        const T *src = pool + ch.foff + ch.np + (int64_t)(ch.np + e.y) * ch.nf;
+        const T *src = ws + ch.cboff + (int64_t)e.y * ch.cbld;
         const int32_t *rm = a.rmap + ch.upd_off;
         for (int r = tid; r < ch.nu; r += nt) col[rm[r]] += src[r];
         __syncthreads();
     }
-    T *dst = pool + d.foff + (int64_t)c * nf;
+    T *dst = ws + d.woff + (int64_t)c * nf;
     for (int i = tid; i < nf; i += nt) dst[i] = col[i];
     if (c < d.np && tid == 0) a.prow[mat * a.n_stride + d.c1 + c] = d.c1 + c;
 }
@@ -152,13 +160,14 @@ __global__ __launch_bounds__(NT) void k_panel(FactorArgs a, int k)
     constexpr int KB = PANEL_KB;
     constexpr int NW = NT / 64;
     const int mat = blockIdx.y;
+    if (skip_matrix(a.only, mat)) return;
     const SnDev d = a.sn[a.list[blockIdx.x]];
     const int np = d.np, nf = d.nf;
     const int kb = min(KB, np - k);
     const int m = np - k;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6;
-    T *F = reinterpret_cast<T *>(a.pool) + mat * a.pool_stride + d.foff;
+    T *F = reinterpret_cast<T *>(a.ws) + mat * a.ws_stride + d.woff;
 
     __shared__ T s_redv[2][NW];
     __shared__ int s_redi[2][NW];
@@ -306,12 +315,13 @@ __global__ __launch_bounds__(256) void k_panel_apply(FactorArgs a, int k)
     __shared__ T s_blk[KB * KB];
     __shared__ int s_dst[2 * KB], s_src[2 * KB], s_cnt;
     const int mat = blockIdx.y;
+    if (skip_matrix(a.only, mat)) return;
     const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
     const SnDev d = a.sn[w.x];
     const int np = d.np, nf = d.nf;
     const int kb = min(KB, np - k);
     const int tid = threadIdx.x;
-    T *F = reinterpret_cast<T *>(a.pool) + mat * a.pool_stride + d.foff;
+    T *F = reinterpret_cast<T *>(a.ws) + mat * a.ws_stride + d.woff;
     for (int e = tid; e < KB * KB; e += blockDim.x) {
         const int i = e % KB, j = e / KB;
         s_blk[e] = (i < kb && j < kb) ? F[(k + i) + (int64_t)(k + j) * nf] : T(0);
@@ -473,7 +483,7 @@ __device__ __forceinline__ void spec_diag_front(const FactorArgs &a, int k, floa
     const int np = d.np, nf = d.nf;
     const int kb = min(KB, np - k);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    T *F = reinterpret_cast<T *>(a.pool) + mat * a.pool_stride + d.foff;
+    T *F = reinterpret_cast<T *>(a.ws) + mat * a.ws_stride + d.woff;
     T *S = reinterpret_cast<T *>(a.spec) + mat * a.spec_stride + (int64_t)slot * SPEC_SLOT;
     const T itau = T(1) / T(tau);
     if (kb == KB) {
@@ -515,6 +525,7 @@ template <typename T, int KB>
 __global__ __launch_bounds__(256) void k_spec_diag(FactorArgs a, int k, float tau)
 {
     __shared__ T lbuf[2 * KB];
+    if (skip_matrix(a.only, blockIdx.y)) return;
     spec_diag_front<T, KB>(a, k, tau, blockIdx.x, lbuf);
 }
 
@@ -523,12 +534,12 @@ __global__ __launch_bounds__(256) void k_spec_diag(FactorArgs a, int k, float ta
 template <typename T>
 __device__ __forceinline__ T *F_row_base(const FactorArgs &a, int mat, const SnDev &d, int idx, int k)
 {
-    return reinterpret_cast<T *>(a.pool) + mat * a.pool_stride + d.foff + idx + (int64_t)k * d.nf;
+    return reinterpret_cast<T *>(a.ws) + mat * a.ws_stride + d.woff + idx + (int64_t)k * d.nf;
 }
 template <typename T>
 __device__ __forceinline__ T *F_col_base(const FactorArgs &a, int mat, const SnDev &d, int idx, int k)
 {
-    return reinterpret_cast<T *>(a.pool) + mat * a.pool_stride + d.foff + (int64_t)idx * d.nf + k;
+    return reinterpret_cast<T *>(a.ws) + mat * a.ws_stride + d.woff + (int64_t)idx * d.nf + k;
 }
 
 // value of lane o of each group of 4 lanes (a DPP quad_perm broadcast; o a literal)
@@ -610,6 +621,7 @@ __global__ __launch_bounds__(4 * SPEC_ROWS) void k_spec_trsm(FactorArgs a, int k
     constexpr int LDM = KB + 1;
     __shared__ T sm[KB * LDM + 2 * KB];
     const int mat = blockIdx.y;
+    if (skip_matrix(a.only, mat)) return;
     const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
     const SnDev d = a.sn[w.x];
     const int np = d.np, nf = d.nf;
@@ -633,7 +645,7 @@ __global__ __launch_bounds__(4 * SPEC_ROWS) void k_spec_trsm(FactorArgs a, int k
     }
     {
         const T *S = spec + mat * a.spec_stride + (int64_t)w.w * SPEC_SLOT;
-        const T *Fd = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff + k + (int64_t)k * nf;
+        const T *Fd = reinterpret_cast<const T *>(a.ws) + mat * a.ws_stride + d.woff + k + (int64_t)k * nf;
         constexpr int PT = KB * KB / (4 * SPEC_ROWS);
         T v[PT];
 #pragma unroll
@@ -684,12 +696,13 @@ __global__ __launch_bounds__(128) void k_inv_leaf(FactorArgs a)
     __shared__ T sA[B * LD];
     __shared__ T sX[B * LD];
     const int mat = blockIdx.y;
+    if (skip_matrix(a.only, mat)) return;
     const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
     const SnDev d = a.sn[w.x];
     const int np = d.np, nf = d.nf, b0 = w.y;
     const int bs = min(B, np - b0);
-    T *pool = reinterpret_cast<T *>(a.pool) + mat * a.pool_stride;
-    const T *F = pool + d.foff;
+    T *pool = reinterpret_cast<T *>(a.store) + mat * a.store_stride;
+    const T *F = pool + d.foff;   // the stored panel (ld nf)
     const int tid = threadIdx.x;
     const bool upper = tid >= 64;          // threads 0..63: inv(L) columns, 64..127: inv(U) columns
     const int c = tid & 63;
@@ -876,9 +889,10 @@ __device__ __forceinline__ void gemm_tile(const GemmProb *__restrict__ probs, co
 template <typename T, int TM, int TN, int USE>
 __global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs,
                                               const GemmTile *__restrict__ tiles, void *pool_v,
-                                              int64_t pool_stride)
+                                              int64_t pool_stride, const int32_t *__restrict__ only)
 {
     __shared__ T lds[gemm_lds_elems<T, TM, TN>()];
+    if (skip_matrix(only, blockIdx.y)) return;
     gemm_tile<T, TM, TN>(probs, tiles[blockIdx.x], pool_v, pool_stride, lds);
 }
 
@@ -916,8 +930,9 @@ __device__ __forceinline__ float pow2_scale(float mx)
 
 __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmProb *__restrict__ probs,
                                                    const GemmTile *__restrict__ tiles, void *pool_v,
-                                                   int64_t pool_stride)
+                                                   int64_t pool_stride, const int32_t *__restrict__ only)
 {
+    if (skip_matrix(only, blockIdx.y)) return;
     constexpr int TM = 64, TN = 64, BK = 32, WM = 32, WN = 32, FM = 2, FN = 2;
     __shared__ float As[BK][TM + 1];
     __shared__ float Bs[BK][TN + 1];
@@ -1012,6 +1027,30 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmProb *__restrict__ 
         }
 }
 
+// ------------------------------------------------------------------------------------------
+// End of a level: the factored panel [L11\U11; L21] and U12 of the large fronts leave the
+// arena for the factor store (one workgroup per front column, the items of the level's column
+// assembly: int2 {front, column}).  F22 stays in the arena as the contribution block.
+// ------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_store(FactorArgs a)
+{
+    const int mat = blockIdx.y;
+    if (skip_matrix(a.only, mat)) return;
+    const int2 w = reinterpret_cast<const int2 *>(a.list)[blockIdx.x];
+    const SnDev d = a.sn[w.x];
+    const int c = w.y, nf = d.nf, np = d.np;
+    const T *src = reinterpret_cast<const T *>(a.ws) + mat * a.ws_stride + d.woff + (int64_t)c * nf;
+    T *store = reinterpret_cast<T *>(a.store) + mat * a.store_stride;
+    if (c < np) {
+        T *dst = store + d.foff + (int64_t)c * nf;
+        for (int i = threadIdx.x; i < nf; i += blockDim.x) dst[i] = src[i];
+    } else {
+        T *dst = store + d.u12off + (int64_t)(c - np) * np;
+        for (int i = threadIdx.x; i < np; i += blockDim.x) dst[i] = src[i];
+    }
+}
+
 // Look-ahead: the first ndiag workgroups factor the diagonal block of block k of front
 // list[blockIdx.x] (one wave; the other lanes leave at once), dispatched first; the rest apply
 // the bulk of block k-1's trailing update (64 x 64 gemm tiles the next panel does not read).
@@ -1022,13 +1061,14 @@ __global__ __launch_bounds__(256) void k_spec_diag_gemm(FactorArgs a, int k, flo
                                                         const GemmTile *__restrict__ tiles, int ndiag, int dprob)
 {
     __shared__ T lds[gemm_lds_elems<T, 64, 64>()];
+    if (skip_matrix(a.only, blockIdx.y)) return;
     if ((int)blockIdx.x >= ndiag) {
-        gemm_tile<T, 64, 64>(probs, tiles[blockIdx.x - ndiag], a.pool, a.pool_stride, lds);
+        gemm_tile<T, 64, 64>(probs, tiles[blockIdx.x - ndiag], a.ws, a.ws_stride, lds);
         return;
     }
     if (dprob >= 0) {
         // this diagonal block's own share of the previous block's update, then its factorization
-        gemm_tile<T, 64, 64>(probs, GemmTile{dprob + (int)blockIdx.x, 0, 0, 0}, a.pool, a.pool_stride, lds);
+        gemm_tile<T, 64, 64>(probs, GemmTile{dprob + (int)blockIdx.x, 0, 0, 0}, a.ws, a.ws_stride, lds);
         __syncthreads();
     }
     spec_diag_front<T, KB>(a, k, tau, (int)blockIdx.x, lds);
@@ -1036,21 +1076,21 @@ __global__ __launch_bounds__(256) void k_spec_diag_gemm(FactorArgs a, int k, flo
 
 template <typename T>
 hipError_t launch_gemm_tile(int tile, int use, const GemmProb *probs, const GemmTile *tiles, int count, void *pool,
-                            int64_t pool_stride, int nmat, hipStream_t st)
+                            int64_t pool_stride, int nmat, const int32_t *only, hipStream_t st)
 {
     const dim3 grid(count, nmat), blk(256);
-#define MF_GEMM_CASE(TI, TM, TN)                                                                              \
-    case TI:                                                                                                 \
-        if (use == 2)                                                                                        \
-            hipLaunchKernelGGL((k_gemm<T, TM, TN, 2>), grid, blk, 0, st, probs, tiles, pool, pool_stride);   \
-        else if (use == 1)                                                                                   \
-            hipLaunchKernelGGL((k_gemm<T, TM, TN, 1>), grid, blk, 0, st, probs, tiles, pool, pool_stride);   \
-        else                                                                                                 \
-            hipLaunchKernelGGL((k_gemm<T, TM, TN, 0>), grid, blk, 0, st, probs, tiles, pool, pool_stride);   \
+#define MF_GEMM_CASE(TI, TM, TN)                                                                                    \
+    case TI:                                                                                                       \
+        if (use == 2)                                                                                              \
+            hipLaunchKernelGGL((k_gemm<T, TM, TN, 2>), grid, blk, 0, st, probs, tiles, pool, pool_stride, only);   \
+        else if (use == 1)                                                                                         \
+            hipLaunchKernelGGL((k_gemm<T, TM, TN, 1>), grid, blk, 0, st, probs, tiles, pool, pool_stride, only);   \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_gemm<T, TM, TN, 0>), grid, blk, 0, st, probs, tiles, pool, pool_stride, only);   \
         break;
     if (tile == GEMM_TILE_BF16) {
         if (sizeof(T) != 4) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_gemm_bf16, grid, blk, 0, st, probs, tiles, pool, pool_stride);
+        hipLaunchKernelGGL(k_gemm_bf16, grid, blk, 0, st, probs, tiles, pool, pool_stride, only);
         return hipGetLastError();
     }
     switch (tile) {
@@ -1064,9 +1104,9 @@ hipError_t launch_gemm_tile(int tile, int use, const GemmProb *probs, const Gemm
     return hipGetLastError();
 }
 template hipError_t launch_gemm_tile<double>(int, int, const GemmProb *, const GemmTile *, int, void *, int64_t,
-                                             int, hipStream_t);
+                                             int, const int32_t *, hipStream_t);
 template hipError_t launch_gemm_tile<float>(int, int, const GemmProb *, const GemmTile *, int, void *, int64_t,
-                                            int, hipStream_t);
+                                            int, const int32_t *, hipStream_t);
 
 // ------------------------------------------------------------------------------------------
 // Host launchers
@@ -1115,9 +1155,11 @@ hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int
         break;
     case L_GEMM:
         return launch_gemm_tile<T>(L.tile, L.variant == 3 ? 2 : L.variant == 1 ? 1 : 0, probs + L.prob_off,
-                                   tiles + L.list_off, L.count,
-                                   base.pool,
-                                   base.pool_stride, nmat, st);
+                                   tiles + L.list_off, L.count, L.on_store ? base.store : base.ws,
+                                   L.on_store ? base.store_stride : base.ws_stride, nmat, base.only, st);
+    case L_STORE:
+        hipLaunchKernelGGL(k_store<T>, grid, dim3(256), 0, st, a);
+        break;
     default:
         return hipErrorInvalidValue;
     }

@@ -21,19 +21,30 @@ struct Symbolic {
 bool analyze(i64 n, const i64 *Ap, const i64 *Ai, int ordering, const i64 *q_in, Symbolic &out,
              std::string &err);
 
-// Per-supernode record in device memory.
+// Per-supernode record in device memory.  Two storage areas per matrix:
+//   * the factor store keeps what the solves and the export read for the life of the plan:
+//     the panel [L11\U11; L21] (nf x np, column-major, ld nf) at foff and U12 (np x nu,
+//     column-major, ld np) at u12off, plus the inverses of large pivot blocks at ioff;
+//   * the workspace arena holds a front while its level is factored (nf x nf, ld nf, at woff)
+//     and its contribution block F22 until the parent has assembled it (at cboff, leading
+//     dimension cbld).  Arena offsets come from a liveness plan over the etree levels, so
+//     fronts whose lifetimes do not overlap share memory (src/multifrontal.jl:180-187,319-332
+//     keeps a LIFO arena for the same reason).
 struct SnDev {
-    i64 foff;         // element offset of the front (column-major, ld = nf) in one matrix's pool
+    i64 foff;          // factor store: panel [L11\U11; L21]
+    i64 u12off;        // factor store: U12
+    i64 woff;          // arena: whole front during its level; -1: the fused kernel keeps it in LDS
+    i64 cboff;         // arena: contribution block
+    i64 ioff;          // >= 0: inv(L11) at ioff and inv(U11) at ioff + np*np (ld np) in the store
     int32_t c1, np, nu, nf;
-    int32_t upd_off;  // offset of this supernode's rows in upd[] / rmap[] / uoff[]
-    int32_t orig_off; // offset of this front's original entries (sorted by column)
+    int32_t upd_off;   // offset of this supernode's rows in upd[] / rmap[] / uoff[]
+    int32_t orig_off;  // offset of this front's original entries (sorted by column)
     int32_t norig;
     int32_t child_off; // offset in child list (children stored in REVERSE order)
     int32_t nchild;
-    int32_t fcol;     // index of this front's first column in the per-front-column arrays
+    int32_t fcol;      // index of this front's first column in the per-front-column arrays
+    int32_t cbld;      // leading dimension of the contribution block (nf in a whole front, nu alone)
     int32_t pad;
-    i64 ioff;         // >= 0: inv(L11) at ioff and inv(U11) at ioff + np*np (ld np) in the pool
-    i64 pad2;
 };
 
 // Grouped gemm problem, column-major, offsets in elements into one matrix's pool:
@@ -61,6 +72,7 @@ enum LaunchKind : int32_t {
     L_SPEC_TRSM = 6,    // speculative schedule: L rows / U columns of the block + pivot check
     L_INV_LEAF = 7,     // inverses of the 64x64 diagonal blocks of L11 and U11 (large fronts)
     L_SPEC_DIAG_GEMM = 8,  // next diagonal block factorization alongside the bulk of the trailing update
+    L_STORE = 9,        // copy the factored panel and U12 of a level's large fronts to the store
 };
 
 struct Launch {
@@ -73,6 +85,7 @@ struct Launch {
     int32_t shmem;      // dynamic LDS bytes
     int32_t level;
     int32_t tile;       // L_GEMM: tile shape (GEMM_TILE_* index)
+    int32_t on_store;   // L_GEMM: operands in the factor store (pivot-block inversion), else the arena
     int32_t tile_off;   // L_SPEC_DIAG_GEMM: offset of its gemm tiles (its fronts are at list_off)
     int32_t ntiles;     // L_SPEC_DIAG_GEMM: gemm tiles before the diagonal-block workgroups
     int32_t dprob;      // L_SPEC_DIAG_GEMM: problem (from prob_off) of the first diagonal block's

@@ -1,10 +1,11 @@
 // mf_kernels.h -- device kernels of the multifrontal numeric phase (gfx950, wave64).
 //
-// Front storage: every supernode s owns a dense nf x nf column-major block at pool + foff(s)
-// (ld = nf) for the whole life of the plan; rows/cols are [pivot rows c1..c2 | update rows],
-// the pre-pivoting order of src/multifrontal.jl:206-211.  After factorization the block holds
-// L11\U11 (rows permuted by the in-block pivoting), L21, U12 and the Schur complement F22,
-// which is the contribution block its parent reads directly (no CB copy, no LIFO arena).
+// Storage (see SnDev): a front lives in the workspace arena while its level is factored, as a
+// dense nf x nf column-major block (ld nf) with rows/cols [pivot rows c1..c2 | update rows],
+// the pre-pivoting order of src/multifrontal.jl:206-211.  When the level is done the panel
+// [L11\U11; L21] (rows permuted by the in-block pivoting) and U12 go to the factor store, and
+// the Schur complement F22 stays in the arena as the contribution block its parent reads,
+// until the parent has assembled it.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -102,7 +103,7 @@ __device__ __forceinline__ float wave_uniform(float v)
 
 struct FactorArgs {
     const SnDev *sn;
-    const int32_t *list;        // fronts of this launch (or columns as int2 for L_ASM_COLS)
+    const int32_t *list;        // fronts of this launch (or columns as int2 for L_ASM_COLS / L_STORE)
     const int32_t *orig_row, *orig_col, *orig_src, *orig_colptr;
     const int32_t *child_list, *rmap;
     const int32_t *ext_colptr;
@@ -110,20 +111,26 @@ struct FactorArgs {
     const int32_t *arow;
     const double *ax;
     const double *rs;
-    void *pool;
+    void *ws;                   // workspace arena (fronts during their level, contribution blocks)
+    void *store;                // factor store (panels, U12, inverses)
     int32_t *prow;
     int32_t *info;
     int32_t *pinfo;             // per-front panel swap composition (PINFO_STRIDE ints each)
     int32_t *flag;              // per-matrix: set when the speculative schedule needs pivoting
+    const int32_t *only;        // per-matrix: when set, matrices with only[mat] == 0 are skipped
     void *spec;                 // SPEC_SLOT elements per front of a speculative diagonal launch
-    int64_t pool_stride, ax_stride, rs_stride, n_stride, pinfo_stride, spec_stride;
+    int64_t ws_stride, store_stride, ax_stride, rs_stride, n_stride, pinfo_stride, spec_stride;
 };
+
+// true when matrix `mat` of the launch is to be skipped (a redo of only the matrices whose
+// interchange-free attempt failed)
+__device__ __forceinline__ bool skip_matrix(const int32_t *only, int mat) { return only && only[mat] == 0; }
 
 struct SolveArgs {
     const SnDev *sn;
     const int32_t *list;
     const int32_t *child_list, *rmap, *upd;
-    const void *pool;
+    const void *store;   // factor store (SnDev::foff, u12off, ioff)
     const int32_t *prow;
     void *w0;      // permuted, scaled RHS in V-row order (n x nrhs)
     void *y;       // forward result in factor-row order (n x nrhs)
@@ -134,13 +141,13 @@ struct SolveArgs {
     const int32_t *ecolptr;  // extend-add lists of the factorization (per front position)
     const int32_t *eupd;     // their entries as offsets of the child's update rows in uv
     int64_t n, nupd, nrhs;
-    int64_t pool_stride, n_stride, part_stride;
+    int64_t store_stride, n_stride, part_stride;
 };
 
 // ---- launchers (mf_factor_kernels.hip, mf_solve_kernels.hip) ----
 template <typename T>
 hipError_t launch_gemm_tile(int tile, int use, const GemmProb *probs, const GemmTile *tiles, int count, void *pool,
-                            int64_t pool_stride, int nmat, hipStream_t st);
+                            int64_t pool_stride, int nmat, const int32_t *only, hipStream_t st);
 hipError_t launch_copy(void *dst, const void *src, int64_t bytes, hipStream_t st);
 template <typename T>
 hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int32_t *lists,
@@ -173,9 +180,11 @@ hipError_t launch_anorm(const int32_t *rp, const int32_t *vmap, const double *ax
                         double *part, double *out, int nmat, hipStream_t st);
 hipError_t launch_rowfac(const int32_t *prow, int32_t *rowfac, int64_t n, hipStream_t st);
 template <typename T>
-hipError_t launch_export(const SnDev *sn, int64_t nsuper, const T *pool, const int32_t *upd, const int32_t *rowfac,
+hipError_t launch_export(const SnDev *sn, int64_t nsuper, const T *store, const int32_t *upd, const int32_t *rowfac,
                          const int64_t *Lp, const int64_t *Up, const int64_t *uoff, int64_t *Li, double *Lx,
                          int64_t *Ui, double *Ux, hipStream_t st);
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st);
+// p[i] = v where mask[i] != 0 (mask NULL: everywhere)
+hipError_t launch_fill_i32_masked(int32_t *p, int32_t v, const int32_t *mask, int64_t n, hipStream_t st);
 
 }  // namespace mf

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <memory>
 #include <new>
 #include <string>
@@ -99,7 +100,8 @@ struct mf_plan {
     int nbatch = 1;
     // sizes
     int64_t n = 0, nnz = 0, nsuper = 0, nupd = 0, nlevels = 0, maxnf = 0;
-    int64_t pool_elems = 0;
+    int64_t store_elems = 0;   // factor store per matrix: panels | U12 | inverses | inversion temps
+    int64_t ws_elems = 0;      // workspace arena per matrix (liveness-planned peak)
     int64_t nnzL = 0, nnzU = 0;
     double flops = 0, gemm_flops = 0, gemm_flops_big = 0;
     // host copies
@@ -109,8 +111,8 @@ struct mf_plan {
     bool has_large = false;
     int pivoting = MF_PIVOT_AUTO;
     float pivot_tol = 0.f;
-    bool spec_ok = true;            // the last interchange-free attempt held
     int last_sched = 0;
+    int last_redone = 0;            // matrices the last call factored on the pivoted schedule
     std::vector<int32_t> lvl_off, lvl_cnt;  // solve lists per level
     // per level: offsets/counts of the forward/backward split-K part and sum item lists
     std::vector<int32_t> lvl_fpo, lvl_fpc, lvl_fso, lvl_fsc, lvl_bpo, lvl_bpc, lvl_bso, lvl_bsc;
@@ -122,14 +124,15 @@ struct mf_plan {
     // 64 < nf <= FRONT_MAXNF; the level lists above hold only the other fronts
     std::vector<int32_t> lvl_s64o, lvl_s64c, lvl_s1ko, lvl_s1kc;
     std::vector<int32_t> lvl_s1k_small;   // every such front of the level has nf <= 256
-    int64_t front_elems = 0, inv_elems = 0, tmp_elems = 0;  // pool = fronts | inverses | gemm temps
+    int64_t factor_elems = 0, inv_elems = 0, tmp_elems = 0;  // store = factors | inverses | gemm temps
+    int64_t ws_sum_elems = 0;  // what every front kept resident would take (for the stats)
     int64_t part_slots = 0;  // partial-sum slots (256 rows x nrhs each) the solve needs
     std::vector<double> lvl_lflops, lvl_uflops, lvl_lbytes, lvl_ubytes, lvl_vec;
     // device
     DevBuf d_sn, d_upd, d_rmap, d_child, d_orow, d_ocol, d_osrc, d_ocolptr, d_ecolptr, d_ext, d_eupd, d_arow;
     DevBuf d_lists, d_probs, d_tiles, d_solvelist, d_qf;
     DevBuf d_csr_rp, d_csr_ci, d_csr_vmap;
-    DevBuf d_pool, d_prow, d_info, d_ax, d_rs, d_pinfo, d_flag, d_spec;
+    DevBuf d_store, d_ws, d_prow, d_info, d_ax, d_rs, d_pinfo, d_flag, d_spec, d_redo;
     int64_t spec_slots = 0;         // most fronts in one speculative diagonal launch
     DevBuf d_w0, d_y, d_x, d_uv, d_part, d_zbuf, d_B, d_X, d_R, d_norms, d_anorm, d_npart;
     DevBuf d_Lp, d_Up, d_uoff, d_Li, d_Lx, d_Ui, d_Ux, d_rowfac;
@@ -138,9 +141,9 @@ struct mf_plan {
     int nmat_factored = 0;
     int rs_given = 0;
     hipStream_t stream = nullptr;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
-    int graph_nmat[2] = {0, 0};
-    int graph_rs[2] = {0, 0};
+    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+    int graph_nmat[3] = {0, 0, 0};
+    int graph_rs[3] = {0, 0, 0};
     // profiling (mf_plan_set_profiling)
     int prof = 0;
     double prof_ms[MF_PROF_CLASSES] = {0}, prof_flops[MF_PROF_CLASSES] = {0}, prof_bytes[MF_PROF_CLASSES] = {0};
@@ -153,7 +156,7 @@ struct mf_plan {
     ~mf_plan()
     {
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-        for (int g = 0; g < 2; g++)
+        for (int g = 0; g < 3; g++)
             if (graph[g]) (void)hipGraphExecDestroy(graph[g]);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -214,6 +217,7 @@ static int launch_class(const Launch &L)
     case L_SPEC_DIAG: return 2;
     case L_SPEC_TRSM: return 2;
     case L_SPEC_DIAG_GEMM: return 2;
+    case L_STORE: return 1;
     default: return L.variant == 3 ? 7 : L.variant == 1 ? 4 : 3;
     }
 }
@@ -303,7 +307,7 @@ static int choose_panel_variant(int rows)
 
 static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper, const int64_t *sstart1,
                       const int64_t *upd_ptr1, const int64_t *upd_idx1, const int64_t *child_ptr1,
-                      const int64_t *child_idx1, const int64_t *Ap1, const int64_t *Ai1)
+                      const int64_t *child_idx1, const int64_t *Ap1, const int64_t *Ai1, bool device)
 {
     P.n = n;
     P.nsuper = nsuper;
@@ -375,25 +379,128 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
         for (int64_t a = 0; a < np[s]; a++) loc[c1[s] + a] = -1;
         for (int64_t t = 0; t < nu[s]; t++) loc[upd[uoff0[s] + t]] = -1;
     }
-    // front offsets and per-front column bases
-    std::vector<int64_t> foff(nsuper), fcol(nsuper + 1, 0);
+    // factor store: per supernode the panel [L11\U11; L21] (nf x np) then U12 (np x nu)
+    std::vector<int64_t> foff(nsuper), u12off(nsuper), fcol(nsuper + 1, 0);
     int64_t pe = 0;
     for (int64_t s = 0; s < nsuper; s++) {
         foff[s] = pe;
-        pe += nf[s] * nf[s];
+        pe += nf[s] * np[s];
+        u12off[s] = pe;
+        pe += np[s] * nu[s];
         fcol[s + 1] = fcol[s] + nf[s];
         P.maxnf = std::max(P.maxnf, nf[s]);
     }
-    P.front_elems = pe;
-    // large fronts keep inv(L11) and inv(U11) (np x np each) after the fronts
+    P.factor_elems = pe;
+    // large fronts keep inv(L11) and inv(U11) (np x np each) after the factors
     std::vector<int64_t> ioff(nsuper, -1);
     for (int64_t s = 0; s < nsuper; s++)
         if (np[s] > INV_NP) {
             ioff[s] = pe;
             pe += 2 * np[s] * np[s];
         }
-    P.inv_elems = pe - P.front_elems;
+    P.inv_elems = pe - P.factor_elems;
     if (fcol[nsuper] >= INT32_MAX) return fail(MF_BADARG, "too many front columns");
+    // workspace arena: a large front (nf > small_nf) holds nf x nf from its level through its
+    // parent's level (the parent assembles its F22 from there); a small front, factored in LDS,
+    // only its nu x nu contribution block.  Offsets come from a level-order simulation of a
+    // best-fit allocator: blocks freed after level L are reused from level L + 1 on.
+    std::vector<int64_t> woff(nsuper, -1), cboff(nsuper, 0), cbld(nsuper, 1);
+    {
+        std::vector<int64_t> sn_parent(nsuper, -1);
+        for (int64_t s = 0; s < nsuper; s++)
+            for (int64_t c : kids[s]) sn_parent[c] = s;
+        std::vector<std::vector<int64_t>> starts(nlev), ends(nlev);
+        std::vector<int64_t> need(nsuper, 0);
+        for (int64_t s = 0; s < nsuper; s++) {
+            const bool large = nf[s] > P.small_nf;
+            need[s] = large ? nf[s] * nf[s] : nu[s] * nu[s];
+            P.ws_sum_elems += nf[s] * nf[s];
+            if (need[s] == 0) continue;
+            need[s] = (need[s] + 15) / 16 * 16;   // 64-byte granules (FP32)
+            starts[level[s]].push_back(s);
+            ends[sn_parent[s] >= 0 ? level[sn_parent[s]] : level[s]].push_back(s);
+        }
+        std::multimap<int64_t, int64_t> by_size;     // free blocks: size -> offset
+        std::map<int64_t, int64_t> by_off;           // free blocks: offset -> size
+        int64_t top = 0;
+        auto release = [&](int64_t off, int64_t sz) {
+            auto nx = by_off.lower_bound(off);
+            if (nx != by_off.end() && off + sz == nx->first) {   // merge with the next block
+                auto r = by_size.equal_range(nx->second);
+                for (auto it = r.first; it != r.second; ++it)
+                    if (it->second == nx->first) { by_size.erase(it); break; }
+                sz += nx->second;
+                nx = by_off.erase(nx);
+            }
+            if (nx != by_off.begin()) {                           // merge with the previous block
+                auto pv = std::prev(nx);
+                if (pv->first + pv->second == off) {
+                    auto r = by_size.equal_range(pv->second);
+                    for (auto it = r.first; it != r.second; ++it)
+                        if (it->second == pv->first) { by_size.erase(it); break; }
+                    off = pv->first;
+                    sz += pv->second;
+                    by_off.erase(pv);
+                }
+            }
+            if (off + sz == top) { top = off; return; }           // the block ends at the top
+            by_off[off] = sz;
+            by_size.emplace(sz, off);
+        };
+        auto acquire = [&](int64_t sz) -> int64_t {
+            auto it = by_size.lower_bound(sz);
+            if (it == by_size.end()) {
+                const int64_t off = top;
+                top += sz;
+                P.ws_elems = std::max(P.ws_elems, top);
+                return off;
+            }
+            const int64_t bsz = it->first, off = it->second;
+            by_size.erase(it);
+            by_off.erase(off);
+            if (bsz > sz) {
+                by_off[off + sz] = bsz - sz;
+                by_size.emplace(bsz - sz, off + sz);
+            }
+            return off;
+        };
+        std::vector<int64_t> blk(nsuper, -1);
+        for (int64_t L = 0; L < nlev; L++) {
+            if (L > 0)
+                for (int64_t s : ends[L - 1]) release(blk[s], need[s]);
+            std::vector<int64_t> v = starts[L];
+            std::stable_sort(v.begin(), v.end(), [&](int64_t x, int64_t y) { return need[x] > need[y]; });
+            for (int64_t s : v) blk[s] = acquire(need[s]);
+        }
+        // the plan's own check: blocks live at the same level never overlap (sweep over the
+        // levels keeping the live blocks ordered by offset; each insertion meets its neighbours)
+        {
+            std::map<int64_t, int64_t> live;   // offset -> end
+            for (int64_t L = 0; L < nlev; L++) {
+                if (L > 0)
+                    for (int64_t s : ends[L - 1]) live.erase(blk[s]);
+                for (int64_t s : starts[L]) {
+                    const int64_t b = blk[s], e = b + need[s];
+                    if (b < 0 || e > P.ws_elems) return fail(MF_BADARG, "internal: arena block out of range");
+                    auto nx = live.lower_bound(b);
+                    if (nx != live.end() && nx->first < e) return fail(MF_BADARG, "internal: arena blocks overlap");
+                    if (nx != live.begin() && std::prev(nx)->second > b)
+                        return fail(MF_BADARG, "internal: arena blocks overlap");
+                    live[b] = e;
+                }
+            }
+        }
+        for (int64_t s = 0; s < nsuper; s++) {
+            if (nf[s] > P.small_nf) {
+                woff[s] = blk[s];
+                cboff[s] = blk[s] + np[s] + np[s] * nf[s];
+                cbld[s] = nf[s];
+            } else {
+                cboff[s] = blk[s] >= 0 ? blk[s] : 0;
+                cbld[s] = std::max<int64_t>(nu[s], 1);
+            }
+        }
+    }
     // original entries: V = As[qf,qf]; entry (iV, jV) goes to the front of column min(iV,jV)
     // (src/multifrontal.jl:216-231)
     std::vector<int64_t> qinv(n);
@@ -491,6 +598,10 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
     for (int64_t s = 0; s < nsuper; s++) {
         SnDev &d = P.sn[s];
         d.foff = foff[s];
+        d.u12off = u12off[s];
+        d.woff = woff[s];
+        d.cboff = cboff[s];
+        d.cbld = (int32_t)cbld[s];
         d.c1 = (int32_t)c1[s];
         d.np = (int32_t)np[s];
         d.nu = (int32_t)nu[s];
@@ -503,7 +614,6 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
         d.fcol = (int32_t)fcol[s];
         d.pad = 0;
         d.ioff = ioff[s];
-        d.pad2 = 0;
     }
     // flops
     for (int64_t s = 0; s < nsuper; s++) {
@@ -548,6 +658,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             Launch L{};
             L.kind = L_GEMM;
             L.variant = schur;
+            L.on_store = schur == 2 ? 1 : 0;   // the pivot-block inversions work in the store
             L.tile = bf16 ? GEMM_TILE_BF16 : tile;
             for (const GemmProb *g : sel) {
                 L.flops += 2.0 * g->m * (double)g->n * g->k;
@@ -604,6 +715,12 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             L.level = (int32_t)lv;
             if (L.shmem > 160 * 1024) return fail(MF_BADARG, "front too large for column assembly");
             sched.push_back(L);
+            Launch St{};   // appended after the level's factorization below
+            St.kind = L_STORE;
+            St.list_off = L.list_off;
+            St.count = L.count;
+            St.level = (int32_t)lv;
+            for (int64_t s : large) St.bytes += 2.0 * ((double)nf[s] * np[s] + (double)np[s] * nu[s]) * elem;
             int64_t maxnp = 0;
             for (int64_t s : large) maxnp = std::max(maxnp, np[s]);
             if (sp == 1) {
@@ -631,9 +748,9 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                 };
                 auto prob = [&](int64_t s, int64_t r0, int64_t c0, int64_t m, int64_t n, int64_t k, int64_t kb) {
                     GemmProb g{};
-                    g.aoff = foff[s] + r0 + k * nf[s];
-                    g.boff = foff[s] + k + c0 * nf[s];
-                    g.coff = foff[s] + r0 + c0 * nf[s];
+                    g.aoff = woff[s] + r0 + k * nf[s];
+                    g.boff = woff[s] + k + c0 * nf[s];
+                    g.coff = woff[s] + r0 + c0 * nf[s];
                     g.m = (int32_t)m;
                     g.n = (int32_t)n;
                     g.k = (int32_t)kb;
@@ -748,9 +865,9 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                     if (k2 < np[s]) {
                         // rows [k2, nf) x cols [k2, np): pivot-column block incl. L21 rows
                         GemmProb g{};
-                        g.aoff = foff[s] + k2 + k * nf[s];
-                        g.boff = foff[s] + k + k2 * nf[s];
-                        g.coff = foff[s] + k2 + k2 * nf[s];
+                        g.aoff = woff[s] + k2 + k * nf[s];
+                        g.boff = woff[s] + k + k2 * nf[s];
+                        g.coff = woff[s] + k2 + k2 * nf[s];
                         g.m = (int32_t)(nf[s] - k2);
                         g.n = (int32_t)(np[s] - k2);
                         g.k = (int32_t)kb;
@@ -758,9 +875,9 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                         pv.push_back(g);
                         if (nu[s] > 0) {  // rows [k2, np) x cols [np, nf): the U12 block
                             GemmProb h{};
-                            h.aoff = foff[s] + k2 + k * nf[s];
-                            h.boff = foff[s] + k + np[s] * nf[s];
-                            h.coff = foff[s] + k2 + np[s] * nf[s];
+                            h.aoff = woff[s] + k2 + k * nf[s];
+                            h.boff = woff[s] + k + np[s] * nf[s];
+                            h.coff = woff[s] + k2 + np[s] * nf[s];
                             h.m = (int32_t)(np[s] - k2);
                             h.n = (int32_t)nu[s];
                             h.k = (int32_t)kb;
@@ -800,15 +917,16 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             for (int64_t s : large) {
                 if (nu[s] == 0) continue;
                 GemmProb g{};
-                g.aoff = foff[s] + np[s];
-                g.boff = foff[s] + np[s] * nf[s];
-                g.coff = foff[s] + np[s] + np[s] * nf[s];
+                g.aoff = woff[s] + np[s];
+                g.boff = woff[s] + np[s] * nf[s];
+                g.coff = woff[s] + np[s] + np[s] * nf[s];
                 g.m = g.n = (int32_t)nu[s];
                 g.k = (int32_t)np[s];
                 g.lda = g.ldb = g.ldc = (int32_t)nf[s];
                 sv.push_back(g);
             }
             add_gemm(sv, (int)lv, 1);
+            sched.push_back(St);
         }
         if (!small.empty()) {
             Launch L{};
@@ -899,7 +1017,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
         }
     }
     }
-    P.pool_elems = pe + P.tmp_elems;
+    P.store_elems = pe + P.tmp_elems;
     // solve lists: all fronts per level
     std::vector<int32_t> slist;
     P.lvl_off.assign(nlev, 0);
@@ -1055,6 +1173,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
     std::vector<int32_t> qf32(n);
     for (int64_t j = 0; j < n; j++) qf32[j] = (int32_t)P.qf[j];
 
+    if (!device) return MF_OK;   // mf_plan_estimate: sizes and schedule only
     // ---- device uploads
     HIPCHK(upload(P.d_sn, P.sn));
     HIPCHK(upload(P.d_upd, upd));
@@ -1077,8 +1196,11 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
     HIPCHK(upload(P.d_csr_ci, ci));
     HIPCHK(upload(P.d_csr_vmap, vmap));
     HIPCHK(upload(P.d_uoff, P.uoff));
-    HIPCHK(P.d_pool.alloc((size_t)P.pool_elems * P.esize() * P.nbatch));
-    HIPCHK(hipMemset(P.d_pool.p, 0, (size_t)P.pool_elems * P.esize() * P.nbatch));
+    HIPCHK(P.d_store.alloc((size_t)P.store_elems * P.esize() * P.nbatch));
+    HIPCHK(hipMemset(P.d_store.p, 0, (size_t)P.store_elems * P.esize() * P.nbatch));
+    HIPCHK(P.d_ws.alloc((size_t)std::max<int64_t>(P.ws_elems, 16) * P.esize() * P.nbatch));
+    HIPCHK(hipMemset(P.d_ws.p, 0, (size_t)std::max<int64_t>(P.ws_elems, 16) * P.esize() * P.nbatch));
+    HIPCHK(P.d_redo.alloc((size_t)4 * P.nbatch));
     HIPCHK(P.d_prow.alloc((size_t)n * 4 * P.nbatch));
     HIPCHK(P.d_info.alloc((size_t)4 * P.nbatch));
     HIPCHK(P.d_pinfo.alloc((size_t)4 * PINFO_STRIDE * nsuper * P.nbatch));
@@ -1091,6 +1213,34 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
     return MF_OK;
 }
 
+static int plan_options(mf_plan &P, const mf_options *opt)
+{
+    if (opt) {
+        if (opt->precision < MF_FP64 || opt->precision > MF_BF16) return fail(MF_BADARG, "unknown precision");
+        P.prec = opt->precision;
+        if (opt->small_nf > 0) P.small_nf = opt->small_nf;
+        P.use_graph = opt->use_graph;
+        P.nbatch = opt->nbatch > 0 ? opt->nbatch : 1;
+        if (opt->pivoting != MF_PIVOT_AUTO && opt->pivoting != MF_PIVOT_PARTIAL)
+            return fail(MF_BADARG, "unknown pivoting mode");
+        P.pivoting = opt->pivoting;
+        if (opt->pivot_tol < 0.f || opt->pivot_tol > 1.f) return fail(MF_BADARG, "pivot_tol must be in [0, 1]");
+        P.pivot_tol = opt->pivot_tol;
+    }
+    if (P.pivot_tol == 0.f) P.pivot_tol = P.prec == MF_FP64 ? 1.f : 0.1f;
+    if (P.small_nf > 128) return fail(MF_BADARG, "small_nf must be <= 128");
+    return MF_OK;
+}
+
+static int plan_args_ok(int64_t n, const int64_t *qf, int64_t nsuper, const int64_t *sstart, const int64_t *upd_ptr,
+                        const int64_t *child_ptr, const int64_t *Ap)
+{
+    if (n <= 0) return fail(MF_DIM, "matrix must have n >= 1");
+    if (!qf || !sstart || !upd_ptr || !child_ptr || !Ap || nsuper <= 0 || nsuper > n)
+        return fail(MF_BADARG, "missing plan arguments");
+    return MF_OK;
+}
+
 extern "C" int mf_plan_create(int64_t n, const int64_t *qf, int64_t nsuper, const int64_t *sstart,
                               const int64_t *upd_ptr, const int64_t *upd_idx, const int64_t *child_ptr,
                               const int64_t *child_idx, const int64_t *Ap, const int64_t *Ai, const mf_options *opt,
@@ -1098,34 +1248,59 @@ extern "C" int mf_plan_create(int64_t n, const int64_t *qf, int64_t nsuper, cons
 {
     if (!out) return fail(MF_BADARG, "out is NULL");
     *out = nullptr;
-    if (n <= 0) return fail(MF_DIM, "matrix must have n >= 1");
-    if (!qf || !sstart || !upd_ptr || !child_ptr || !Ap || nsuper <= 0 || nsuper > n)
-        return fail(MF_BADARG, "missing plan arguments");
+    int rc = plan_args_ok(n, qf, nsuper, sstart, upd_ptr, child_ptr, Ap);
+    if (rc != MF_OK) return rc;
     std::unique_ptr<mf_plan> P(new (std::nothrow) mf_plan);
     if (!P) return fail(MF_OOM, "host allocation failed");
-    if (opt) {
-        if (opt->precision < MF_FP64 || opt->precision > MF_BF16) return fail(MF_BADARG, "unknown precision");
-        P->prec = opt->precision;
-        if (opt->small_nf > 0) P->small_nf = opt->small_nf;
-        P->use_graph = opt->use_graph;
-        P->nbatch = opt->nbatch > 0 ? opt->nbatch : 1;
-        if (opt->pivoting != MF_PIVOT_AUTO && opt->pivoting != MF_PIVOT_PARTIAL)
-            return fail(MF_BADARG, "unknown pivoting mode");
-        P->pivoting = opt->pivoting;
-        if (opt->pivot_tol < 0.f || opt->pivot_tol > 1.f) return fail(MF_BADARG, "pivot_tol must be in [0, 1]");
-        P->pivot_tol = opt->pivot_tol;
-    }
-    if (P->pivot_tol == 0.f) P->pivot_tol = P->prec == MF_FP64 ? 1.f : 0.1f;
-    if (P->small_nf > 128) return fail(MF_BADARG, "small_nf must be <= 128");
+    if ((rc = plan_options(*P, opt)) != MF_OK) return rc;
     HIPCHK(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
-    int rc;
     try {
-        rc = build_plan(*P, n, qf, nsuper, sstart, upd_ptr, upd_idx, child_ptr, child_idx, Ap, Ai);
+        rc = build_plan(*P, n, qf, nsuper, sstart, upd_ptr, upd_idx, child_ptr, child_idx, Ap, Ai, true);
     } catch (const std::bad_alloc &) {
         return fail(MF_OOM, "host allocation failed while building the plan");
     }
     if (rc != MF_OK) return rc;
     *out = P.release();
+    return MF_OK;
+}
+
+static void fill_stats(const mf_plan *P, mf_plan_stats *st)
+{
+    st->factor_flops = P->flops;
+    st->gemm_flops = P->gemm_flops;
+    st->gemm_flops_big = P->gemm_flops_big;
+    st->front_bytes = (double)(P->store_elems + P->ws_elems) * (double)P->esize() * P->nbatch;
+    st->store_bytes = (double)P->store_elems * (double)P->esize() * P->nbatch;
+    st->arena_bytes = (double)P->ws_elems * (double)P->esize() * P->nbatch;
+    st->resident_front_bytes = (double)P->ws_sum_elems * (double)P->esize() * P->nbatch;
+    st->last_redone = P->last_redone;
+    st->nsuper = P->nsuper;
+    st->nlevels = P->nlevels;
+    st->launches = (int64_t)P->sched[P->last_sched].size();
+    st->last_schedule = P->last_sched;
+    st->max_nf = P->maxnf;
+}
+
+extern "C" int mf_plan_estimate(int64_t n, const int64_t *qf, int64_t nsuper, const int64_t *sstart,
+                                const int64_t *upd_ptr, const int64_t *upd_idx, const int64_t *child_ptr,
+                                const int64_t *child_idx, const int64_t *Ap, const int64_t *Ai, const mf_options *opt,
+                                mf_plan_stats *st)
+{
+    if (!st) return fail(MF_BADARG, "NULL stats");
+    int rc = plan_args_ok(n, qf, nsuper, sstart, upd_ptr, child_ptr, Ap);
+    if (rc != MF_OK) return rc;
+    std::unique_ptr<mf_plan> P(new (std::nothrow) mf_plan);
+    if (!P) return fail(MF_OOM, "host allocation failed");
+    if ((rc = plan_options(*P, opt)) != MF_OK) return rc;
+    try {
+        rc = build_plan(*P, n, qf, nsuper, sstart, upd_ptr, upd_idx, child_ptr, child_idx, Ap, Ai, false);
+    } catch (const std::bad_alloc &) {
+        return fail(MF_OOM, "host allocation failed while building the plan");
+    }
+    if (rc != MF_OK) return rc;
+    P->last_sched = P->has_large && P->pivoting == MF_PIVOT_AUTO ? 1 : 0;
+    fill_stats(P.get(), st);
+    st->last_redone = 0;
     return MF_OK;
 }
 
@@ -1142,21 +1317,17 @@ extern "C" int mf_nnz(const mf_plan *P, int64_t *nnzL, int64_t *nnzU)
 extern "C" int mf_plan_get_stats(const mf_plan *P, mf_plan_stats *st)
 {
     if (!P || !st) return fail(MF_BADARG, "NULL argument");
-    st->factor_flops = P->flops;
-    st->gemm_flops = P->gemm_flops;
-    st->gemm_flops_big = P->gemm_flops_big;
-    st->front_bytes = (double)P->pool_elems * (double)P->esize();
-    st->nsuper = P->nsuper;
-    st->nlevels = P->nlevels;
-    st->launches = (int64_t)P->sched[P->last_sched].size();
-    st->last_schedule = P->last_sched;
-    st->max_nf = P->maxnf;
+    fill_stats(P, st);
     return MF_OK;
 }
 
 // ------------------------------------------------------------------ factor
+// Graph slots: the interchange-free schedule, the pivoted schedule for every matrix, and the
+// pivoted schedule for only the matrices whose interchange-free attempt failed.
+enum { G_SPEC = 0, G_PIV_ALL = 1, G_PIV_REDO = 2 };
+
 template <typename T>
-static int run_factor(mf_plan &P, const double *d_ax, const double *d_rs, int nmat, int sp)
+static int run_factor(mf_plan &P, const double *d_ax, const double *d_rs, int nmat, int sp, bool redo)
 {
     FactorArgs a{};
     a.sn = P.d_sn.as<SnDev>();
@@ -1171,56 +1342,70 @@ static int run_factor(mf_plan &P, const double *d_ax, const double *d_rs, int nm
     a.arow = P.d_arow.as<int32_t>();
     a.ax = d_ax;
     a.rs = d_rs;
-    a.pool = P.d_pool.p;
+    a.ws = P.d_ws.p;
+    a.store = P.d_store.p;
     a.prow = P.d_prow.as<int32_t>();
     a.info = P.d_info.as<int32_t>();
     a.pinfo = P.d_pinfo.as<int32_t>();
     a.flag = P.d_flag.as<int32_t>();
+    // a redo reads the failed attempts' flags (a copy: the pivoted kernels leave them alone,
+    // but the mask must not change under a captured graph's feet either way)
+    a.only = redo ? P.d_redo.as<int32_t>() : nullptr;
     a.spec = P.d_spec.p;
     a.spec_stride = P.spec_slots * SPEC_SLOT;
     a.pinfo_stride = (int64_t)PINFO_STRIDE * P.nsuper;
-    a.pool_stride = P.pool_elems;
+    a.ws_stride = P.ws_elems;
+    a.store_stride = P.store_elems;
     a.ax_stride = P.nnz;
     a.rs_stride = P.n;
     a.n_stride = P.n;
-    HIPCHK(launch_fill_i32(P.d_info.as<int32_t>(), INT32_MAX, nmat, P.stream));
-    HIPCHK(launch_fill_i32(P.d_flag.as<int32_t>(), 0, nmat, P.stream));
+    HIPCHK(launch_fill_i32_masked(P.d_info.as<int32_t>(), INT32_MAX, a.only, nmat, P.stream));
+    if (!redo) HIPCHK(launch_fill_i32(P.d_flag.as<int32_t>(), 0, nmat, P.stream));
     for (const Launch &L : P.sched[sp])
         HIPCHK(profiled(P, launch_class(L), L.flops * nmat, L.bytes * nmat, [&] {
             return launch_factor_step<T>(L, a, P.d_lists.as<int32_t>(), P.d_probs.as<GemmProb>(),
                                          P.d_tiles.as<GemmTile>(), nmat, P.stream);
         }));
-    HIPCHK(launch_anorm(P.d_csr_rp.as<int32_t>(), P.d_csr_vmap.as<int32_t>(), d_ax, P.nnz, P.n,
-                        P.d_npart.as<double>(), P.d_anorm.as<double>(), nmat, P.stream));
+    if (!redo)
+        HIPCHK(launch_anorm(P.d_csr_rp.as<int32_t>(), P.d_csr_vmap.as<int32_t>(), d_ax, P.nnz, P.n,
+                            P.d_npart.as<double>(), P.d_anorm.as<double>(), nmat, P.stream));
     return MF_OK;
 }
 
-// Enqueues schedule sp (directly or through its captured graph).
-static int enqueue_schedule(mf_plan *P, const double *d_ax, const double *d_rs, int nmat, int sp)
+// Enqueues schedule sp for every matrix, or (redo) the pivoted schedule for the matrices whose
+// interchange-free attempt failed, directly or through a captured graph.
+static int enqueue_schedule(mf_plan *P, const double *d_ax, const double *d_rs, int nmat, int sp, bool redo)
 {
     const int rsg = d_rs != nullptr;
+    const int gi = sp == 1 ? G_SPEC : redo ? G_PIV_REDO : G_PIV_ALL;
+    auto run = [&]() {
+        return P->prec == MF_FP64 ? run_factor<double>(*P, d_ax, d_rs, nmat, sp, redo)
+                                  : run_factor<float>(*P, d_ax, d_rs, nmat, sp, redo);
+    };
     if (P->use_graph && !P->prof) {
         // inputs are always read from the plan's own buffers, so a captured graph stays valid
-        if (!P->graph[sp] || P->graph_nmat[sp] != nmat || rsg != P->graph_rs[sp]) {
-            if (P->graph[sp]) { (void)hipGraphExecDestroy(P->graph[sp]); P->graph[sp] = nullptr; }
-            hipGraph_t g;
+        if (!P->graph[gi] || P->graph_nmat[gi] != nmat || rsg != P->graph_rs[gi]) {
+            if (P->graph[gi]) { (void)hipGraphExecDestroy(P->graph[gi]); P->graph[gi] = nullptr; }
+            hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
-            int rc = P->prec == MF_FP64 ? run_factor<double>(*P, d_ax, d_rs, nmat, sp)
-                                        : run_factor<float>(*P, d_ax, d_rs, nmat, sp);
-            hipError_t ec = hipStreamEndCapture(P->stream, &g);
-            if (rc != MF_OK) return rc;
-            HIPCHK(ec);
-            hipError_t ei = hipGraphInstantiate(&P->graph[sp], g, nullptr, nullptr, 0);
+            const int rc = run();
+            const hipError_t ec = hipStreamEndCapture(P->stream, &g);
+            if (rc != MF_OK || ec != hipSuccess) {
+                if (ec == hipSuccess && g) (void)hipGraphDestroy(g);
+                if (rc != MF_OK) return rc;
+                HIPCHK(ec);
+            }
+            const hipError_t ei = hipGraphInstantiate(&P->graph[gi], g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) P->graph[gi] = nullptr;
             HIPCHK(ei);
-            P->graph_nmat[sp] = nmat;
-            P->graph_rs[sp] = rsg;
+            P->graph_nmat[gi] = nmat;
+            P->graph_rs[gi] = rsg;
         }
-        HIPCHK(hipGraphLaunch(P->graph[sp], P->stream));
+        HIPCHK(hipGraphLaunch(P->graph[gi], P->stream));
         return MF_OK;
     }
-    return P->prec == MF_FP64 ? run_factor<double>(*P, d_ax, d_rs, nmat, sp)
-                              : run_factor<float>(*P, d_ax, d_rs, nmat, sp);
+    return run();
 }
 
 static int factor_common(mf_plan *P, const double *d_ax, const double *d_rs, int nmat, int32_t check,
@@ -1228,24 +1413,27 @@ static int factor_common(mf_plan *P, const double *d_ax, const double *d_rs, int
 {
     int rc;
     std::vector<int32_t> info(nmat), flag(nmat, 0);
-    // Interchange-free schedule first when it applies; if any multiplier broke the bound,
-    // the pivoted schedule reruns the whole factorization from the assembly on.
-    bool spec = P->has_large && P->pivoting == MF_PIVOT_AUTO && P->spec_ok;
+    // Every call first tries the interchange-free schedule (when it applies); the matrices whose
+    // multiplier check failed are then factored again on the pivoted schedule, from the assembly
+    // on, while the others keep their factors.  The policy is per call and per matrix: one
+    // matrix that needs interchanges changes nothing for the next call or the rest of a batch.
+    const bool spec = P->has_large && P->pivoting == MF_PIVOT_AUTO;
+    int nbad = nmat;
     if (spec) {
-        if ((rc = enqueue_schedule(P, d_ax, d_rs, nmat, 1)) != MF_OK) return rc;
+        if ((rc = enqueue_schedule(P, d_ax, d_rs, nmat, 1, false)) != MF_OK) return rc;
         HIPCHK(hipMemcpyAsync(flag.data(), P->d_flag.p, (size_t)nmat * 4, hipMemcpyDeviceToHost, P->stream));
         HIPCHK(hipStreamSynchronize(P->stream));
-        bool bad = false;
-        for (int m = 0; m < nmat; m++) bad |= flag[m] != 0;
-        if (bad) {
-            spec = false;
-            P->spec_ok = false;
+        nbad = 0;
+        for (int m = 0; m < nmat; m++) nbad += flag[m] != 0;
+        if (nbad > 0 && nbad < nmat) {
+            HIPCHK(hipMemcpyAsync(P->d_redo.p, P->d_flag.p, (size_t)nmat * 4, hipMemcpyDeviceToDevice, P->stream));
+            if ((rc = enqueue_schedule(P, d_ax, d_rs, nmat, 0, true)) != MF_OK) return rc;
         }
     }
-    if (!spec) {
-        if ((rc = enqueue_schedule(P, d_ax, d_rs, nmat, 0)) != MF_OK) return rc;
-    }
-    P->last_sched = spec ? 1 : 0;
+    if (nbad == nmat)
+        if ((rc = enqueue_schedule(P, d_ax, d_rs, nmat, 0, false)) != MF_OK) return rc;
+    P->last_sched = spec && nbad == 0 ? 1 : 0;
+    P->last_redone = spec ? nbad : nmat;
     P->rs_given = d_rs != nullptr;
     HIPCHK(hipMemcpyAsync(info.data(), P->d_info.p, (size_t)nmat * 4, hipMemcpyDeviceToHost, P->stream));
     HIPCHK(hipStreamSynchronize(P->stream));
@@ -1320,7 +1508,7 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
     a.child_list = P.d_child.as<int32_t>();
     a.rmap = P.d_rmap.as<int32_t>();
     a.upd = P.d_upd.as<int32_t>();
-    a.pool = P.d_pool.p;
+    a.store = P.d_store.p;
     a.prow = P.d_prow.as<int32_t>();
     a.w0 = P.d_w0.p;
     a.y = P.d_y.p;
@@ -1333,7 +1521,7 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
     a.n = n;
     a.nupd = P.nupd;
     a.nrhs = nrhs;
-    a.pool_stride = P.pool_elems;
+    a.store_stride = P.store_elems;
     a.n_stride = n;
     a.part_stride = std::max<int64_t>(P.part_slots, 1) * 256 * nrhs;
     const double e = (double)P.esize();
@@ -1613,13 +1801,13 @@ static int export_factors(mf_plan *P, int64_t mat, int64_t *Lcolptr, int64_t *Lr
         HIPCHK(P->d_Ux.alloc((size_t)P->nnzU * 8));
         HIPCHK(launch_rowfac(d_prow, P->d_rowfac.as<int32_t>(), n, P->stream));
         if (P->prec == MF_FP64)
-            HIPCHK(launch_export<double>(P->d_sn.as<SnDev>(), P->nsuper, P->d_pool.as<double>() + mat * P->pool_elems,
+            HIPCHK(launch_export<double>(P->d_sn.as<SnDev>(), P->nsuper, P->d_store.as<double>() + mat * P->store_elems,
                                          P->d_upd.as<int32_t>(),
                                          P->d_rowfac.as<int32_t>(), P->d_Lp.as<int64_t>(), P->d_Up.as<int64_t>(),
                                          P->d_uoff.as<int64_t>(), P->d_Li.as<int64_t>(), P->d_Lx.as<double>(),
                                          P->d_Ui.as<int64_t>(), P->d_Ux.as<double>(), P->stream));
         else
-            HIPCHK(launch_export<float>(P->d_sn.as<SnDev>(), P->nsuper, P->d_pool.as<float>() + mat * P->pool_elems,
+            HIPCHK(launch_export<float>(P->d_sn.as<SnDev>(), P->nsuper, P->d_store.as<float>() + mat * P->store_elems,
                                         P->d_upd.as<int32_t>(),
                                         P->d_rowfac.as<int32_t>(), P->d_Lp.as<int64_t>(), P->d_Up.as<int64_t>(),
                                         P->d_uoff.as<int64_t>(), P->d_Li.as<int64_t>(), P->d_Lx.as<double>(),
@@ -1738,9 +1926,9 @@ extern "C" int mf_time_gemm(int32_t precision, int64_t m, int64_t n, int64_t k, 
     HIPCHK(hipEventCreate(&e1));
     auto launch = [&]() {
         return esz == 8 ? launch_gemm_tile<double>(tile, 1, dprobs.as<GemmProb>(), dtiles.as<GemmTile>(), (int)tv.size(),
-                                                    pool.p, 0, 1, st)
+                                                    pool.p, 0, 1, nullptr, st)
                         : launch_gemm_tile<float>(tile, 1, dprobs.as<GemmProb>(), dtiles.as<GemmTile>(), (int)tv.size(),
-                                                   pool.p, 0, 1, st);
+                                                   pool.p, 0, 1, nullptr, st);
     };
     hipError_t err = launch();   // warm-up
     if (err == hipSuccess) err = hipEventRecord(e0, st);

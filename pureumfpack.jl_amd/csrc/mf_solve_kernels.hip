@@ -16,6 +16,20 @@ __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n)
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+__global__ void k_fill_i32_masked(int32_t *p, int32_t v, const int32_t *mask, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (!mask || mask[i]) p[i] = v;
+}
+
+hipError_t launch_fill_i32_masked(int32_t *p, int32_t v, const int32_t *mask, int64_t n, hipStream_t st)
+{
+    int blocks = (int)std::min<int64_t>((n + 255) / 256, 1024);
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_fill_i32_masked, dim3(blocks), dim3(256), 0, st, p, v, mask, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st)
 {
     int blocks = (int)std::min<int64_t>((n + 255) / 256, 1024);
@@ -137,7 +151,7 @@ __global__ __launch_bounds__(256) void k_fwd_tri(SolveArgs a)
     const SnDev d = a.sn[a.list[blockIdx.x]];
     if (d.ioff >= 0) return;   // large front: inv(L11) is applied by k_rect_part kind 2
     const int nf = d.nf, np = d.np, c1 = d.c1;
-    const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
+    const T *F = reinterpret_cast<const T *>(a.store) + mat * a.store_stride + d.foff;
     const int32_t *prow = a.prow + mat * a.n_stride;
     const int tid = threadIdx.x, lane = tid & 63;
     const int64_t vb = (int64_t)mat * a.nrhs;
@@ -265,7 +279,7 @@ __global__ __launch_bounds__(256) void k_rect_part(SolveArgs a, int kind)
     const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
     const SnDev d = a.sn[w.x];
     const int nf = d.nf, np = d.np, nu = d.nu, c1 = d.c1;
-    const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
+    const T *F = reinterpret_cast<const T *>(a.store) + mat * a.store_stride + d.foff;
     const int tid = threadIdx.x;
     const int row = w.y + tid;
     const int nrows = kind == 0 ? nu : np, ncols = kind == 1 ? nu : np;
@@ -279,16 +293,16 @@ __global__ __launch_bounds__(256) void k_rect_part(SolveArgs a, int kind)
     const int32_t *prow = a.prow + mat * a.n_stride;
     T *part = reinterpret_cast<T *>(a.part) + mat * a.part_stride + (int64_t)w.w * 256 * a.nrhs;
     const int32_t *upd = a.upd + d.upd_off;
-    // row r of the operand: kind 0 L21 row (np + r) over columns a; kind 1 U12 row r over
-    // columns np + t; kinds 2 / 3 row r of inv(L11) / inv(U11) (ld np)
-    const T *pool = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride;
+    // row r of the operand: kind 0 L21 row (np + r) over columns a (ld nf); kind 1 U12 row r
+    // over columns t (ld np); kinds 2 / 3 row r of inv(L11) / inv(U11) (ld np)
+    const T *store = reinterpret_cast<const T *>(a.store) + mat * a.store_stride;
     const T *rowp;
-    int ld = nf;
+    int ld = np;
     switch (kind) {
-    case 0: rowp = F + (np + row); break;
-    case 1: rowp = F + row + (int64_t)np * nf; break;
-    case 2: rowp = pool + d.ioff + row; ld = np; break;
-    default: rowp = pool + d.ioff + (int64_t)np * np + row; ld = np; break;
+    case 0: rowp = F + (np + row); ld = nf; break;
+    case 1: rowp = store + d.u12off + row; break;
+    case 2: rowp = store + d.ioff + row; break;
+    default: rowp = store + d.ioff + (int64_t)np * np + row; break;
     }
     const int nrhs = (int)a.nrhs;
     for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
@@ -400,7 +414,7 @@ __global__ __launch_bounds__(256) void k_rect_direct(SolveArgs a)
     const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
     const SnDev d = a.sn[w.x];
     const int nf = d.nf, np = d.np, nu = d.nu, c1 = d.c1;
-    const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
+    const T *F = reinterpret_cast<const T *>(a.store) + mat * a.store_stride + d.foff;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int row = w.y + lane;
     const int64_t vb = (int64_t)mat * a.nrhs;
@@ -450,7 +464,7 @@ __global__ __launch_bounds__(256) void k_bwd_tri(SolveArgs a)
     const SnDev d = a.sn[a.list[blockIdx.x]];
     if (d.ioff >= 0) return;  // large front: inv(U11) was applied by k_rect_part kind 3
     const int nf = d.nf, np = d.np, c1 = d.c1;
-    const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
+    const T *F = reinterpret_cast<const T *>(a.store) + mat * a.store_stride + d.foff;
     const int tid = threadIdx.x, lane = tid & 63;
     const int64_t vb = (int64_t)mat * a.nrhs;
     T *x = reinterpret_cast<T *>(a.x) + vb * a.n;
@@ -555,7 +569,7 @@ __device__ __forceinline__ void fwd_front_w(const SolveArgs &a, const int32_t *_
 {
     const SnDev d = a.sn[a.list[slot]];
     const int nf = d.nf, np = d.np, c1 = d.c1;
-    const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
+    const T *F = reinterpret_cast<const T *>(a.store) + mat * a.store_stride + d.foff;
     const int32_t *prow = a.prow + mat * a.n_stride;
     const int p = threadIdx.x & 63;
     const bool on = p < nf;
@@ -610,7 +624,7 @@ __device__ __forceinline__ void bwd_front_w(const SolveArgs &a, int slot, int ma
 {
     const SnDev d = a.sn[a.list[slot]];
     const int nf = d.nf, np = d.np, nu = d.nu, c1 = d.c1;
-    const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
+    const T *F = reinterpret_cast<const T *>(a.store) + mat * a.store_stride + d.foff;
     const int p = threadIdx.x & 63;
     const int64_t vb = (int64_t)mat * a.nrhs;
     const T *y = reinterpret_cast<const T *>(a.y) + vb * a.n;
@@ -625,11 +639,11 @@ __device__ __forceinline__ void bwd_front_w(const SolveArgs &a, int slot, int ma
             v[r] = (r < rc && p < nf) ? (p < np ? y : x)[(int64_t)(r0 + r) * a.n + src] : T(0);
         // s_i -= sum_t U12[i, t] x_upd[t] (the update values sit in lanes np + t)
         T acc[SOLVE_RC] = {};
-        const T *urow = F + p + (int64_t)np * nf;
+        const T *urow = reinterpret_cast<const T *>(a.store) + mat * a.store_stride + d.u12off + p;   // U12, ld np
         for (int t0 = 0; t0 < nu; t0 += 8) {
             T uu[8];
 #pragma unroll
-            for (int q = 0; q < 8; q++) uu[q] = (p < np && t0 + q < nu) ? urow[(int64_t)(t0 + q) * nf] : T(0);
+            for (int q = 0; q < 8; q++) uu[q] = (p < np && t0 + q < nu) ? urow[(int64_t)(t0 + q) * np] : T(0);
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 const int t = t0 + q;
@@ -705,7 +719,7 @@ __device__ __forceinline__ void fwd_front_b(const SolveArgs &a, const int32_t *_
     const int mat = mat_;
     const SnDev d = a.sn[a.list[slot]];
     const int nf = d.nf, np = d.np, c1 = d.c1;
-    const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
+    const T *F = reinterpret_cast<const T *>(a.store) + mat * a.store_stride + d.foff;
     const int32_t *prow = a.prow + mat * a.n_stride;
     const int tid = threadIdx.x, lane = tid & 63;
     const int64_t vb = (int64_t)mat * a.nrhs;
@@ -850,7 +864,8 @@ __device__ __forceinline__ void bwd_front_b(const SolveArgs &a, int slot, int ma
     const int mat = mat_;
     const SnDev d = a.sn[a.list[slot]];
     const int nf = d.nf, np = d.np, nu = d.nu, c1 = d.c1;
-    const T *F = reinterpret_cast<const T *>(a.pool) + mat * a.pool_stride + d.foff;
+    const T *F = reinterpret_cast<const T *>(a.store) + mat * a.store_stride + d.foff;
+    const T *U12 = reinterpret_cast<const T *>(a.store) + mat * a.store_stride + d.u12off;   // ld np
     const int32_t *upd = a.upd + d.upd_off;
     const int tid = threadIdx.x, lane = tid & 63;
     const int64_t vb = (int64_t)mat * a.nrhs;
@@ -893,13 +908,13 @@ __device__ __forceinline__ void bwd_front_b(const SolveArgs &a, int slot, int ma
             const int i = i0 + ti;
             T acc[SOLVE_RC] = {};
             if (i < np) {
-                const T *row = F + i + (int64_t)(np + ts) * nf;
+                const T *row = U12 + i + (int64_t)ts * np;
                 const int cnt = nu > ts ? (nu - ts + ts_n - 1) / ts_n : 0;   // columns ts + k * ts_n
                 int k = 0;
                 for (; k + 16 <= cnt; k += 16) {
                     T u[16];
 #pragma unroll
-                    for (int q = 0; q < 16; q++) u[q] = row[(int64_t)(k + q) * ts_n * nf];
+                    for (int q = 0; q < 16; q++) u[q] = row[(int64_t)(k + q) * ts_n * np];
 #pragma unroll
                     for (int q = 0; q < 16; q++)
 #pragma unroll
@@ -908,14 +923,14 @@ __device__ __forceinline__ void bwd_front_b(const SolveArgs &a, int slot, int ma
                 for (; k + 4 <= cnt; k += 4) {
                     T u[4];
 #pragma unroll
-                    for (int q = 0; q < 4; q++) u[q] = row[(int64_t)(k + q) * ts_n * nf];
+                    for (int q = 0; q < 4; q++) u[q] = row[(int64_t)(k + q) * ts_n * np];
 #pragma unroll
                     for (int q = 0; q < 4; q++)
 #pragma unroll
                         for (int r = 0; r < SOLVE_RC; r++) acc[r] += u[q] * g[r * MAXNF + np + ts + (k + q) * ts_n];
                 }
                 for (; k < cnt; k++) {
-                    const T u = row[(int64_t)k * ts_n * nf];
+                    const T u = row[(int64_t)k * ts_n * np];
 #pragma unroll
                     for (int r = 0; r < SOLVE_RC; r++) acc[r] += u * g[r * MAXNF + np + ts + k * ts_n];
                 }
@@ -1131,7 +1146,7 @@ __global__ void k_rowfac(const int32_t *__restrict__ prow, int32_t *__restrict__
 // One workgroup per supernode: writes its L columns (diagonal first, pivot rows ascending,
 // then its update rows relabelled through rowfac) and its U blocks at precomputed offsets.
 template <typename T>
-__global__ __launch_bounds__(256) void k_export(const SnDev *__restrict__ sn, const T *__restrict__ pool,
+__global__ __launch_bounds__(256) void k_export(const SnDev *__restrict__ sn, const T *__restrict__ store,
                                                 const int32_t *__restrict__ upd, const int32_t *__restrict__ rowfac,
                                                 const int64_t *__restrict__ Lp, const int64_t *__restrict__ Up,
                                                 const int64_t *__restrict__ uoff, int64_t *__restrict__ Li,
@@ -1140,7 +1155,7 @@ __global__ __launch_bounds__(256) void k_export(const SnDev *__restrict__ sn, co
 {
     const SnDev d = sn[blockIdx.x];
     const int nf = d.nf, np = d.np, nu = d.nu, c1 = d.c1;
-    const T *F = pool + d.foff;
+    const T *F = store + d.foff, *U12 = store + d.u12off;
     for (int a = 0; a < np; a++) {
         const int g = c1 + a;
         const int64_t lp = Lp[g];
@@ -1164,7 +1179,7 @@ __global__ __launch_bounds__(256) void k_export(const SnDev *__restrict__ sn, co
         const int64_t up = uoff[d.upd_off + t];
         for (int b = threadIdx.x; b < np; b += blockDim.x) {
             Ui[up + b] = c1 + b + 1;
-            Ux[up + b] = (double)F[b + (int64_t)(np + t) * nf];
+            Ux[up + b] = (double)U12[b + (int64_t)t * np];
         }
     }
 }
@@ -1325,11 +1340,11 @@ hipError_t launch_rowfac(const int32_t *prow, int32_t *rowfac, int64_t n, hipStr
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_export(const SnDev *sn, int64_t nsuper, const T *pool, const int32_t *upd, const int32_t *rowfac,
+hipError_t launch_export(const SnDev *sn, int64_t nsuper, const T *store, const int32_t *upd, const int32_t *rowfac,
                          const int64_t *Lp, const int64_t *Up, const int64_t *uoff, int64_t *Li, double *Lx,
                          int64_t *Ui, double *Ux, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_export<T>, dim3((unsigned)nsuper), dim3(256), 0, st, sn, pool, upd, rowfac, Lp, Up, uoff, Li,
+    hipLaunchKernelGGL(k_export<T>, dim3((unsigned)nsuper), dim3(256), 0, st, sn, store, upd, rowfac, Lp, Up, uoff, Li,
                        Lx, Ui, Ux);
     return hipGetLastError();
 }

@@ -36,7 +36,8 @@ class mf_plan_stats(ctypes.Structure):
     _fields_ = [("last_schedule", ctypes.c_int64), ("factor_flops", ctypes.c_double), ("gemm_flops", ctypes.c_double),
                 ("gemm_flops_big", ctypes.c_double), ("front_bytes", ctypes.c_double),
                 ("nsuper", ctypes.c_int64), ("nlevels", ctypes.c_int64), ("launches", ctypes.c_int64),
-                ("max_nf", ctypes.c_int64)]
+                ("max_nf", ctypes.c_int64), ("store_bytes", ctypes.c_double), ("arena_bytes", ctypes.c_double),
+                ("resident_front_bytes", ctypes.c_double), ("last_redone", ctypes.c_int64)]
 
 
 # exported symbol -> (restype, argtypes)
@@ -84,6 +85,9 @@ SIGNATURES = {
     "mf_mlp_last_error": (ctypes.c_char_p, []),
     "mf_copy_probe": (ctypes.c_int, [ctypes.c_int64, c_f64p]),
     "mf_plan_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(mf_plan_stats)]),
+    "mf_plan_estimate": (ctypes.c_int, [ctypes.c_int64, c_i64p, ctypes.c_int64, c_i64p, c_i64p, c_i64p, c_i64p,
+                                        c_i64p, c_i64p, c_i64p, ctypes.POINTER(mf_options),
+                                        ctypes.POINTER(mf_plan_stats)]),
     "mf_plan_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "mf_plan_get_profile": (ctypes.c_int, [ctypes.c_void_p, c_f64p, c_i64p, c_f64p, c_f64p]),
     "mf_last_error": (ctypes.c_char_p, []),

@@ -275,6 +275,26 @@ class _Plan:
         return L, U, p - 1, q - 1
 
 
+def plan_estimate(A, S: SymbolicMF, precision="fp64", small_nf=0, nbatch=1, pivoting="auto"):
+    """Sizes and schedule statistics of the device plan for A's pattern (factor store, workspace
+    arena, flops, launches), computed on the host by mf_plan_estimate without touching the GPU."""
+    lib = _lib.load()
+    if precision not in _PREC:
+        raise ArgumentError(f"unknown precision {precision}")
+    if pivoting not in _PIVOT:
+        raise ArgumentError(f"unknown pivoting {pivoting}")
+    A = _csc(A)
+    opt = _lib.mf_options(_PREC[precision], int(small_nf), 0, int(nbatch), _PIVOT[pivoting], 0.0)
+    Ap, Ai = _pattern_1based(A)
+    arrs = [np.ascontiguousarray(x, dtype=np.int64) + 1 for x in
+            (S.qf, S.sstart, S.upd_ptr, S.upd_idx, S.child_ptr, S.child_idx)]
+    st = _lib.mf_plan_stats()
+    _check(lib.mf_plan_estimate(A.shape[0], _lib.ptr(arrs[0]), S.nsuper, _lib.ptr(arrs[1]), _lib.ptr(arrs[2]),
+                                _lib.ptr(arrs[3]), _lib.ptr(arrs[4]), _lib.ptr(arrs[5]), _lib.ptr(Ap), _lib.ptr(Ai),
+                                ctypes.byref(opt), ctypes.byref(st)), "mf_plan_estimate")
+    return {f: getattr(st, f) for f, _ in st._fields_}
+
+
 class MFFactorization:
     """What multifrontal_lu returns (GPLUFactorization, src/gplu.jl:45-54): A[p, q] = L U.
     L, U, p and q are exported from the device on first access."""

@@ -124,6 +124,8 @@ def test_batch_redoes_only_the_matrices_that_pivot():
     vals = np.stack([Ax[0], weak.data, Ax[1], weak.data * 1.5, Ax[2]])
     B = I.BatchedLU(P, 5, precision="fp64", symbolic=S, use_graph=False)
     B.refactor(vals)
+    st = B._plan.stats()
+    assert st["last_redone"] == 2 and st["last_schedule"] == 0
     for m in range(5):
         Am = B.matrix(vals[m])
         R = oracle_splu(Am, S)
