@@ -919,13 +919,16 @@ __device__ __forceinline__ float wave_max(float v)
     return v;
 }
 
-// 2^-e with e the exponent of mx (mx in [2^(e-1), 2^e)); 1 for mx == 0
-__device__ __forceinline__ float pow2_scale(float mx)
+// The exponent e of mx (mx in [2^(e-1), 2^e)), 0 for mx == 0.  e is kept >= -100 so that
+// the scale 2^-e stays finite: fill entries of strongly convective operators decay
+// exponentially and reach FP32 subnormals, where 2^-e would overflow (and 0 * inf = NaN).  A
+// slice below 2^-101 then keeps its tiny magnitude, and 2^(ea + eb) flushes its product.
+__device__ __forceinline__ int pow2_exp(float mx)
 {
-    if (!(mx > 0.0f) || !isfinite(mx)) return 1.0f;
+    if (!(mx > 0.0f) || !isfinite(mx)) return 0;
     int e;
     (void)frexpf(mx, &e);
-    return ldexpf(1.0f, -e);
+    return max(e, -100);
 }
 
 __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmProb *__restrict__ probs,
@@ -977,8 +980,9 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmProb *__restrict__ 
             smax[1][wid] = mb;
         }
         __syncthreads();
-        const float sa = pow2_scale(fmaxf(fmaxf(smax[0][0], smax[0][1]), fmaxf(smax[0][2], smax[0][3])));
-        const float sb = pow2_scale(fmaxf(fmaxf(smax[1][0], smax[1][1]), fmaxf(smax[1][2], smax[1][3])));
+        const int ea = pow2_exp(fmaxf(fmaxf(smax[0][0], smax[0][1]), fmaxf(smax[0][2], smax[0][3])));
+        const int eb = pow2_exp(fmaxf(fmaxf(smax[1][0], smax[1][1]), fmaxf(smax[1][2], smax[1][3])));
+        const float sa = ldexpf(1.0f, -ea), sb = ldexpf(1.0f, -eb);
         acc4 sl[FM][FN];
 #pragma unroll
         for (int i = 0; i < FM; i++)
@@ -1007,7 +1011,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmProb *__restrict__ 
                 for (int j = 0; j < FN; j++)
                     sl[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[i], bv[j], sl[i][j], 0, 0, 0);
         }
-        const float un = 1.0f / (sa * sb);   // a power of two: exact
+        const float un = ldexpf(1.0f, ea + eb);   // a power of two: exact (or flushed to zero)
 #pragma unroll
         for (int i = 0; i < FM; i++)
 #pragma unroll

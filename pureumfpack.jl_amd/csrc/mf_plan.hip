@@ -92,6 +92,16 @@ struct mf_symbolic {
     Symbolic S;
 };
 
+// Buffers of one solve call: the sweep vectors in the front precision (w0 permuted scaled
+// RHS, y forward result, x backward result, uv update vectors, part split-K partials, zbuf
+// large-front staging) and the FP64 refinement vectors (B, X, R) with the norm partials.  They
+// are carved from one region: a plan-owned allocation, or the file-scope scratch array of the
+// allocation-free entry point.
+struct SolveBufs {
+    void *w0 = nullptr, *y = nullptr, *x = nullptr, *uv = nullptr, *part = nullptr, *zbuf = nullptr;
+    double *B = nullptr, *X = nullptr, *R = nullptr, *norms = nullptr, *npart = nullptr;
+};
+
 struct mf_plan {
     // options
     int prec = MF_FP64;
@@ -134,7 +144,8 @@ struct mf_plan {
     DevBuf d_csr_rp, d_csr_ci, d_csr_vmap;
     DevBuf d_store, d_ws, d_prow, d_info, d_ax, d_rs, d_pinfo, d_flag, d_spec, d_redo;
     int64_t spec_slots = 0;         // most fronts in one speculative diagonal launch
-    DevBuf d_w0, d_y, d_x, d_uv, d_part, d_zbuf, d_B, d_X, d_R, d_norms, d_anorm, d_npart;
+    DevBuf d_solve, d_anorm, d_npart;   // solve buffers (carved as SolveBufs), ||A||, its partials
+    SolveBufs sb;                       // the buffers the current solve call uses
     DevBuf d_Lp, d_Up, d_uoff, d_Li, d_Lx, d_Ui, d_Ux, d_rowfac;
     int64_t solve_nrhs = 0;
     bool factored = false;
@@ -1498,10 +1509,10 @@ extern "C" int mf_refactor_batch_device(mf_plan *P, int64_t nmat, const double *
 template <typename T>
 static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumulate, int nmat)
 {
-    // d_rhs: n x nrhs per matrix (ld n); result into P.d_X (ld n), overwritten or accumulated
+    // d_rhs: n x nrhs per matrix (ld n); result into P.sb.X (ld n), overwritten or accumulated
     const int64_t n = P.n;
     const double *rs = P.rs_given ? P.d_rs.as<double>() : nullptr;
-    HIPCHK(launch_prep_rhs<T>(d_rhs, n, rs, n, P.d_qf.as<int32_t>(), P.d_w0.as<T>(), n, nrhs, n * nrhs, nmat,
+    HIPCHK(launch_prep_rhs<T>(d_rhs, n, rs, n, P.d_qf.as<int32_t>(), reinterpret_cast<T *>(P.sb.w0), n, nrhs, n * nrhs, nmat,
                               P.stream));
     SolveArgs a{};
     a.sn = P.d_sn.as<SnDev>();
@@ -1510,12 +1521,12 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
     a.upd = P.d_upd.as<int32_t>();
     a.store = P.d_store.p;
     a.prow = P.d_prow.as<int32_t>();
-    a.w0 = P.d_w0.p;
-    a.y = P.d_y.p;
-    a.x = P.d_x.p;
-    a.uv = P.d_uv.p;
-    a.part = P.d_part.p;
-    a.zbuf = P.d_zbuf.p;
+    a.w0 = P.sb.w0;
+    a.y = P.sb.y;
+    a.x = P.sb.x;
+    a.uv = P.sb.uv;
+    a.part = P.sb.part;
+    a.zbuf = P.sb.zbuf;
     a.ecolptr = P.d_ecolptr.as<int32_t>();
     a.eupd = P.d_eupd.as<int32_t>();
     a.n = n;
@@ -1616,7 +1627,7 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
                 return launch_solve_level<T>(5, a, sl, P.lvl_off[lv], P.lvl_cnt[lv], nmat, P.stream);
             }));
     }
-    HIPCHK(launch_finish_x<T>(P.d_x.as<T>(), P.d_qf.as<int32_t>(), P.d_X.as<double>(), n, n, nrhs, accumulate,
+    HIPCHK(launch_finish_x<T>(reinterpret_cast<const T *>(P.sb.x), P.d_qf.as<int32_t>(), P.sb.X, n, n, nrhs, accumulate,
                               n * nrhs, nmat, P.stream));
     return MF_OK;
 }
@@ -1624,7 +1635,7 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
 static int solve_core(mf_plan *P, int64_t nrhs, int nmat, int32_t refine_steps, double bwd_tol,
                       mf_solve_report *rep)
 {
-    // B already in P->d_B (n x nrhs x nmat); result left in P->d_X
+    // B already in P->sb.B (n x nrhs x nmat); result left in P->sb.X
     const int64_t n = P->n;
     auto t0 = std::chrono::steady_clock::now();
     int rc;
@@ -1634,12 +1645,11 @@ static int solve_core(mf_plan *P, int64_t nrhs, int nmat, int32_t refine_steps, 
     };
     auto norms = [&](std::vector<double> &h) -> int {
         HIPCHK(launch_residual(P->d_csr_rp.as<int32_t>(), P->d_csr_ci.as<int32_t>(), P->d_csr_vmap.as<int32_t>(),
-                               P->d_ax.as<double>(), P->nnz, P->d_X.as<double>(), n, P->d_B.as<double>(), n,
-                               P->d_R.as<double>(), n, nrhs, n * nrhs, n * nrhs, nmat, P->stream));
-        HIPCHK(launch_norms(P->d_R.as<double>(), P->d_X.as<double>(), P->d_B.as<double>(), n, nrhs,
-                            P->d_npart.as<double>(), P->d_norms.as<double>(), nmat, P->stream));
+                               P->d_ax.as<double>(), P->nnz, P->sb.X, n, P->sb.B, n,
+                               P->sb.R, n, nrhs, n * nrhs, n * nrhs, nmat, P->stream));
+        HIPCHK(launch_norms(P->sb.R, P->sb.X, P->sb.B, n, nrhs, P->sb.npart, P->sb.norms, nmat, P->stream));
         h.resize((size_t)5 * nrhs * nmat + nmat);
-        HIPCHK(hipMemcpyAsync(h.data(), P->d_norms.p, (size_t)5 * nrhs * nmat * 8, hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipMemcpyAsync(h.data(), P->sb.norms, (size_t)5 * nrhs * nmat * 8, hipMemcpyDeviceToHost, P->stream));
         HIPCHK(hipMemcpyAsync(h.data() + 5 * nrhs * nmat, P->d_anorm.p, (size_t)nmat * 8, hipMemcpyDeviceToHost,
                               P->stream));
         HIPCHK(hipStreamSynchronize(P->stream));
@@ -1658,7 +1668,7 @@ static int solve_core(mf_plan *P, int64_t nrhs, int nmat, int32_t refine_steps, 
             }
         }
     };
-    if ((rc = once(P->d_B.as<double>(), 0)) != MF_OK) return rc;
+    if ((rc = once(P->sb.B, 0)) != MF_OK) return rc;
     int it = 0, conv = 0;
     std::vector<double> h;
     double be = 0, r2 = 0;
@@ -1669,11 +1679,11 @@ static int solve_core(mf_plan *P, int64_t nrhs, int nmat, int32_t refine_steps, 
             if (be <= bwd_tol) { conv = 1; break; }
         } else {
             HIPCHK(launch_residual(P->d_csr_rp.as<int32_t>(), P->d_csr_ci.as<int32_t>(),
-                                   P->d_csr_vmap.as<int32_t>(), P->d_ax.as<double>(), P->nnz, P->d_X.as<double>(), n,
-                                   P->d_B.as<double>(), n, P->d_R.as<double>(), n, nrhs, n * nrhs, n * nrhs, nmat,
+                                   P->d_csr_vmap.as<int32_t>(), P->d_ax.as<double>(), P->nnz, P->sb.X, n,
+                                   P->sb.B, n, P->sb.R, n, nrhs, n * nrhs, n * nrhs, nmat,
                                    P->stream));
         }
-        if ((rc = once(P->d_R.as<double>(), 1)) != MF_OK) return rc;
+        if ((rc = once(P->sb.R, 1)) != MF_OK) return rc;
     }
     if (rep) {
         if ((rc = norms(h)) != MF_OK) return rc;
@@ -1695,21 +1705,37 @@ static int solve_core(mf_plan *P, int64_t nrhs, int nmat, int32_t refine_steps, 
     return MF_OK;
 }
 
+// Carves the solve buffers for (nrhs, nmat) from `base` (256-byte aligned pieces) into *b;
+// returns the bytes needed.  base == nullptr: size only.
+static size_t solve_layout(const mf_plan &P, int64_t nrhs, int nmat, unsigned char *base, SolveBufs *b)
+{
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> void * {
+        void *p = base ? base + off : nullptr;
+        off += (bytes + 255) / 256 * 256;
+        return p;
+    };
+    const size_t e = P.esize(), nv = (size_t)P.n * nrhs * nmat;
+    SolveBufs t;
+    t.w0 = take(nv * e);
+    t.y = take(nv * e);
+    t.x = take(nv * e);
+    t.uv = take((size_t)std::max<int64_t>(P.nupd, 1) * nrhs * nmat * e);
+    t.part = take((size_t)std::max<int64_t>(P.part_slots, 1) * 256 * nrhs * nmat * e);
+    t.zbuf = take(nv * e);
+    t.B = static_cast<double *>(take(nv * 8));
+    t.X = static_cast<double *>(take(nv * 8));
+    t.R = static_cast<double *>(take(nv * 8));
+    t.norms = static_cast<double *>(take((size_t)5 * nrhs * nmat * 8));
+    t.npart = static_cast<double *>(take((size_t)5 * 64 * nrhs * nmat * 8));
+    if (b) *b = t;
+    return off;
+}
+
 static int ensure_solve_buffers(mf_plan *P, int64_t nrhs, int nmat)
 {
-    size_t e = P->esize();
-    size_t nv = (size_t)P->n * nrhs * nmat;
-    HIPCHK(P->d_w0.alloc(nv * e));
-    HIPCHK(P->d_y.alloc(nv * e));
-    HIPCHK(P->d_x.alloc(nv * e));
-    HIPCHK(P->d_uv.alloc((size_t)std::max<int64_t>(P->nupd, 1) * nrhs * nmat * e));
-    HIPCHK(P->d_part.alloc((size_t)std::max<int64_t>(P->part_slots, 1) * 256 * nrhs * nmat * e));
-    HIPCHK(P->d_zbuf.alloc(nv * e));
-    HIPCHK(P->d_B.alloc(nv * 8));
-    HIPCHK(P->d_X.alloc(nv * 8));
-    HIPCHK(P->d_R.alloc(nv * 8));
-    HIPCHK(P->d_norms.alloc((size_t)5 * nrhs * nmat * 8));
-    HIPCHK(P->d_npart.alloc(std::max<size_t>((size_t)8 * 256 * P->nbatch, (size_t)5 * 64 * nrhs * nmat * 8)));
+    HIPCHK(P->d_solve.alloc(solve_layout(*P, nrhs, nmat, nullptr, nullptr)));
+    solve_layout(*P, nrhs, nmat, P->d_solve.as<unsigned char>(), &P->sb);
     return MF_OK;
 }
 
@@ -1722,9 +1748,9 @@ extern "C" int mf_solve_device(mf_plan *P, int64_t nrhs, const double *d_B, int6
     int rc = ensure_solve_buffers(P, nrhs, 1);
     if (rc != MF_OK) return rc;
     const size_t n8 = (size_t)P->n * 8;
-    HIPCHK(hipMemcpy2DAsync(P->d_B.p, n8, d_B, (size_t)ldb * 8, n8, nrhs, hipMemcpyDeviceToDevice, P->stream));
+    HIPCHK(hipMemcpy2DAsync(P->sb.B, n8, d_B, (size_t)ldb * 8, n8, nrhs, hipMemcpyDeviceToDevice, P->stream));
     if ((rc = solve_core(P, nrhs, 1, refine_steps, bwd_tol, rep)) != MF_OK) return rc;
-    HIPCHK(hipMemcpy2DAsync(d_X, (size_t)ldx * 8, P->d_X.p, n8, n8, nrhs, hipMemcpyDeviceToDevice, P->stream));
+    HIPCHK(hipMemcpy2DAsync(d_X, (size_t)ldx * 8, P->sb.X, n8, n8, nrhs, hipMemcpyDeviceToDevice, P->stream));
     HIPCHK(hipStreamSynchronize(P->stream));
     return MF_OK;
 }
@@ -1738,9 +1764,9 @@ extern "C" int mf_solve(mf_plan *P, int64_t nrhs, const double *B, int64_t ldb, 
     int rc = ensure_solve_buffers(P, nrhs, 1);
     if (rc != MF_OK) return rc;
     const size_t n8 = (size_t)P->n * 8;
-    HIPCHK(hipMemcpy2DAsync(P->d_B.p, n8, B, (size_t)ldb * 8, n8, nrhs, hipMemcpyHostToDevice, P->stream));
+    HIPCHK(hipMemcpy2DAsync(P->sb.B, n8, B, (size_t)ldb * 8, n8, nrhs, hipMemcpyHostToDevice, P->stream));
     if ((rc = solve_core(P, nrhs, 1, refine_steps, bwd_tol, rep)) != MF_OK) return rc;
-    HIPCHK(hipMemcpy2DAsync(X, (size_t)ldx * 8, P->d_X.p, n8, n8, nrhs, hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipMemcpy2DAsync(X, (size_t)ldx * 8, P->sb.X, n8, n8, nrhs, hipMemcpyDeviceToHost, P->stream));
     HIPCHK(hipStreamSynchronize(P->stream));
     return MF_OK;
 }
@@ -1754,9 +1780,9 @@ static int solve_batch(mf_plan *P, int64_t nmat, int64_t nrhs, const double *B, 
     int rc = ensure_solve_buffers(P, nrhs, (int)nmat);
     if (rc != MF_OK) return rc;
     size_t bytes = (size_t)P->n * nrhs * nmat * 8;
-    HIPCHK(hipMemcpyAsync(P->d_B.p, B, bytes, in_kind, P->stream));
+    HIPCHK(hipMemcpyAsync(P->sb.B, B, bytes, in_kind, P->stream));
     if ((rc = solve_core(P, nrhs, (int)nmat, refine_steps, bwd_tol, rep)) != MF_OK) return rc;
-    HIPCHK(hipMemcpyAsync(X, P->d_X.p, bytes, out_kind, P->stream));
+    HIPCHK(hipMemcpyAsync(X, P->sb.X, bytes, out_kind, P->stream));
     HIPCHK(hipStreamSynchronize(P->stream));
     return MF_OK;
 }
