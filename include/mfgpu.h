@@ -139,6 +139,15 @@ int mf_solve(mf_plan *P, int64_t nrhs, const double *B, int64_t ldb, double *X, 
 /* Same with B and X in device memory. */
 int mf_solve_device(mf_plan *P, int64_t nrhs, const double *d_B, int64_t ldb, double *d_X,
                     int64_t ldx, int32_t refine_steps, double bwd_tol, mf_solve_report *rep);
+/* mf_solve_device that allocates nothing: no reference counterpart.  It is for callers that
+ * cannot allocate device memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The solve's vectors live in a file-scope device array of 64 MiB (one copy
+ * per GPU, held until the process ends); a solve needing more (about 7 n nrhs doubles plus
+ * the update vectors) returns MF_BADARG with the size in mf_last_error().  Calls on one GPU
+ * share the array, so calls that overlap on one GPU (concurrent plans) race.  Profiling must be
+ * off.  Same arguments and results as mf_solve_device. */
+int mf_solve_device_noalloc(mf_plan *P, int64_t nrhs, const double *d_B, int64_t ldb, double *d_X,
+                            int64_t ldx, int32_t refine_steps, double bwd_tol, mf_solve_report *rep);
 /* Batched solve after mf_refactor_batch: matrix m solves B[m] (n x nrhs, contiguous). */
 int mf_solve_batch(mf_plan *P, int64_t nmat, int64_t nrhs, const double *B, double *X,
                    int32_t refine_steps, double bwd_tol, mf_solve_report *rep);

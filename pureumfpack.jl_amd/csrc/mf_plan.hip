@@ -15,6 +15,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1768,6 +1769,67 @@ extern "C" int mf_solve(mf_plan *P, int64_t nrhs, const double *B, int64_t ldb, 
     if ((rc = solve_core(P, nrhs, 1, refine_steps, bwd_tol, rep)) != MF_OK) return rc;
     HIPCHK(hipMemcpy2DAsync(X, (size_t)ldx * 8, P->sb.X, n8, n8, nrhs, hipMemcpyDeviceToHost, P->stream));
     HIPCHK(hipStreamSynchronize(P->stream));
+    return MF_OK;
+}
+
+// ------------------------------------------------------------------ allocation-free solve
+// The scratch of mf_solve_device_noalloc: a file-scope device array of MF_NOALLOC_SCRATCH
+// bytes, one copy on each GPU the library runs on, held until the process ends.  In host code
+// `scratch` names a host-side stand-in, not a device address: the address of the current
+// device's copy comes from hipGetSymbolAddress, once per device, and only that pointer is
+// handed to kernels and copies.  Every call on one GPU shares the array and leaves its
+// contents behind; nothing in the solve reads a scratch byte before writing it, so nothing is
+// cleared.  Calls that overlap on one GPU (two plans on two host threads) race on it.
+static constexpr size_t MF_NOALLOC_SCRATCH = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[MF_NOALLOC_SCRATCH];
+static constexpr int MF_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch[MF_MAX_DEVICES] = {};
+
+static int scratch_address(void **out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MF_MAX_DEVICES) return fail(MF_BADARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(MF_HIP, "hipGetSymbolAddress gave no address for the solve scratch");
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return MF_OK;
+}
+
+extern "C" int mf_solve_device_noalloc(mf_plan *P, int64_t nrhs, const double *d_B, int64_t ldb, double *d_X,
+                                       int64_t ldx, int32_t refine_steps, double bwd_tol, mf_solve_report *rep)
+{
+    if (!P || !d_B || !d_X) return fail(MF_BADARG, "NULL argument");
+    if (!P->factored) return fail(MF_BADARG, "plan has no factorization");
+    if (nrhs < 1 || ldb < P->n || ldx < P->n || refine_steps < 0) return fail(MF_DIM, "bad RHS dimensions");
+    if (P->prof) return fail(MF_BADARG, "profiling creates events: turn it off for the allocation-free solve");
+    const size_t need = solve_layout(*P, nrhs, 1, nullptr, nullptr);
+    if (need > MF_NOALLOC_SCRATCH)
+        return fail(MF_BADARG, "this solve needs " + std::to_string(need) + " bytes of scratch; the allocation-free "
+                    "entry point has " + std::to_string(MF_NOALLOC_SCRATCH) + " (use mf_solve_device)");
+    void *base = nullptr;
+    int rc = scratch_address(&base);
+    if (rc != MF_OK) return rc;
+    const SolveBufs own = P->sb;
+    solve_layout(*P, nrhs, 1, static_cast<unsigned char *>(base), &P->sb);
+    const size_t n8 = (size_t)P->n * 8;
+    hipError_t e = hipMemcpy2DAsync(P->sb.B, n8, d_B, (size_t)ldb * 8, n8, nrhs, hipMemcpyDeviceToDevice, P->stream);
+    if (e == hipSuccess) {
+        rc = solve_core(P, nrhs, 1, refine_steps, bwd_tol, rep);
+        if (rc == MF_OK) {
+            e = hipMemcpy2DAsync(d_X, (size_t)ldx * 8, P->sb.X, n8, n8, nrhs, hipMemcpyDeviceToDevice, P->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(P->stream);
+        }
+    }
+    P->sb = own;
+    if (rc != MF_OK) return rc;
+    HIPCHK(e);
     return MF_OK;
 }
 

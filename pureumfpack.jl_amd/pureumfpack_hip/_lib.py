@@ -56,6 +56,9 @@ SIGNATURES = {
     "mf_solve_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
                                        ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double,
                                        ctypes.POINTER(mf_solve_report)]),
+    "mf_solve_device_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
+                                               ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double,
+                                               ctypes.POINTER(mf_solve_report)]),
     "mf_solve_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, c_f64p, c_f64p,
                                       ctypes.c_int32, ctypes.c_double, ctypes.POINTER(mf_solve_report)]),
     "mf_solve_batch_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
