@@ -114,6 +114,9 @@ def run_gpu(hip, A, S, Rs, B, precision, steps, warmup, tol, max_refine, graph=F
     res["front_tflops"] = st["factor_flops"] / res["factor_s"] / 1e12
     res["launches_per_factor"] = st["launches"]
     res["front_bytes"] = st["front_bytes"]
+    res["store_bytes"] = st["store_bytes"]
+    res["arena_bytes"] = st["arena_bytes"]
+    res["resident_front_bytes"] = st["resident_front_bytes"]
     # profiled pass (HIP events around every launch) for the per-class roofline figures
     lib.mf_plan_set_profiling(plan.h, 1)
     step()
@@ -130,6 +133,12 @@ def run_gpu(hip, A, S, Rs, B, precision, steps, warmup, tol, max_refine, graph=F
         hip.free(p)
     del plan
     return res, X
+
+
+def rel_error(X, Xref):
+    """max over columns of ||x - x_ref||_inf / ||x_ref||_inf"""
+    X = np.asarray(X).reshape(Xref.shape)
+    return float((np.abs(X - Xref).max(axis=0) / np.abs(Xref).max(axis=0)).max())
 
 
 def run_batch(hip, k, nmat, nrhs, precision, steps, warmup, tol, max_refine, graph=True):
@@ -173,6 +182,7 @@ def run_batch(hip, k, nmat, nrhs, precision, steps, warmup, tol, max_refine, gra
     hip.sync()
     total = time.perf_counter() - t0
     X = hip.download(dX, rhs.shape, order="C")
+    rep_bwd, rep_it = rep.backward_error, rep.iterations
     # independent check of every matrix's backward error (host, FP64)
     worst = 0.0
     for m in range(nmat):
@@ -181,14 +191,24 @@ def run_batch(hip, k, nmat, nrhs, precision, steps, warmup, tol, max_refine, gra
         den = abs(A).sum(axis=1).max() * np.abs(X[m]).max(axis=1) + np.abs(rhs[m]).max(axis=1)
         worst = max(worst, float((np.abs(R).max(axis=0) / den).max()))
     st = plan.stats()
+    del B, plan
+    # the same batch through the FP64 path once: each result's error against the FP64 solve
+    B64 = I.BatchedLU(P, nmat, precision="fp64", use_graph=False)
+    rc = lib.mf_refactor_batch_device(B64._plan.h, nmat, dAx, dRs, info.ctypes.data_as(_lib.c_i64p))
+    rc = rc or lib.mf_solve_batch_device(B64._plan.h, nmat, nrhs, dB, dX, 1, 0.0, ctypes.byref(rep))
+    if rc:
+        raise RuntimeError(lib.mf_last_error().decode())
+    X64 = hip.download(dX, rhs.shape, order="C")
+    err64 = max(rel_error(X[m].T, X64[m].T) for m in range(nmat))
+    del B64
     for p_ in (dAx, dRs, dB, dX):
         hip.free(p_)
     fac_s = float(np.mean(fac))
     return {"workload": f"{nmat} x poisson3d-{k} pattern (n={n}), unsymmetric values, {nrhs} RHS each, "
                         f"{precision} fronts + FP64 refinement to bwd err <= {tol:g}, hipGraph={int(graph)}",
             "seconds": total / steps, "factor_s": fac_s, "solve_s": float(np.mean(sol)),
-            "matrices_per_s": nmat * steps / total, "refine_iterations": int(rep.iterations),
-            "backward_error": float(rep.backward_error), "backward_error_host_check": worst,
+            "matrices_per_s": nmat * steps / total, "refine_iterations": int(rep_it),
+            "backward_error": float(rep_bwd), "backward_error_host_check": worst, "error_vs_fp64": err64,
             "front_tflops": st["factor_flops"] * nmat / fac_s / 1e12, "plan_s": t_plan,
             "launches_per_factor": st["launches"]}
 
@@ -203,11 +223,14 @@ def run_convdiff(hip, k, steps, warmup):
     t_an = time.perf_counter() - t0
     Rs = I.row_scaling(A, I.SCALE_SUM)
     B = np.random.default_rng(5).standard_normal((A.shape[0], 1))
-    res, _ = run_gpu(hip, A, S, Rs, B, "bf16", steps, warmup, 1e-12, 60, graph=False)
+    res, X = run_gpu(hip, A, S, Rs, B, "bf16", steps, warmup, 1e-12, 60, graph=False)
     res.pop("profile", None)
+    r64, X64 = run_gpu(hip, A, S, Rs, B, "fp64", 1, 0, 0.0, 1, graph=False)
     res.update({"workload": f"convdiff3d-{k} (n={A.shape[0]}), COLAMD, SCALE_SUM, 1 RHS, bf16 Schur + FP32 "
                             f"getrf/trsm, refine to bwd err <= 1e-12", "analysis_s": t_an,
-                "nnz_LU": int(S.predicted_fill()), "nsuper": int(S.nsuper)})
+                "nnz_LU": int(S.predicted_fill()), "nsuper": int(S.nsuper),
+                "error_vs_fp64": rel_error(X, X64), "fp64_seconds": r64["seconds"],
+                "fp64_backward_error": r64["backward_error"]})
     return res
 
 
@@ -266,9 +289,7 @@ def main():
     ap.add_argument("--paths", default="fp64,fp32,bf16")
     ap.add_argument("--graph", type=int, default=1, help="replay the factor schedule as a hipGraph")
     ap.add_argument("--batch", default="512,16,64", help="nmat,k,nrhs of the same-pattern batch run ('' = skip)")
-    ap.add_argument("--convdiff", type=int, default=32,
-                    help="grid of the conv-diff COLAMD + BF16 run (0 = skip; 64 does not fit one GPU's HBM "
-                         "with every front resident)")
+    ap.add_argument("--convdiff", type=int, default=64, help="grid of the conv-diff COLAMD + BF16 run (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration) and exit")
     args = ap.parse_args()
@@ -290,12 +311,15 @@ def main():
     t_an = time.perf_counter() - t0
     Rs = I.row_scaling(A, I.SCALE_SUM)
     hip = Hip()
-    results = {}
+    results, Xs = {}, {}
     for prec in args.paths.split(","):
-        results[prec], X = run_gpu(hip, A, S, Rs, B, prec, args.steps, args.warmup, args.tol, args.max_refine,
-                                   graph=bool(args.graph))
+        results[prec], Xs[prec] = run_gpu(hip, A, S, Rs, B, prec, args.steps, args.warmup, args.tol,
+                                          args.max_refine, graph=bool(args.graph))
         print(f"# {prec}: {json.dumps(results[prec]['seconds'])} s/step, bwd {results[prec]['backward_error']:.2e}, "
               f"{results[prec]['refine_iterations']} refine", file=sys.stderr, flush=True)
+    if "fp64" in Xs:
+        for prec in results:
+            results[prec]["error_vs_fp64"] = rel_error(Xs[prec], Xs["fp64"])
     main_p = "fp32" if "fp32" in results else list(results)[0]
     R = results[main_p]
     # dominant MFMA kernel: the Schur complements of the fronts with nf >= 1024

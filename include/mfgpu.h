@@ -96,6 +96,15 @@ int mf_symbolic_sizes(const mf_symbolic *S, int64_t *nsuper, int64_t *nupd, int6
 int mf_symbolic_export(const mf_symbolic *S, int64_t *qf, int64_t *parent, int64_t *sstart,
                        int64_t *upd_ptr, int64_t *upd_idx, int64_t *child_ptr,
                        int64_t *child_idx);
+/* Relaxed supernode amalgamation of an analysis, in place (opt-in; no reference counterpart:
+ * the reference names it as further work, README.md:186-188, results/PERF_DIAGNOSIS.md:76-78).
+ * A supernode merges into its parent when it is the child whose columns end right before the
+ * parent's and either the merged pivot block has at most max_small_np columns or the explicit
+ * zeros the merge stores are at most max_zero_fraction of the merged supernode's L and U
+ * entries.  Fewer, larger fronts and a shallower tree; L and U then hold explicit zeros, so
+ * the pattern no longer equals the reference's (mf_symbolic_export gives the relaxed arrays and
+ * sizes).  *merged = number of merges. */
+int mf_symbolic_relax(mf_symbolic *S, double max_zero_fraction, int64_t max_small_np, int64_t *merged);
 void mf_symbolic_destroy(mf_symbolic *S);
 
 /* ---- the numeric phase ---- */

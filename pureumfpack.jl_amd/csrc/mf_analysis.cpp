@@ -523,4 +523,74 @@ bool analyze(i64 n, const i64 *Ap, const i64 *Ai, int ordering, const i64 *q_in,
     return true;
 }
 
+// Relaxed supernode amalgamation (opt-in; it changes the L/U pattern, so the parity path never
+// uses it).  Supernodes are visited in postorder; a supernode merges into its parent when its
+// columns end right before the parent's (it is the child the postorder places last, so the
+// merged column range stays contiguous) and either the merged pivot block has at most npmax
+// columns or the explicit zeros the merge stores in L and U (the child's columns take the
+// parent's rows) stay within a fraction zmax of the merged supernode's entries.  The child's
+// own children become the merged supernode's.  The etree property (a child's update rows lie
+// in its parent's front) keeps every original entry and every contribution block inside the
+// merged fronts.  The result is again a SymbolicMF: supernode ranges, their update rows and
+// children (ascending), the column etree of the filled structure and sum |colstruct[j]|.
+int64_t relax_supernodes(Symbolic &S, double zmax, int64_t npmax)
+{
+    const i64 ns = (i64)S.sstart.size() - 1;
+    if (ns <= 1) return 0;
+    std::vector<i64> c1(ns), np(ns), nu(ns), zeros(ns, 0), par(ns, -1);
+    std::vector<char> alive(ns, 1);
+    for (i64 s = 0; s < ns; s++) {
+        c1[s] = S.sstart[s];
+        np[s] = S.sstart[s + 1] - S.sstart[s];
+        nu[s] = (i64)S.upd[s].size();
+        for (i64 c : S.children[s]) par[c] = s;
+    }
+    auto entries = [](i64 p, i64 u) { return p * (p + 1) + 2 * p * u; };   // L and U values of a supernode
+    int64_t merged = 0;
+    for (i64 s = 0; s < ns; s++) {
+        const i64 p = par[s];
+        if (p < 0 || S.sstart[s + 1] != c1[p]) continue;
+        const i64 npm = np[s] + np[p];
+        const i64 tot = entries(npm, nu[p]);
+        const i64 z = tot - entries(np[s], nu[s]) - entries(np[p], nu[p]) + zeros[s] + zeros[p];
+        if (npm > npmax && (double)z > zmax * (double)tot) continue;
+        c1[p] = c1[s];
+        np[p] = npm;
+        zeros[p] = z;
+        alive[s] = 0;
+        merged++;
+    }
+    if (!merged) return 0;
+    // new numbering of the surviving supernodes (still a postorder) and their children
+    std::vector<i64> newid(ns, -1), root(ns);
+    i64 m = 0;
+    for (i64 s = 0; s < ns; s++)
+        if (alive[s]) newid[s] = m++;
+    for (i64 s = ns - 1; s >= 0; s--) root[s] = alive[s] ? s : root[par[s]];   // parents come later
+    std::vector<i64> sstart(m + 1);
+    std::vector<std::vector<i64>> upd(m), children(m);
+    for (i64 s = 0; s < ns; s++) {
+        if (!alive[s]) continue;
+        const i64 t = newid[s];
+        sstart[t] = c1[s];
+        upd[t] = std::move(S.upd[s]);
+    }
+    sstart[m] = S.n;
+    for (i64 s = 0; s < ns; s++)
+        if (alive[s] && par[s] >= 0) children[newid[root[par[s]]]].push_back(newid[s]);
+    for (auto &c : children) std::sort(c.begin(), c.end());
+    S.sstart = std::move(sstart);
+    S.upd = std::move(upd);
+    S.children = std::move(children);
+    // etree and column counts of the filled (relaxed) structure
+    S.ncolstruct = 0;
+    for (i64 t = 0; t < m; t++) {
+        const i64 a = S.sstart[t], b = S.sstart[t + 1], w = b - a, u = (i64)S.upd[t].size();
+        S.ncolstruct += w * (w - 1) / 2 + w * u;
+        for (i64 j = a; j + 1 < b; j++) S.parent[j] = j + 1;
+        S.parent[b - 1] = u > 0 ? S.upd[t][0] : -1;
+    }
+    return merged;
+}
+
 }  // namespace mf

@@ -299,6 +299,117 @@ __global__ __launch_bounds__(NT) void k_panel(FactorArgs a, int k)
 }
 
 // ------------------------------------------------------------------------------------------
+// Large fronts, step 2 for pivot blocks taller than registers hold (np - k > 3072 rows): the
+// same panel factorization with the panel kept in the front (global memory, L2-resident:
+// kb <= 16 columns) instead of registers.  Thread t owns rows t, t + NT, ...; at step j the
+// workgroup argmax (first maximum) picks the pivot row, the pivot row and row j are staged in
+// LDS, and each thread rewrites only its own rows (the pivot row's owner writes old row j's
+// values there, row j's owner the pivot row), so no two threads write one element.  Barriers
+// order the steps; the global stores of one step are visible to the workgroup after them.
+// ------------------------------------------------------------------------------------------
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_panel_big(FactorArgs a, int k)
+{
+    constexpr int KB = PANEL_KB;
+    constexpr int NW = NT / 64;
+    const int mat = blockIdx.y;
+    if (skip_matrix(a.only, mat)) return;
+    const SnDev d = a.sn[a.list[blockIdx.x]];
+    const int np = d.np, nf = d.nf;
+    const int kb = min(KB, np - k);
+    const int m = np - k;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wid = tid >> 6;
+    T *F = reinterpret_cast<T *>(a.ws) + mat * a.ws_stride + d.woff + k + (int64_t)k * nf;   // panel origin
+
+    __shared__ T s_redv[NW];
+    __shared__ int s_redi[NW];
+    __shared__ T s_prow[KB];
+    __shared__ T s_jrow[KB];
+    __shared__ int s_ipiv[KB];
+    __shared__ int s_piv;
+
+    for (int j = 0; j < kb; j++) {
+        T best = T(-1);
+        int bi = 0x7fffffff;
+        for (int i = j + tid; i < m; i += NT) {
+            const T v = fabs(F[i + (int64_t)j * nf]);
+            if (v > best) { best = v; bi = i; }
+        }
+        wave_argmax(best, bi);
+        if (lane == 0) { s_redv[wid] = best; s_redi[wid] = bi; }
+        __syncthreads();
+        if (tid == 0) {
+            T b = s_redv[0];
+            int p = s_redi[0];
+            for (int w = 1; w < NW; w++) {
+                const T ob = s_redv[w];
+                const int oi = s_redi[w];
+                if (ob > b || (ob == b && oi < p)) { b = ob; p = oi; }
+            }
+            s_piv = p;
+        }
+        __syncthreads();
+        const int piv = s_piv;
+        if (tid < kb) {
+            s_prow[tid] = F[piv + (int64_t)tid * nf];
+            s_jrow[tid] = F[j + (int64_t)tid * nf];
+        }
+        __syncthreads();
+        const T pv = s_prow[j];
+        if (tid == 0) {
+            s_ipiv[j] = piv;
+            if (pv == T(0)) atomicMin(a.info + mat, d.c1 + k + j);
+        }
+        const T rcp = T(1) / pv;
+        const bool small_piv = fabs(pv) < Tiny<T>::v();
+        for (int i = j + tid; i < m; i += NT) {
+            if (i == j) {
+                if (piv != j)
+                    for (int jj = 0; jj < kb; jj++) F[i + (int64_t)jj * nf] = s_prow[jj];
+                continue;
+            }
+            T l = i == piv ? s_jrow[j] : F[i + (int64_t)j * nf];
+            if (pv != T(0)) l = small_piv ? l / pv : l * rcp;
+            if (i == piv)
+                for (int jj = 0; jj < j; jj++) F[i + (int64_t)jj * nf] = s_jrow[jj];
+            F[i + (int64_t)j * nf] = l;
+            for (int jj = j + 1; jj < kb; jj++) {
+                const T v = i == piv ? s_jrow[jj] : F[i + (int64_t)jj * nf];
+                F[i + (int64_t)jj * nf] = v - l * s_prow[jj];
+            }
+        }
+        __syncthreads();
+    }
+    // compose the kb sequential swaps, as k_panel does
+    if (tid == 0) {
+        int dst[2 * KB], src[2 * KB], cont[2 * KB];
+        int cnt = kb;
+        for (int q = 0; q < kb; q++) dst[q] = q;
+        for (int j = 0; j < kb; j++) {
+            int p = s_ipiv[j], found = 0;
+            for (int q = 0; q < cnt; q++) found |= (dst[q] == p);
+            if (!found) dst[cnt++] = p;
+        }
+        for (int q = 0; q < cnt; q++) cont[q] = q;
+        for (int j = 0; j < kb; j++) {
+            int p = s_ipiv[j], qp = 0;
+            for (int q = 0; q < cnt; q++)
+                if (dst[q] == p) qp = q;
+            int t = cont[j]; cont[j] = cont[qp]; cont[qp] = t;
+        }
+        for (int q = 0; q < cnt; q++) src[q] = dst[cont[q]];
+        int32_t *pi = a.pinfo + (int64_t)mat * a.pinfo_stride + (int64_t)a.list[blockIdx.x] * PINFO_STRIDE;
+        pi[0] = cnt;
+        for (int q = 0; q < cnt; q++) { pi[1 + q] = dst[q]; pi[1 + 2 * KB + q] = src[q]; }
+        int32_t *prow = a.prow + mat * a.n_stride + d.c1 + k;
+        int orig[2 * KB];
+        for (int q = 0; q < cnt; q++) orig[q] = prow[src[q]];
+        for (int q = 0; q < cnt; q++) prow[dst[q]] = orig[q];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // Large fronts, step 2b: after k_panel, for every front of the launch
 //   * work item kind 0 (256 columns per workgroup, one column per thread): permute the rows
 //     touched by the panel's swaps in every column outside the panel (LAPACK dlaswp on A11 and
@@ -786,22 +897,39 @@ template <> struct Mfma<float> {
 // 16 x 16 accumulators.  Slices of BK columns of A and rows of B are double-buffered in LDS:
 // the global loads of slice s+1 are in flight (registers) while slice s is multiplied, then
 // stored to the other buffer, one barrier per slice.  The MFMA computes B^T A^T, so each lane
-// ends up with C elements of 16 consecutive rows: the read-modify-write of C is coalesced.
-// Every element of C is accumulated over k in the same order for every tile shape.
+// ends up with C elements of consecutive rows: the read-modify-write of C is coalesced.
+// Accumulator block i of a wave covers the rows wm + FM s + i (s = lane % 16, the MFMA's
+// row slot), interleaved rather than 16 apart, so a lane's FM operands of one k step are
+// adjacent in LDS and arrive in one wide read (likewise FN for B).  Every element of C is
+// accumulated over k in the same order (k steps of 4, ascending) for every tile shape, so the
+// choice of tile never changes a result bit.
+template <typename T> struct GemmCfg {
+    static constexpr int BK = sizeof(T) == 8 ? 16 : 32;   // k per LDS slice
+    static constexpr int PAD = sizeof(T) == 8 ? 2 : 4;    // row padding: keeps 16-byte rows, staggers banks
+};
 // LDS elements a gemm tile needs (two buffers of the A and B slices)
 template <typename T, int TM, int TN>
-constexpr int gemm_lds_elems() { return 2 * (sizeof(T) == 8 ? 16 : 32) * ((TM + 1) + (TN + 1)); }
+constexpr int gemm_lds_elems() { return 2 * GemmCfg<T>::BK * ((TM + GemmCfg<T>::PAD) + (TN + GemmCfg<T>::PAD)); }
+
+// FM consecutive elements of LDS into registers (the compiler merges the loads into wide reads)
+template <typename T, int F>
+__device__ __forceinline__ void lds_run(const T *p, T (&v)[F])
+{
+#pragma unroll
+    for (int i = 0; i < F; i++) v[i] = p[i];
+}
 
 template <typename T, int TM, int TN>
 __device__ __forceinline__ void gemm_tile(const GemmProb *__restrict__ probs, const GemmTile tl, void *pool_v,
                                           int64_t pool_stride, T *lds)
 {
     typedef typename Mfma<T>::acc_t acc_t;
-    constexpr int BK = sizeof(T) == 8 ? 16 : 32;
+    constexpr int BK = GemmCfg<T>::BK, PAD = GemmCfg<T>::PAD;
+    constexpr int LA = TM + PAD, LB = TN + PAD;
     constexpr int WM = TM / 2, WN = TN / 2, FM = WM / 16, FN = WN / 16;
     constexpr int NA = TM * BK / 256, NB = TN * BK / 256;
-    T (*As)[BK][TM + 1] = reinterpret_cast<T (*)[BK][TM + 1]>(lds);
-    T (*Bs)[BK][TN + 1] = reinterpret_cast<T (*)[BK][TN + 1]>(lds + 2 * BK * (TM + 1));
+    T *As = lds;                      // [2][BK][LA]
+    T *Bs = lds + 2 * BK * LA;        // [2][BK][LB]
     const GemmProb pr = probs[tl.prob];
     T *pool = reinterpret_cast<T *>(pool_v) + blockIdx.y * pool_stride;
     const T *A = pool + pr.aoff;
@@ -810,6 +938,7 @@ __device__ __forceinline__ void gemm_tile(const GemmProb *__restrict__ probs, co
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int wm = (wid & 1) * WM, wn = (wid >> 1) * WN;
     const int m0 = tl.tm, n0 = tl.tn;
+    const int slot = lane & 15, kl = lane >> 4;
     acc_t acc[FM][FN];
 #pragma unroll
     for (int i = 0; i < FM; i++)
@@ -836,12 +965,12 @@ __device__ __forceinline__ void gemm_tile(const GemmProb *__restrict__ probs, co
 #pragma unroll
         for (int i = 0; i < NA; i++) {
             const int e = tid + 256 * i;
-            As[buf][e / TM][e % TM] = ra[i];
+            As[(buf * BK + e / TM) * LA + e % TM] = ra[i];
         }
 #pragma unroll
         for (int i = 0; i < NB; i++) {
             const int e = tid + 256 * i;
-            Bs[buf][e % BK][e / BK] = rb[i];
+            Bs[(buf * BK + e % BK) * LB + e / BK] = rb[i];
         }
     };
     const int nk = (pr.k + BK - 1) / BK;
@@ -851,13 +980,13 @@ __device__ __forceinline__ void gemm_tile(const GemmProb *__restrict__ probs, co
     for (int s = 0; s < nk; s++) {
         const int cur = s & 1;
         if (s + 1 < nk) gload((s + 1) * BK);
+        const T *Ac = As + (cur * BK + kl) * LA + wm + FM * slot;
+        const T *Bc = Bs + (cur * BK + kl) * LB + wn + FN * slot;
 #pragma unroll
         for (int kq = 0; kq < BK; kq += 4) {
             T av[FM], bv[FN];
-#pragma unroll
-            for (int i = 0; i < FM; i++) av[i] = As[cur][kq + (lane >> 4)][wm + 16 * i + (lane & 15)];
-#pragma unroll
-            for (int j = 0; j < FN; j++) bv[j] = Bs[cur][kq + (lane >> 4)][wn + 16 * j + (lane & 15)];
+            lds_run<T, FM>(Ac + kq * LA, av);
+            lds_run<T, FN>(Bc + kq * LB, bv);
 #pragma unroll
             for (int i = 0; i < FM; i++)
 #pragma unroll
@@ -866,21 +995,22 @@ __device__ __forceinline__ void gemm_tile(const GemmProb *__restrict__ probs, co
         if (s + 1 < nk) sstore(cur ^ 1);
         __syncthreads();
     }
-    // acc[i][j][q] = C[wm + 16 i + lane % 16][wn + 16 j + row(lane, q)]
+    // acc[i][j][q] = C[wm + FM (lane % 16) + i][wn + FN row(lane, q) + j]
     const bool add = pr.flags & GEMM_ADD, set = pr.flags & GEMM_SET;
-    const int gr0 = m0 + wm + (lane & 15);
+    const int gr0 = m0 + wm + FM * slot;
 #pragma unroll
-    for (int i = 0; i < FM; i++)
+    for (int j = 0; j < FN; j++)
 #pragma unroll
-        for (int j = 0; j < FN; j++) {
-            const int gr = gr0 + 16 * i;
+        for (int q = 0; q < 4; q++) {
+            const int gc = n0 + wn + FN * Mfma<T>::row(lane, q) + j;
+            if (gc >= pr.n) continue;
+            T *cp = C + (int64_t)gc * pr.ldc;
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int gc = n0 + wn + 16 * j + Mfma<T>::row(lane, q);
-                if (gr < pr.m && gc < pr.n) {
-                    T *cp = C + gr + (int64_t)gc * pr.ldc;
+            for (int i = 0; i < FM; i++) {
+                const int gr = gr0 + i;
+                if (gr < pr.m) {
                     const T v = add ? acc[i][j][q] : -acc[i][j][q];
-                    *cp = set ? v : *cp + v;
+                    cp[gr] = set ? v : cp[gr] + v;
                 }
             }
         }
@@ -891,7 +1021,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs
                                               const GemmTile *__restrict__ tiles, void *pool_v,
                                               int64_t pool_stride, const int32_t *__restrict__ only)
 {
-    __shared__ T lds[gemm_lds_elems<T, TM, TN>()];
+    __shared__ __align__(16) T lds[gemm_lds_elems<T, TM, TN>()];
     if (skip_matrix(only, blockIdx.y)) return;
     gemm_tile<T, TM, TN>(probs, tiles[blockIdx.x], pool_v, pool_stride, lds);
 }
@@ -1064,7 +1194,7 @@ __global__ __launch_bounds__(256) void k_spec_diag_gemm(FactorArgs a, int k, flo
                                                         const GemmProb *__restrict__ probs,
                                                         const GemmTile *__restrict__ tiles, int ndiag, int dprob)
 {
-    __shared__ T lds[gemm_lds_elems<T, 64, 64>()];
+    __shared__ __align__(16) T lds[gemm_lds_elems<T, 64, 64>()];
     if (skip_matrix(a.only, blockIdx.y)) return;
     if ((int)blockIdx.x >= ndiag) {
         gemm_tile<T, 64, 64>(probs, tiles[blockIdx.x - ndiag], a.ws, a.ws_stride, lds);
@@ -1137,6 +1267,7 @@ hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int
         case 2: hipLaunchKernelGGL((k_panel<T, 256, 4>), grid, dim3(256), 0, st, a, L.k); break;
         case 3: hipLaunchKernelGGL((k_panel<T, 256, 8>), grid, dim3(256), 0, st, a, L.k); break;
         case 4: hipLaunchKernelGGL((k_panel<T, 512, 6>), grid, dim3(512), 0, st, a, L.k); break;
+        case 5: hipLaunchKernelGGL((k_panel_big<T, 1024>), grid, dim3(1024), 0, st, a, L.k); break;
         default: return hipErrorInvalidValue;
         }
         break;
@@ -1184,6 +1315,7 @@ int panel_variant_rows(int v)
     case 2: return 1024;
     case 3: return 2048;
     case 4: return 3072;
+    case 5: return INT32_MAX;   // k_panel_big: any height
     default: return 0;
     }
 }

@@ -20,6 +20,8 @@ struct Symbolic {
 
 bool analyze(i64 n, const i64 *Ap, const i64 *Ai, int ordering, const i64 *q_in, Symbolic &out,
              std::string &err);
+// relaxed supernode amalgamation of an analysis (in place); returns the merges made
+int64_t relax_supernodes(Symbolic &S, double zmax, int64_t npmax);
 
 // Per-supernode record in device memory.  Two storage areas per matrix:
 //   * the factor store keeps what the solves and the export read for the life of the plan:

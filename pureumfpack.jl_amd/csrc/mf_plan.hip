@@ -307,12 +307,27 @@ extern "C" int mf_symbolic_export(const mf_symbolic *S, int64_t *qf, int64_t *pa
     return MF_OK;
 }
 
+extern "C" int mf_symbolic_relax(mf_symbolic *S, double max_zero_fraction, int64_t max_small_np, int64_t *merged)
+{
+    if (!S) return fail(MF_BADARG, "NULL symbolic");
+    if (!(max_zero_fraction >= 0.0 && max_zero_fraction < 1.0) || max_small_np < 0)
+        return fail(MF_BADARG, "relaxation limits out of range");
+    int64_t m = 0;
+    try {
+        m = relax_supernodes(S->S, max_zero_fraction, max_small_np);
+    } catch (const std::bad_alloc &) {
+        return fail(MF_OOM, "host allocation failed in the amalgamation");
+    }
+    if (merged) *merged = m;
+    return MF_OK;
+}
+
 extern "C" void mf_symbolic_destroy(mf_symbolic *S) { delete S; }
 
 // ------------------------------------------------------------------ plan
 static int choose_panel_variant(int rows)
 {
-    for (int v = 0; v < 5; v++)
+    for (int v = 0; v < 6; v++)
         if (panel_variant_rows(v) >= rows) return v;
     return -1;
 }

@@ -45,6 +45,7 @@ SIGNATURES = {
     "mf_analyze": (ctypes.c_int, [ctypes.c_int64, c_i64p, c_i64p, ctypes.c_int32, c_i64p, c_void_pp]),
     "mf_symbolic_sizes": (ctypes.c_int, [ctypes.c_void_p, c_i64p, c_i64p, c_i64p, c_i64p]),
     "mf_symbolic_export": (ctypes.c_int, [ctypes.c_void_p, c_i64p, c_i64p, c_i64p, c_i64p, c_i64p, c_i64p, c_i64p]),
+    "mf_symbolic_relax": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int64, c_i64p]),
     "mf_symbolic_destroy": (None, [ctypes.c_void_p]),
     "mf_plan_create": (ctypes.c_int, [ctypes.c_int64, c_i64p, ctypes.c_int64, c_i64p, c_i64p, c_i64p, c_i64p,
                                       c_i64p, c_i64p, c_i64p, ctypes.POINTER(mf_options), c_void_pp]),

@@ -121,12 +121,17 @@ class SymbolicMF:
         return 2 * (self.n + self.ncolstruct)
 
 
-def symbolic_mf(A, q=None, ordering="amd"):
+RELAX_DEFAULT = (0.05, 0)   # (max explicit-zero fraction, max pivot block merged regardless)
+
+
+def symbolic_mf(A, q=None, ordering="amd", relax=None):
     """Symbolic analysis (src/symbolic.jl:211-226) on the host, in the library's C++ code.
 
     `q` (0-based) fixes the order before postordering like `symbolic_mf(A; q)`; otherwise
     `ordering` is "amd" (amd_order_sym on A+A^T, the reference default) or "colamd" (AMD on
-    A^T A, src/amd.jl:183-188)."""
+    A^T A, src/amd.jl:183-188).  `relax` (opt-in, no reference counterpart) applies relaxed
+    supernode amalgamation: True for RELAX_DEFAULT or a (max_zero_fraction, max_small_np)
+    pair; the factors then store explicit zeros and no longer have the reference's pattern."""
     A = _csc(A)
     n = A.shape[1]
     if A.shape[0] != n:
@@ -146,6 +151,10 @@ def symbolic_mf(A, q=None, ordering="amd"):
     h = ctypes.c_void_p()
     _check(lib.mf_analyze(n, _lib.ptr(Ap), _lib.ptr(Ai), mode, _lib.ptr(qq), ctypes.byref(h)), "mf_analyze")
     try:
+        if relax:
+            zf, snp = RELAX_DEFAULT if relax is True else relax
+            merged = ctypes.c_int64()
+            _check(lib.mf_symbolic_relax(h, float(zf), int(snp), ctypes.byref(merged)), "mf_symbolic_relax")
         ns, nu, nc, ncs = (ctypes.c_int64() for _ in range(4))
         _check(lib.mf_symbolic_sizes(h, ctypes.byref(ns), ctypes.byref(nu), ctypes.byref(nc), ctypes.byref(ncs)))
         ns, nu, nc = ns.value, nu.value, nc.value

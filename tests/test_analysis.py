@@ -7,7 +7,10 @@ from pureumfpack_hip import interface as I
 from pureumfpack_hip import matrices as M
 from oracle import amd as OA
 from oracle import numeric as ON
+from oracle import scaling as OS
 from oracle import symbolic as OY
+
+from _util import oracle_splu
 
 MATS = {
     "poisson2d-8": lambda: M.poisson2d(8),
@@ -114,3 +117,46 @@ def test_colamd_ata_pattern_brute_force(n, seed):
         assert not np.array_equal(W, C)
     G = I.symbolic_mf(A, ordering="colamd")
     assert np.array_equal(G.qf, OY.symbolic_mf(A, q=OA.colamd_order(A)).qf)
+
+
+@pytest.mark.parametrize("name", ["poisson3d-16", "randmat-300", "convdiff-8", "arrowband-200"])
+@pytest.mark.parametrize("relax", [True, (0.1, 8), (0.3, 32)])
+def test_relaxed_amalgamation_structure(name, relax):
+    """Relaxed amalgamation (opt-in) keeps qf, merges supernodes into their parents and only adds
+    explicit zeros: every column's relaxed row set contains its exact structure (the oracle's
+    colstruct, src/symbolic.jl:122-160), supernodes stay contiguous with nested structures, each
+    child's update rows lie in its parent's front, and nnz(L) + nnz(U) = predicted_fill."""
+    A = MATS[name]()
+    S0 = I.symbolic_mf(A)
+    S1 = I.symbolic_mf(A, relax=relax)
+    assert np.array_equal(S0.qf, S1.qf)
+    assert S1.nsuper < S0.nsuper and S1.predicted_fill() >= S0.predicted_fill()
+    O = OY.symbolic_mf(A)
+    n = A.shape[0]
+    ss = np.asarray(S1.sstart)
+    total = 0
+    for t in range(S1.nsuper):
+        c1, c2 = ss[t], ss[t + 1] - 1
+        upd = set(int(v) for v in S1.upd(t))
+        assert all(v > c2 for v in upd)
+        for j in range(c1, c2 + 1):
+            rows = set(range(j + 1, c2 + 1)) | upd
+            assert set(O.colstruct[j]) <= rows
+            total += len(rows)
+        front = set(range(c1, c2 + 1)) | upd
+        for c in S1.children(t):
+            assert c < t and set(int(v) for v in S1.upd(c)) <= front
+    assert S1.predicted_fill() == 2 * (n + total)
+    # the numeric oracle factors the relaxed structure exactly (explicit zeros included)
+    R = oracle_splu(A, S1)
+    As = OS.apply_row_scaling(A, R.Rs)
+    assert R.L.nnz + R.U.nnz == S1.predicted_fill()
+    assert abs(As[R.p, :][:, R.q] - R.L @ R.U).max() <= 1e-13 * abs(As).max()
+
+
+def test_relaxed_amalgamation_argument_checks():
+    A = M.poisson3d(4)
+    with pytest.raises(I.ArgumentError):
+        I.symbolic_mf(A, relax=(1.5, 0))
+    with pytest.raises(I.ArgumentError):
+        I.symbolic_mf(A, relax=(0.1, -1))
