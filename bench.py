@@ -141,7 +141,7 @@ def rel_error(X, Xref):
     return float((np.abs(X - Xref).max(axis=0) / np.abs(Xref).max(axis=0)).max())
 
 
-def run_batch(hip, k, nmat, nrhs, precision, steps, warmup, tol, max_refine, graph=True):
+def run_batch(hip, k, nmat, nrhs, precision, steps, warmup, tol, max_refine, graph=True, relax=False):
     """Same-pattern batch: nmat matrices with the poisson3d(k) pattern and unsymmetric values
     (diag 6 + U[0,1], off-diagonals -U[0.5,1.5]), nrhs randn right-hand sides each; one step =
     mf_refactor_batch_device + mf_solve_batch_device refined to bwd err <= tol."""
@@ -149,7 +149,8 @@ def run_batch(hip, k, nmat, nrhs, precision, steps, warmup, tol, max_refine, gra
     P, Ax = M.poisson3d_batch_values(k, nmat, seed=4)
     n = P.shape[0]
     t0 = time.perf_counter()
-    B = I.BatchedLU(P, nmat, precision=precision, use_graph=graph)
+    S = I.symbolic_mf(P, relax=relax)
+    B = I.BatchedLU(P, nmat, precision=precision, use_graph=graph, symbolic=S)
     Rs = np.stack([I.row_scaling(B.matrix(v), I.SCALE_SUM) for v in Ax])
     t_plan = time.perf_counter() - t0
     rhs = np.random.default_rng(3).standard_normal((nmat, nrhs, n))   # per matrix n x nrhs, column-major
@@ -193,7 +194,7 @@ def run_batch(hip, k, nmat, nrhs, precision, steps, warmup, tol, max_refine, gra
     st = plan.stats()
     del B, plan
     # the same batch through the FP64 path once: each result's error against the FP64 solve
-    B64 = I.BatchedLU(P, nmat, precision="fp64", use_graph=False)
+    B64 = I.BatchedLU(P, nmat, precision="fp64", use_graph=False, symbolic=S)
     rc = lib.mf_refactor_batch_device(B64._plan.h, nmat, dAx, dRs, info.ctypes.data_as(_lib.c_i64p))
     rc = rc or lib.mf_solve_batch_device(B64._plan.h, nmat, nrhs, dB, dX, 1, 0.0, ctypes.byref(rep))
     if rc:
@@ -205,7 +206,9 @@ def run_batch(hip, k, nmat, nrhs, precision, steps, warmup, tol, max_refine, gra
         hip.free(p_)
     fac_s = float(np.mean(fac))
     return {"workload": f"{nmat} x poisson3d-{k} pattern (n={n}), unsymmetric values, {nrhs} RHS each, "
-                        f"{precision} fronts + FP64 refinement to bwd err <= {tol:g}, hipGraph={int(graph)}",
+                        f"{precision} fronts + FP64 refinement to bwd err <= {tol:g}, hipGraph={int(graph)}"
+                        + (", relaxed amalgamation" if relax else ""),
+            "nsuper": int(S.nsuper), "nlevels": int(st["nlevels"]),
             "seconds": total / steps, "factor_s": fac_s, "solve_s": float(np.mean(sol)),
             "matrices_per_s": nmat * steps / total, "refine_iterations": int(rep_it),
             "backward_error": float(rep_bwd), "backward_error_host_check": worst, "error_vs_fp64": err64,
@@ -213,13 +216,13 @@ def run_batch(hip, k, nmat, nrhs, precision, steps, warmup, tol, max_refine, gra
             "launches_per_factor": st["launches"]}
 
 
-def run_convdiff(hip, k, steps, warmup):
+def run_convdiff(hip, k, steps, warmup, relax=False):
     """Unsymmetric 3D convection-diffusion (upwind, velocity (1, 0.5, 0.25), cell Peclet 5),
     COLAMD ordering, BF16 Schur complements with per-tile power-of-two scaling, FP32
     getrf/trsm, FP64 refinement to backward error <= 1e-12; 1 RHS randn(5)."""
     A = M.convdiff3d(k)
     t0 = time.perf_counter()
-    S = I.symbolic_mf(A, ordering="colamd")
+    S = I.symbolic_mf(A, ordering="colamd", relax=relax)
     t_an = time.perf_counter() - t0
     Rs = I.row_scaling(A, I.SCALE_SUM)
     B = np.random.default_rng(5).standard_normal((A.shape[0], 1))
@@ -227,7 +230,8 @@ def run_convdiff(hip, k, steps, warmup):
     res.pop("profile", None)
     r64, X64 = run_gpu(hip, A, S, Rs, B, "fp64", 1, 0, 0.0, 1, graph=False)
     res.update({"workload": f"convdiff3d-{k} (n={A.shape[0]}), COLAMD, SCALE_SUM, 1 RHS, bf16 Schur + FP32 "
-                            f"getrf/trsm, refine to bwd err <= 1e-12", "analysis_s": t_an,
+                            f"getrf/trsm, refine to bwd err <= 1e-12" + (", relaxed amalgamation" if relax else ""),
+                "analysis_s": t_an, "nlevels": int(I.plan_estimate(A, S, precision="bf16")["nlevels"]),
                 "nnz_LU": int(S.predicted_fill()), "nsuper": int(S.nsuper),
                 "error_vs_fp64": rel_error(X, X64), "fp64_seconds": r64["seconds"],
                 "fp64_backward_error": r64["backward_error"]})
@@ -286,7 +290,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-14)
     ap.add_argument("--max-refine", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--paths", default="fp64,fp32,bf16")
+    ap.add_argument("--paths", default="fp64,fp32,bf16,fp64+relax,fp32+relax,bf16+relax",
+                    help="precisions to run; '+relax' uses the relaxed supernode amalgamation")
     ap.add_argument("--graph", type=int, default=1, help="replay the factor schedule as a hipGraph")
     ap.add_argument("--batch", default="512,16,64", help="nmat,k,nrhs of the same-pattern batch run ('' = skip)")
     ap.add_argument("--convdiff", type=int, default=64, help="grid of the conv-diff COLAMD + BF16 run (0 = skip)")
@@ -309,58 +314,79 @@ def main():
     t0 = time.perf_counter()
     S = I.symbolic_mf(A)
     t_an = time.perf_counter() - t0
+    paths = args.paths.split(",")
+    S_relax = I.symbolic_mf(A, relax=True) if any(p.endswith("+relax") for p in paths) else None
     Rs = I.row_scaling(A, I.SCALE_SUM)
     hip = Hip()
     results, Xs = {}, {}
-    for prec in args.paths.split(","):
-        results[prec], Xs[prec] = run_gpu(hip, A, S, Rs, B, prec, args.steps, args.warmup, args.tol,
-                                          args.max_refine, graph=bool(args.graph))
-        print(f"# {prec}: {json.dumps(results[prec]['seconds'])} s/step, bwd {results[prec]['backward_error']:.2e}, "
-              f"{results[prec]['refine_iterations']} refine", file=sys.stderr, flush=True)
+    for path in paths:
+        prec, relaxed = path.split("+")[0], path.endswith("+relax")
+        results[path], Xs[path] = run_gpu(hip, A, S_relax if relaxed else S, Rs, B, prec, args.steps, args.warmup,
+                                          args.tol, args.max_refine, graph=bool(args.graph))
+        results[path]["analysis"] = "relaxed amalgamation" if relaxed else "the reference's supernodes"
+        print(f"# {path}: {json.dumps(results[path]['seconds'])} s/step, bwd {results[path]['backward_error']:.2e}, "
+              f"{results[path]['refine_iterations']} refine", file=sys.stderr, flush=True)
     if "fp64" in Xs:
-        for prec in results:
-            results[prec]["error_vs_fp64"] = rel_error(Xs[prec], Xs["fp64"])
-    main_p = "fp32" if "fp32" in results else list(results)[0]
+        for path in results:
+            results[path]["error_vs_fp64"] = rel_error(Xs[path], Xs["fp64"])
+    # `value`: the fastest mixed-precision path that reaches the backward-error target
+    mixed = [p for p in results if not p.startswith("fp64") and results[p]["backward_error"] <= args.tol]
+    main_p = min(mixed, key=lambda p: results[p]["seconds"]) if mixed else list(results)[0]
     R = results[main_p]
-    # dominant MFMA kernel: the Schur complements of the fronts with nf >= 1024
+    prec_main = main_p.split("+")[0]
+    same_fp64 = "fp64+relax" if main_p.endswith("+relax") else "fp64"
+    # dominant MFMA kernel: the Schur complements of the fronts with nf >= 1024 (the FP32 gemm;
+    # BF16 paths run k_gemm_bf16 there)
     prof = R["profile"]["schur_gemm_nf1024"]
-    roof = {"bound": "mfma", "kernel": "k_gemm<T, 64, 64, 2> (Schur complements, fronts nf >= 1024)",
+    peak = PEAK[prec_main]
+    roof = {"bound": "mfma",
+            "kernel": ("k_gemm_bf16" if prec_main == "bf16" else "k_gemm<T, 64, 64, 2>") +
+                      " (Schur complements, fronts nf >= 1024)",
             "achieved": prof["tflops"],
-            "peak": PEAK[main_p], "unit": "TFLOP/s", "frac": prof["tflops"] / PEAK[main_p],
-            "traffic": measured_traffic(args.k, main_p),
+            "peak": peak, "unit": "TFLOP/s", "frac": prof["tflops"] / peak,
+            "traffic": measured_traffic(args.k, main_p.replace("+", "_")),
             "algorithmic_bytes_per_launch": prof["gbytes"] * 1e9 / max(1, prof["launches"]),
-            "launch_us": prof["ms"] * 1e3 / max(1, prof["launches"])}
+            "launch_us": prof["ms"] * 1e3 / max(1, prof["launches"]), "path": main_p}
+    analysis = S_relax if main_p.endswith("+relax") else S
     out = {
         "metric": METRIC, "value": R["seconds"], "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": R["seconds"] * 1e3, "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": main_p, "data": "synthetic",
+        "vs_baseline": None, "dtype": prec_main, "data": "synthetic",
         "config": {"workload": f"poisson3d-{args.k} + Diagonal(U[0,0.1]), {args.nrhs} RHS randn(3), AMD, SCALE_SUM, "
                                f"refine to bwd err <= {args.tol:g}", "n": n, "nnz": int(A.nnz), "nrhs": args.nrhs,
-                   "nsuper": int(S.nsuper), "nnz_LU": int(S.predicted_fill()), "path": main_p},
+                   "nsuper": int(analysis.nsuper), "nnz_LU": int(analysis.predicted_fill()), "path": main_p,
+                   "analysis": R["analysis"]},
         "front_tflops": R["front_tflops"], "analysis_s": t_an,
         "paths": results,
         "speedup_vs_fp64": (results["fp64"]["seconds"] / R["seconds"]) if "fp64" in results else None,
+        "speedup_vs_fp64_same_analysis": (results[same_fp64]["seconds"] / R["seconds"]) if same_fp64 in results
+        else None,
         "roofline": roof,
     }
     if args.batch:
         nmat, kb, nrb = (int(v) for v in args.batch.split(","))
-        out["batched"] = run_batch(hip, kb, nmat, nrb, "fp32", args.steps, args.warmup, args.tol, args.max_refine,
-                                   graph=bool(args.graph))
-        print(f"# batched: {out['batched']['seconds']:.4f} s/step, {out['batched']['matrices_per_s']:.0f} matrices/s, "
-              f"bwd {out['batched']['backward_error_host_check']:.2e}", file=sys.stderr, flush=True)
+        for key, relaxed in (("batched", False), ("batched_relaxed", True)):
+            out[key] = run_batch(hip, kb, nmat, nrb, "fp32", args.steps, args.warmup, args.tol, args.max_refine,
+                                 graph=bool(args.graph), relax=relaxed)
+            print(f"# {key}: {out[key]['seconds']:.4f} s/step, {out[key]['matrices_per_s']:.0f} matrices/s, "
+                  f"bwd {out[key]['backward_error_host_check']:.2e}", file=sys.stderr, flush=True)
     if args.convdiff:
-        out["convdiff_colamd_bf16"] = run_convdiff(hip, args.convdiff, args.steps, args.warmup)
-        cd = out["convdiff_colamd_bf16"]
-        print(f"# convdiff {args.convdiff}^3 colamd bf16: {cd['seconds']:.4f} s/step, bwd {cd['backward_error']:.2e}, "
-              f"{cd['refine_iterations']} refine", file=sys.stderr, flush=True)
+        for key, relaxed, steps, warmup in (("convdiff_colamd_bf16_relaxed", True, args.steps, args.warmup),
+                                            ("convdiff_colamd_bf16", False, 2, 1)):
+            out[key] = run_convdiff(hip, args.convdiff, steps, warmup, relax=relaxed)
+            cd = out[key]
+            print(f"# {key} {args.convdiff}^3: {cd['seconds']:.4f} s/step, bwd {cd['backward_error']:.2e}, "
+                  f"{cd['refine_iterations']} refine", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A, S, Rs, B, refine=1)
     # accuracy of every leg against its target: a timing of wrong answers is no timing
     failed = [f"{p}: bwd {r['backward_error']:.2e}" for p, r in results.items() if r["backward_error"] > args.tol]
-    if "batched" in out and out["batched"]["backward_error_host_check"] > args.tol:
-        failed.append(f"batched: bwd {out['batched']['backward_error_host_check']:.2e}")
-    if "convdiff_colamd_bf16" in out and out["convdiff_colamd_bf16"]["backward_error"] > 1e-12:
-        failed.append(f"convdiff: bwd {out['convdiff_colamd_bf16']['backward_error']:.2e}")
+    for key in ("batched", "batched_relaxed"):
+        if key in out and out[key]["backward_error_host_check"] > args.tol:
+            failed.append(f"{key}: bwd {out[key]['backward_error_host_check']:.2e}")
+    for key in ("convdiff_colamd_bf16", "convdiff_colamd_bf16_relaxed"):
+        if key in out and out[key]["backward_error"] > 1e-12:
+            failed.append(f"{key}: bwd {out[key]['backward_error']:.2e}")
     out["accuracy_ok"] = not failed
     if failed:
         print("# ACCURACY CHECK FAILED: " + "; ".join(failed), file=sys.stderr, flush=True)

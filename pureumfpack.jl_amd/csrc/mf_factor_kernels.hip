@@ -1027,19 +1027,25 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs
 }
 
 // ------------------------------------------------------------------------------------------
-// Schur complement in BF16 (precision MF_BF16): C -= A B with A, B held in FP32 fronts.  Each
-// k-slice of BK = 32 is scaled per tile by powers of two (A slice by 2^-ea, B slice by 2^-eb,
-// ea and eb the exponents of the slices' largest magnitudes), rounded to BF16 (nearest even),
-// multiplied on MFMA 16x16x16 BF16 with FP32 accumulation into a per-slice accumulator, and
-// added to the FP32 result times 2^(ea+eb) (exact).  64 x 64 tiles, 4 waves of 32 x 32.
+// Schur complements in BF16 (precision MF_BF16, fronts with nf >= 1024): C -= A B with A, B
+// and C in FP32 fronts.
+//   k_pack_bf16: one workgroup per operand tile of PK_ROWS rows of A (or columns of B) by
+//     PK_K k values: the tile is staged in LDS, its largest magnitude gives a power-of-two
+//     exponent e (max |v| < 2^e), and v * 2^-e is rounded to BF16 (nearest even) into a
+//     contiguous tile laid out [row][k] (k fastest), the order the MFMA operands are read in.
+//   k_gemm_bf16: PK_ROWS x PK_ROWS of C per 256-thread workgroup (2 x 2 waves of 64 x 64, MFMA
+//     16x16x32 BF16 with FP32 accumulation), one packed k-slice of A and B per step, double-
+//     buffered in LDS with 16-byte loads and stores; the slice's FP32 partial product is added
+//     to the FP32 result times 2^(ea + eb), which is exact.
 // ------------------------------------------------------------------------------------------
-typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef short bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16v8_t __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ short to_bf16(float x)
+__device__ __forceinline__ unsigned short to_bf16(float x)
 {
     unsigned u = __float_as_uint(x);
     u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even (finite inputs)
-    return (short)(u >> 16);
+    return (unsigned short)(u >> 16);
 }
 
 __device__ __forceinline__ float wave_max(float v)
@@ -1052,7 +1058,7 @@ __device__ __forceinline__ float wave_max(float v)
 // The exponent e of mx (mx in [2^(e-1), 2^e)), 0 for mx == 0.  e is kept >= -100 so that
 // the scale 2^-e stays finite: fill entries of strongly convective operators decay
 // exponentially and reach FP32 subnormals, where 2^-e would overflow (and 0 * inf = NaN).  A
-// slice below 2^-101 then keeps its tiny magnitude, and 2^(ea + eb) flushes its product.
+// tile below 2^-101 then keeps its tiny magnitude, and 2^(ea + eb) flushes its product.
 __device__ __forceinline__ int pow2_exp(float mx)
 {
     if (!(mx > 0.0f) || !isfinite(mx)) return 0;
@@ -1061,102 +1067,156 @@ __device__ __forceinline__ int pow2_exp(float mx)
     return max(e, -100);
 }
 
+// items int4 {problem, operand (0 A, 1 B), row tile, k tile}
+__global__ __launch_bounds__(256) void k_pack_bf16(FactorArgs a, const GemmProb *__restrict__ probs)
+{
+    constexpr int LD = PK_K + 1;
+    __shared__ float t[PK_ROWS * LD];
+    __shared__ float smax[4];
+    const int mat = blockIdx.y;
+    if (skip_matrix(a.only, mat)) return;
+    const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
+    const GemmProb pr = probs[w.x];
+    const bool isA = w.y == 0;
+    const float *ws = reinterpret_cast<const float *>(a.ws) + mat * a.ws_stride;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int r0 = w.z * PK_ROWS, k0 = w.w * PK_K;
+    float mx = 0.0f;
+    if (isA) {   // A(m x k), column-major: threads walk rows (coalesced)
+        const float *A = ws + pr.aoff;
+        const int r = tid % PK_ROWS;
+#pragma unroll 4
+        for (int kk = tid / PK_ROWS; kk < PK_K; kk += 256 / PK_ROWS) {
+            const int gr = r0 + r, gk = k0 + kk;
+            const float v = (gr < pr.m && gk < pr.k) ? A[gr + (int64_t)gk * pr.lda] : 0.0f;
+            t[r * LD + kk] = v;
+            mx = fmaxf(mx, fabsf(v));
+        }
+    } else {     // B(k x n), column-major: threads walk k (coalesced); tile row = column of B
+        const float *B = ws + pr.boff;
+        const int kk = tid % PK_K;
+#pragma unroll 4
+        for (int c = tid / PK_K; c < PK_ROWS; c += 256 / PK_K) {
+            const int gc = r0 + c, gk = k0 + kk;
+            const float v = (gc < pr.n && gk < pr.k) ? B[gk + (int64_t)gc * pr.ldb] : 0.0f;
+            t[c * LD + kk] = v;
+            mx = fmaxf(mx, fabsf(v));
+        }
+    }
+    mx = wave_max(mx);
+    if (lane == 0) smax[tid >> 6] = mx;
+    __syncthreads();
+    const int e = pow2_exp(fmaxf(fmaxf(smax[0], smax[1]), fmaxf(smax[2], smax[3])));
+    const float sc = ldexpf(1.0f, -e);
+    const int64_t tile = (isA ? pr.apk : pr.bpk) + (int64_t)w.z * ((pr.k + PK_K - 1) / PK_K) + w.w;
+    bf16x8_t *dst = reinterpret_cast<bf16x8_t *>(reinterpret_cast<unsigned short *>(a.pack) +
+                                                 (mat * a.pack_tiles + tile) * (int64_t)PK_TILE);
+#pragma unroll
+    for (int h = 0; h < PK_TILE / 8 / 256; h++) {
+        const int ch = tid + 256 * h, row = ch / (PK_K / 8), kc = (ch % (PK_K / 8)) * 8;
+        bf16x8_t v;
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = (short)to_bf16(t[row * LD + kc + q] * sc);
+        dst[ch] = v;
+    }
+    if (tid == 0) a.pexp[mat * a.pack_tiles + tile] = e;
+}
+
 __global__ __launch_bounds__(256) void k_gemm_bf16(const GemmProb *__restrict__ probs,
                                                    const GemmTile *__restrict__ tiles, void *pool_v,
-                                                   int64_t pool_stride, const int32_t *__restrict__ only)
+                                                   int64_t pool_stride, const int32_t *__restrict__ only,
+                                                   const unsigned short *__restrict__ pack,
+                                                   const int32_t *__restrict__ pexp, int64_t pack_tiles)
 {
-    if (skip_matrix(only, blockIdx.y)) return;
-    constexpr int TM = 64, TN = 64, BK = 32, WM = 32, WN = 32, FM = 2, FN = 2;
-    __shared__ float As[BK][TM + 1];
-    __shared__ float Bs[BK][TN + 1];
-    __shared__ float smax[2][4];
+    constexpr int TM = PK_ROWS, BK = PK_K, WM = TM / 2, FM = WM / 16;   // 128, 64, 64, 4
+    constexpr int LDK = BK + 8;                    // bf16 per LDS row (16-byte aligned, staggered banks)
+    constexpr int CH = TM * BK / 8 / 256;          // 16-byte chunks per thread per operand tile
+    __shared__ __align__(16) unsigned short As[2][TM * LDK];
+    __shared__ __align__(16) unsigned short Bs[2][TM * LDK];
     typedef float acc4 __attribute__((ext_vector_type(4)));
+    if (skip_matrix(only, blockIdx.y)) return;
     const GemmTile tl = tiles[blockIdx.x];
     const GemmProb pr = probs[tl.prob];
-    float *pool = reinterpret_cast<float *>(pool_v) + blockIdx.y * pool_stride;
-    const float *A = pool + pr.aoff;
-    const float *B = pool + pr.boff;
-    float *C = pool + pr.coff;
+    const int mat = blockIdx.y;
+    float *C = reinterpret_cast<float *>(pool_v) + mat * pool_stride + pr.coff;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int wm = (wid & 1) * WM, wn = (wid >> 1) * WN;
-    const int m0 = tl.tm, n0 = tl.tn;
-    acc4 acc[FM][FN];
+    const int wm = (wid & 1) * WM, wn = (wid >> 1) * WM;
+    const int kt_n = (pr.k + BK - 1) / BK;
+    const int64_t abase = mat * pack_tiles + pr.apk + (int64_t)(tl.tm / TM) * kt_n;
+    const int64_t bbase = mat * pack_tiles + pr.bpk + (int64_t)(tl.tn / TM) * kt_n;
+    const bf16x8_t *Ag = reinterpret_cast<const bf16x8_t *>(pack);
+    acc4 acc[FM][FM];
 #pragma unroll
     for (int i = 0; i < FM; i++)
 #pragma unroll
-        for (int j = 0; j < FN; j++) acc[i][j] = acc4{0, 0, 0, 0};
-    for (int k0 = 0; k0 < pr.k; k0 += BK) {
-        float ma = 0.0f, mb = 0.0f;
+        for (int j = 0; j < FM; j++) acc[i][j] = acc4{0, 0, 0, 0};
+    bf16x8_t ra[CH], rb[CH];
+    auto gload = [&](int kt) {
+        const bf16x8_t *pa = Ag + (abase + kt) * (PK_TILE / 8), *pb = Ag + (bbase + kt) * (PK_TILE / 8);
 #pragma unroll
-        for (int i = 0; i < TM * BK / 256; i++) {
-            const int e = tid + 256 * i, r = e % TM, kk = e / TM;
-            const int gr = m0 + r, gk = k0 + kk;
-            const float v = (gr < pr.m && gk < pr.k) ? A[gr + (int64_t)gk * pr.lda] : 0.0f;
-            As[kk][r] = v;
-            ma = fmaxf(ma, fabsf(v));
+        for (int h = 0; h < CH; h++) {
+            ra[h] = pa[tid + 256 * h];
+            rb[h] = pb[tid + 256 * h];
         }
+    };
+    auto sstore = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < TN * BK / 256; i++) {
-            const int e = tid + 256 * i, kk = e % BK, cc = e / BK;
-            const int gc = n0 + cc, gk = k0 + kk;
-            const float v = (gc < pr.n && gk < pr.k) ? B[gk + (int64_t)gc * pr.ldb] : 0.0f;
-            Bs[kk][cc] = v;
-            mb = fmaxf(mb, fabsf(v));
+        for (int h = 0; h < CH; h++) {
+            const int ch = tid + 256 * h, row = ch / (BK / 8), kc = (ch % (BK / 8)) * 8;
+            *reinterpret_cast<bf16x8_t *>(&As[buf][row * LDK + kc]) = ra[h];
+            *reinterpret_cast<bf16x8_t *>(&Bs[buf][row * LDK + kc]) = rb[h];
         }
-        ma = wave_max(ma);
-        mb = wave_max(mb);
-        if (lane == 0) {
-            smax[0][wid] = ma;
-            smax[1][wid] = mb;
-        }
-        __syncthreads();
-        const int ea = pow2_exp(fmaxf(fmaxf(smax[0][0], smax[0][1]), fmaxf(smax[0][2], smax[0][3])));
-        const int eb = pow2_exp(fmaxf(fmaxf(smax[1][0], smax[1][1]), fmaxf(smax[1][2], smax[1][3])));
-        const float sa = ldexpf(1.0f, -ea), sb = ldexpf(1.0f, -eb);
-        acc4 sl[FM][FN];
+    };
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    const int srow = lane & 15, skq = (lane >> 4) * 8;
+    for (int kt = 0; kt < kt_n; kt++) {
+        const int cur = kt & 1;
+        const float un = ldexpf(1.0f, pexp[abase + kt] + pexp[bbase + kt]);   // exact power of two
+        if (kt + 1 < kt_n) gload(kt + 1);
+        acc4 sl[FM][FM];
 #pragma unroll
         for (int i = 0; i < FM; i++)
 #pragma unroll
-            for (int j = 0; j < FN; j++) sl[i][j] = acc4{0, 0, 0, 0};
+            for (int j = 0; j < FM; j++) sl[i][j] = acc4{0, 0, 0, 0};
 #pragma unroll
-        for (int kq = 0; kq < BK; kq += 16) {
-            // lane l holds A[row l%16][k 4(l/16) .. +3] and B[k 4(l/16) .. +3][col l%16]
-            const int kb = kq + 4 * (lane >> 4);
-            bf16x4_t av[FM], bv[FN];
+        for (int ks = 0; ks < BK; ks += 32) {
+            bf16x8_t av[FM], bv[FM];
 #pragma unroll
-            for (int i = 0; i < FM; i++) {
-                const int r = wm + 16 * i + (lane & 15);
-                av[i] = bf16x4_t{to_bf16(As[kb][r] * sa), to_bf16(As[kb + 1][r] * sa), to_bf16(As[kb + 2][r] * sa),
-                                 to_bf16(As[kb + 3][r] * sa)};
-            }
+            for (int i = 0; i < FM; i++)
+                av[i] = *reinterpret_cast<const bf16x8_t *>(&As[cur][(wm + 16 * i + srow) * LDK + ks + skq]);
 #pragma unroll
-            for (int j = 0; j < FN; j++) {
-                const int c = wn + 16 * j + (lane & 15);
-                bv[j] = bf16x4_t{to_bf16(Bs[kb][c] * sb), to_bf16(Bs[kb + 1][c] * sb), to_bf16(Bs[kb + 2][c] * sb),
-                                 to_bf16(Bs[kb + 3][c] * sb)};
-            }
+            for (int j = 0; j < FM; j++)
+                bv[j] = *reinterpret_cast<const bf16x8_t *>(&Bs[cur][(wn + 16 * j + srow) * LDK + ks + skq]);
 #pragma unroll
             for (int i = 0; i < FM; i++)
 #pragma unroll
-                for (int j = 0; j < FN; j++)
-                    sl[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[i], bv[j], sl[i][j], 0, 0, 0);
+                for (int j = 0; j < FM; j++)
+                    sl[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16v8_t, bv[j]),
+                                                                       __builtin_bit_cast(bf16v8_t, av[i]),
+                                                                       sl[i][j], 0, 0, 0);
         }
-        const float un = ldexpf(1.0f, ea + eb);   // a power of two: exact (or flushed to zero)
 #pragma unroll
         for (int i = 0; i < FM; i++)
 #pragma unroll
-            for (int j = 0; j < FN; j++) acc[i][j] += sl[i][j] * un;
+            for (int j = 0; j < FM; j++) acc[i][j] += sl[i][j] * un;
+        if (kt + 1 < kt_n) sstore(cur ^ 1);
         __syncthreads();
     }
+    // acc[i][j][q] = C[tm + wm + 16 i + lane % 16][tn + wn + 16 j + 4 (lane / 16) + q]
+    const int gr0 = tl.tm + wm + srow;
 #pragma unroll
-    for (int i = 0; i < FM; i++)
+    for (int j = 0; j < FM; j++)
 #pragma unroll
-        for (int j = 0; j < FN; j++) {
-            const int gc = n0 + wn + 16 * j + (lane & 15);
+        for (int q = 0; q < 4; q++) {
+            const int gc = tl.tn + wn + 16 * j + 4 * (lane >> 4) + q;
+            if (gc >= pr.n) continue;
+            float *cp = C + (int64_t)gc * pr.ldc;
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int gr = m0 + wm + 16 * i + Mfma<float>::row(lane, q);
-                if (gr < pr.m && gc < pr.n) C[gr + (int64_t)gc * pr.ldc] -= acc[i][j][q];
+            for (int i = 0; i < FM; i++) {
+                const int gr = gr0 + 16 * i;
+                if (gr < pr.m) cp[gr] -= acc[i][j][q];
             }
         }
 }
@@ -1210,7 +1270,8 @@ __global__ __launch_bounds__(256) void k_spec_diag_gemm(FactorArgs a, int k, flo
 
 template <typename T>
 hipError_t launch_gemm_tile(int tile, int use, const GemmProb *probs, const GemmTile *tiles, int count, void *pool,
-                            int64_t pool_stride, int nmat, const int32_t *only, hipStream_t st)
+                            int64_t pool_stride, int nmat, const int32_t *only, hipStream_t st, const void *pack,
+                            const int32_t *pexp, int64_t pack_tiles)
 {
     const dim3 grid(count, nmat), blk(256);
 #define MF_GEMM_CASE(TI, TM, TN)                                                                                    \
@@ -1224,7 +1285,9 @@ hipError_t launch_gemm_tile(int tile, int use, const GemmProb *probs, const Gemm
         break;
     if (tile == GEMM_TILE_BF16) {
         if (sizeof(T) != 4) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_gemm_bf16, grid, blk, 0, st, probs, tiles, pool, pool_stride, only);
+        if (!pack || !pexp) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_gemm_bf16, grid, blk, 0, st, probs, tiles, pool, pool_stride, only,
+                           static_cast<const unsigned short *>(pack), pexp, pack_tiles);
         return hipGetLastError();
     }
     switch (tile) {
@@ -1238,9 +1301,9 @@ hipError_t launch_gemm_tile(int tile, int use, const GemmProb *probs, const Gemm
     return hipGetLastError();
 }
 template hipError_t launch_gemm_tile<double>(int, int, const GemmProb *, const GemmTile *, int, void *, int64_t,
-                                             int, const int32_t *, hipStream_t);
+                                             int, const int32_t *, hipStream_t, const void *, const int32_t *, int64_t);
 template hipError_t launch_gemm_tile<float>(int, int, const GemmProb *, const GemmTile *, int, void *, int64_t,
-                                            int, const int32_t *, hipStream_t);
+                                            int, const int32_t *, hipStream_t, const void *, const int32_t *, int64_t);
 
 // ------------------------------------------------------------------------------------------
 // Host launchers
@@ -1291,7 +1354,12 @@ hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int
     case L_GEMM:
         return launch_gemm_tile<T>(L.tile, L.variant == 3 ? 2 : L.variant == 1 ? 1 : 0, probs + L.prob_off,
                                    tiles + L.list_off, L.count, L.on_store ? base.store : base.ws,
-                                   L.on_store ? base.store_stride : base.ws_stride, nmat, base.only, st);
+                                   L.on_store ? base.store_stride : base.ws_stride, nmat, base.only, st, base.pack,
+                                   base.pexp, base.pack_tiles);
+    case L_PACK_BF16:
+        if (sizeof(T) != 4) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_pack_bf16, grid, dim3(256), 0, st, a, probs + L.prob_off);
+        break;
     case L_STORE:
         hipLaunchKernelGGL(k_store<T>, grid, dim3(256), 0, st, a);
         break;

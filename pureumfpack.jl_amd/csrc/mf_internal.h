@@ -57,6 +57,7 @@ struct GemmProb {
     int32_t m, n, k, lda, ldb, ldc;
     int32_t flags;
     int32_t pad;
+    i64 apk, bpk;   // BF16 Schur complements: first packed tile of A and of B (tile index in the pack buffer)
 };
 
 // Grouped gemm tile: problem index and tile origin.
@@ -75,6 +76,7 @@ enum LaunchKind : int32_t {
     L_INV_LEAF = 7,     // inverses of the 64x64 diagonal blocks of L11 and U11 (large fronts)
     L_SPEC_DIAG_GEMM = 8,  // next diagonal block factorization alongside the bulk of the trailing update
     L_STORE = 9,        // copy the factored panel and U12 of a level's large fronts to the store
+    L_PACK_BF16 = 10,   // BF16 Schur complements: scale, round and pack the operand tiles
 };
 
 struct Launch {

@@ -30,7 +30,10 @@ constexpr int GEMM_BM = 64, GEMM_BN = 64, GEMM_BK = 16;
 // shapes accumulate every C element over k in the same order (BK = 16 slices, MFMA k = 4),
 // so the choice of tile never changes a result bit.
 constexpr int GEMM_NTILES = 4;
-constexpr int GEMM_TILE_BF16 = 64;   // the BF16 Schur-complement kernel (64 x 64, FP32 fronts)
+constexpr int GEMM_TILE_BF16 = 64;   // the BF16 Schur-complement kernel (packed operands)
+// BF16 Schur complements: operands packed in tiles of PK_ROWS rows (of A, or columns of B) x
+// PK_K (k), each with its own power-of-two exponent; the gemm covers PK_ROWS x PK_ROWS of C
+constexpr int PK_ROWS = 128, PK_K = 64, PK_TILE = PK_ROWS * PK_K;
 constexpr int GEMM_TILE_M[GEMM_NTILES] = {64, 128, 64, 128};
 constexpr int GEMM_TILE_N[GEMM_NTILES] = {64, 64, 128, 128};
 // tile policy: size buckets of sqrt(m n) (upper bounds, the last open-ended) for the two gemm
@@ -119,7 +122,9 @@ struct FactorArgs {
     int32_t *flag;              // per-matrix: set when the speculative schedule needs pivoting
     const int32_t *only;        // per-matrix: when set, matrices with only[mat] == 0 are skipped
     void *spec;                 // SPEC_SLOT elements per front of a speculative diagonal launch
-    int64_t ws_stride, store_stride, ax_stride, rs_stride, n_stride, pinfo_stride, spec_stride;
+    void *pack;                 // BF16 packed operand tiles (PK_TILE bf16 each)
+    int32_t *pexp;              // their exponents
+    int64_t ws_stride, store_stride, ax_stride, rs_stride, n_stride, pinfo_stride, spec_stride, pack_tiles;
 };
 
 // true when matrix `mat` of the launch is to be skipped (a redo of only the matrices whose
@@ -147,7 +152,8 @@ struct SolveArgs {
 // ---- launchers (mf_factor_kernels.hip, mf_solve_kernels.hip) ----
 template <typename T>
 hipError_t launch_gemm_tile(int tile, int use, const GemmProb *probs, const GemmTile *tiles, int count, void *pool,
-                            int64_t pool_stride, int nmat, const int32_t *only, hipStream_t st);
+                            int64_t pool_stride, int nmat, const int32_t *only, hipStream_t st,
+                            const void *pack = nullptr, const int32_t *pexp = nullptr, int64_t pack_tiles = 0);
 hipError_t launch_copy(void *dst, const void *src, int64_t bytes, hipStream_t st);
 template <typename T>
 hipError_t launch_factor_step(const Launch &L, const FactorArgs &base, const int32_t *lists,

@@ -145,6 +145,8 @@ struct mf_plan {
     DevBuf d_csr_rp, d_csr_ci, d_csr_vmap;
     DevBuf d_store, d_ws, d_prow, d_info, d_ax, d_rs, d_pinfo, d_flag, d_spec, d_redo;
     int64_t spec_slots = 0;         // most fronts in one speculative diagonal launch
+    int64_t pack_tiles = 0;         // BF16 operand tiles one launch packs at most
+    DevBuf d_pack, d_pexp;
     DevBuf d_solve, d_anorm, d_npart;   // solve buffers (carved as SolveBufs), ||A||, its partials
     SolveBufs sb;                       // the buffers the current solve call uses
     DevBuf d_Lp, d_Up, d_uoff, d_Li, d_Lx, d_Ui, d_Ux, d_rowfac;
@@ -230,6 +232,7 @@ static int launch_class(const Launch &L)
     case L_SPEC_TRSM: return 2;
     case L_SPEC_DIAG_GEMM: return 2;
     case L_STORE: return 1;
+    case L_PACK_BF16: return 1;
     default: return L.variant == 3 ? 7 : L.variant == 1 ? 4 : 3;
     }
 }
@@ -676,17 +679,62 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             pv.swap(rest);
         }
         const int use = schur == 1 || schur == 3 ? 1 : 0;
-        const bool bf16 = use == 1 && P.prec == MF_BF16;
+        if (schur == 3 && P.prec == MF_BF16) {
+            // BF16 Schur complements of the fronts with nf >= 1024: one launch packs the operand
+            // tiles (scaled, rounded), the next multiplies them; pack tiles are reused launch by launch
+            Launch Pk{}, L{};
+            Pk.kind = L_PACK_BF16;
+            L.kind = L_GEMM;
+            L.variant = schur;
+            L.tile = GEMM_TILE_BF16;
+            Pk.prob_off = L.prob_off = (int32_t)probs.size();
+            L.list_off = (int32_t)tiles.size();
+            std::vector<int32_t> items;
+            int64_t tcount = 0;
+            for (size_t q = 0; q < pv.size(); q++) {
+                GemmProb g = pv[q];
+                const int64_t mt = (g.m + PK_ROWS - 1) / PK_ROWS, nt = (g.n + PK_ROWS - 1) / PK_ROWS,
+                              kt = (g.k + PK_K - 1) / PK_K;
+                g.apk = tcount;
+                g.bpk = tcount + mt * kt;
+                tcount += (mt + nt) * kt;
+                for (int64_t r = 0; r < mt; r++)
+                    for (int64_t c = 0; c < kt; c++) {
+                        items.push_back((int32_t)q); items.push_back(0); items.push_back((int32_t)r); items.push_back((int32_t)c);
+                    }
+                for (int64_t r = 0; r < nt; r++)
+                    for (int64_t c = 0; c < kt; c++) {
+                        items.push_back((int32_t)q); items.push_back(1); items.push_back((int32_t)r); items.push_back((int32_t)c);
+                    }
+                for (int tn = 0; tn < g.n; tn += PK_ROWS)
+                    for (int tm = 0; tm < g.m; tm += PK_ROWS) tiles.push_back({(int32_t)q, tm, tn, 0});
+                probs.push_back(g);
+                L.flops += 2.0 * g.m * (double)g.n * g.k;
+                L.bytes += ((double)(mt + nt) * kt * PK_TILE * 2.0) + 2.0 * g.m * (double)g.n * 4.0;
+                Pk.bytes += ((double)g.m * g.k + (double)g.k * g.n) * 4.0 + (double)(mt + nt) * kt * PK_TILE * 2.0;
+            }
+            P.pack_tiles = std::max(P.pack_tiles, tcount);
+            while (lists.size() & 3) lists.push_back(0);
+            Pk.list_off = (int32_t)lists.size();
+            lists.insert(lists.end(), items.begin(), items.end());
+            Pk.count = (int32_t)(items.size() / 4);
+            Pk.level = L.level = lvl;
+            L.count = (int32_t)(tiles.size() - L.list_off);
+            xcd_order(tiles, L.list_off);
+            gemm_dst->push_back(Pk);
+            gemm_dst->push_back(L);
+            return;
+        }
         for (int tile = 0; tile < GEMM_NTILES; tile++) {
             std::vector<const GemmProb *> sel;
             for (const GemmProb &g : pv)
-                if (bf16 ? tile == 0 : policy[use * GEMM_NBUCKETS + gemm_bucket(g.m, g.n)] == tile) sel.push_back(&g);
+                if (policy[use * GEMM_NBUCKETS + gemm_bucket(g.m, g.n)] == tile) sel.push_back(&g);
             if (sel.empty()) continue;
             Launch L{};
             L.kind = L_GEMM;
             L.variant = schur;
             L.on_store = schur == 2 ? 1 : 0;   // the pivot-block inversions work in the store
-            L.tile = bf16 ? GEMM_TILE_BF16 : tile;
+            L.tile = tile;
             for (const GemmProb *g : sel) {
                 L.flops += 2.0 * g->m * (double)g->n * g->k;
                 L.bytes += ((double)g->m * g->k + (double)g->k * g->n + 2.0 * g->m * (double)g->n) * ebytes;
@@ -1228,6 +1276,8 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
     HIPCHK(P.d_ws.alloc((size_t)std::max<int64_t>(P.ws_elems, 16) * P.esize() * P.nbatch));
     HIPCHK(hipMemset(P.d_ws.p, 0, (size_t)std::max<int64_t>(P.ws_elems, 16) * P.esize() * P.nbatch));
     HIPCHK(P.d_redo.alloc((size_t)4 * P.nbatch));
+    HIPCHK(P.d_pack.alloc((size_t)std::max<int64_t>(P.pack_tiles, 1) * PK_TILE * 2 * P.nbatch));
+    HIPCHK(P.d_pexp.alloc((size_t)std::max<int64_t>(P.pack_tiles, 1) * 4 * P.nbatch));
     HIPCHK(P.d_prow.alloc((size_t)n * 4 * P.nbatch));
     HIPCHK(P.d_info.alloc((size_t)4 * P.nbatch));
     HIPCHK(P.d_pinfo.alloc((size_t)4 * PINFO_STRIDE * nsuper * P.nbatch));
@@ -1296,7 +1346,8 @@ static void fill_stats(const mf_plan *P, mf_plan_stats *st)
     st->factor_flops = P->flops;
     st->gemm_flops = P->gemm_flops;
     st->gemm_flops_big = P->gemm_flops_big;
-    st->front_bytes = (double)(P->store_elems + P->ws_elems) * (double)P->esize() * P->nbatch;
+    st->front_bytes = ((double)(P->store_elems + P->ws_elems) * (double)P->esize() + (double)P->pack_tiles * PK_TILE * 2.0) *
+                      P->nbatch;
     st->store_bytes = (double)P->store_elems * (double)P->esize() * P->nbatch;
     st->arena_bytes = (double)P->ws_elems * (double)P->esize() * P->nbatch;
     st->resident_front_bytes = (double)P->ws_sum_elems * (double)P->esize() * P->nbatch;
@@ -1380,6 +1431,9 @@ static int run_factor(mf_plan &P, const double *d_ax, const double *d_rs, int nm
     a.only = redo ? P.d_redo.as<int32_t>() : nullptr;
     a.spec = P.d_spec.p;
     a.spec_stride = P.spec_slots * SPEC_SLOT;
+    a.pack = P.d_pack.p;
+    a.pexp = P.d_pexp.as<int32_t>();
+    a.pack_tiles = P.pack_tiles;
     a.pinfo_stride = (int64_t)PINFO_STRIDE * P.nsuper;
     a.ws_stride = P.ws_elems;
     a.store_stride = P.store_elems;
