@@ -58,7 +58,8 @@ typedef struct mf_options {
     int32_t precision; /* enum mf_precision */
     int32_t small_nf;  /* fronts with nf <= small_nf take the one-workgroup fused kernel;
                           0 = library default */
-    int32_t use_graph; /* 1: capture the factor schedule in a hipGraph and replay it */
+    int32_t use_graph; /* 1: capture the factor schedule and the solve sweeps in hipGraphs and
+                          replay them (same kernels in the same order: identical results) */
     int32_t nbatch;    /* matrices sharing the pattern that one plan holds (>= 1) */
     int32_t pivoting;  /* enum mf_pivoting */
     float pivot_tol;   /* MF_PIVOT_AUTO accepts the diagonal while every multiplier satisfies

@@ -149,6 +149,14 @@ struct mf_plan {
     DevBuf d_pack, d_pexp;
     DevBuf d_solve, d_anorm, d_npart;   // solve buffers (carved as SolveBufs), ||A||, its partials
     SolveBufs sb;                       // the buffers the current solve call uses
+    bool sb_scratch = false;            // sb points into the allocation-free scratch
+    struct SolveGraph {
+        hipGraphExec_t exec = nullptr;
+        int64_t nrhs = 0;
+        int nmat = 0;
+        const double *rhs = nullptr;
+        const void *w0 = nullptr;
+    } sgraph[2];                        // captured sweeps: [0] solve from B, [1] correction from R
     DevBuf d_Lp, d_Up, d_uoff, d_Li, d_Lx, d_Ui, d_Ux, d_rowfac;
     int64_t solve_nrhs = 0;
     bool factored = false;
@@ -172,6 +180,8 @@ struct mf_plan {
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         for (int g = 0; g < 3; g++)
             if (graph[g]) (void)hipGraphExecDestroy(graph[g]);
+        for (int g = 0; g < 2; g++)
+            if (sgraph[g].exec) (void)hipGraphExecDestroy(sgraph[g].exec);
         if (stream) (void)hipStreamDestroy(stream);
     }
     size_t esize() const { return prec == MF_FP64 ? 8 : 4; }
@@ -1709,9 +1719,37 @@ static int solve_core(mf_plan *P, int64_t nrhs, int nmat, int32_t refine_steps, 
     const int64_t n = P->n;
     auto t0 = std::chrono::steady_clock::now();
     int rc;
-    auto once = [&](const double *rhs, int acc) {
+    auto eager = [&](const double *rhs, int acc) {
         return P->prec == MF_FP64 ? solve_once<double>(*P, rhs, nrhs, acc, nmat)
                                   : solve_once<float>(*P, rhs, nrhs, acc, nmat);
+    };
+    // With use_graph, each sweep pair (the solve from B, the correction from R) is captured once
+    // per (nrhs, nmat, buffers) and replayed: the same kernels in the same order, so the same bits.
+    auto once = [&](const double *rhs, int acc) -> int {
+        if (!P->use_graph || P->prof || P->sb_scratch) return eager(rhs, acc);
+        mf_plan::SolveGraph &G = P->sgraph[acc];
+        if (!G.exec || G.nrhs != nrhs || G.nmat != nmat || G.rhs != rhs || G.w0 != P->sb.w0) {
+            if (G.exec) { (void)hipGraphExecDestroy(G.exec); G.exec = nullptr; }
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
+            const int r = eager(rhs, acc);
+            const hipError_t ec = hipStreamEndCapture(P->stream, &g);
+            if (r != MF_OK || ec != hipSuccess) {
+                if (ec == hipSuccess && g) (void)hipGraphDestroy(g);
+                if (r != MF_OK) return r;
+                HIPCHK(ec);
+            }
+            const hipError_t ei = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) G.exec = nullptr;
+            HIPCHK(ei);
+            G.nrhs = nrhs;
+            G.nmat = nmat;
+            G.rhs = rhs;
+            G.w0 = P->sb.w0;
+        }
+        HIPCHK(hipGraphLaunch(G.exec, P->stream));
+        return MF_OK;
     };
     auto norms = [&](std::vector<double> &h) -> int {
         HIPCHK(launch_residual(P->d_csr_rp.as<int32_t>(), P->d_csr_ci.as<int32_t>(), P->d_csr_vmap.as<int32_t>(),
@@ -1887,6 +1925,7 @@ extern "C" int mf_solve_device_noalloc(mf_plan *P, int64_t nrhs, const double *d
     if (rc != MF_OK) return rc;
     const SolveBufs own = P->sb;
     solve_layout(*P, nrhs, 1, static_cast<unsigned char *>(base), &P->sb);
+    P->sb_scratch = true;   // no graph capture: instantiating one allocates
     const size_t n8 = (size_t)P->n * 8;
     hipError_t e = hipMemcpy2DAsync(P->sb.B, n8, d_B, (size_t)ldb * 8, n8, nrhs, hipMemcpyDeviceToDevice, P->stream);
     if (e == hipSuccess) {
@@ -1897,6 +1936,7 @@ extern "C" int mf_solve_device_noalloc(mf_plan *P, int64_t nrhs, const double *d
         }
     }
     P->sb = own;
+    P->sb_scratch = false;
     if (rc != MF_OK) return rc;
     HIPCHK(e);
     return MF_OK;

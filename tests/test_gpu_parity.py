@@ -203,3 +203,21 @@ def test_many_rhs_chunks_match_columns(name, k):
     for j in (0, 7, 8, 15, 16, 20):
         xj = I.solve(F, B[:, j], refine=1)
         assert np.abs(X[:, j] - xj).max() <= 1e-14 * np.abs(xj).max()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_graph_replayed_solves_match_eager(precision):
+    """With use_graph the solve sweeps are captured once and replayed: every replay (several
+    solves in a row, with refinement steps in between, and a change of the RHS count that forces
+    a new capture) gives the eager plan's result bit for bit."""
+    A = M.poisson3d_perturbed(12, seed=5)
+    S = I.symbolic_mf(A)
+    Fe = I.splu(A, method="multifrontal", symbolic=S, precision=precision)
+    Fg = I.splu(A, method="multifrontal", symbolic=S, precision=precision, use_graph=True)
+    rng = np.random.default_rng(6)
+    for nrhs in (3, 3, 5, 3):
+        B = rng.standard_normal((A.shape[0], nrhs))
+        Xe = I.solve(Fe, B, refine=3, tol=1e-15)
+        Xg = I.solve(Fg, B, refine=3, tol=1e-15)
+        assert np.array_equal(Xe, Xg)
+        assert Fe.last_report["iterations"] == Fg.last_report["iterations"]
