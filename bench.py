@@ -292,7 +292,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--paths", default="fp64,fp32,bf16,fp64+relax,fp32+relax,bf16+relax",
                     help="precisions to run; '+relax' uses the relaxed supernode amalgamation")
-    ap.add_argument("--graph", type=int, default=1, help="replay the factor schedule as a hipGraph")
+    ap.add_argument("--graph", type=int, default=1, help="replay the factor schedule and solve sweeps as hipGraphs")
+    ap.add_argument("--tune", type=int, default=1, help="tune the gemm tile policy on the plan's Schur launches")
     ap.add_argument("--batch", default="512,16,64", help="nmat,k,nrhs of the same-pattern batch run ('' = skip)")
     ap.add_argument("--convdiff", type=int, default=64, help="grid of the conv-diff COLAMD + BF16 run (0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -318,6 +319,20 @@ def main():
     S_relax = I.symbolic_mf(A, relax=True) if any(p.endswith("+relax") for p in paths) else None
     Rs = I.row_scaling(A, I.SCALE_SUM)
     hip = Hip()
+    # tile policy per precision from the plan's own Schur launches: the cost model predicts, a
+    # measurement confirms every non-default pick (a few hundred timings, under a second)
+    tuning = {}
+    if args.tune:
+        from pureumfpack_hip import tuning as T
+        for prec in sorted({p.split("+")[0] for p in paths} - {"bf16"}):
+            t0 = time.perf_counter()
+            tu = T.PlanTuner(S_relax if S_relax is not None else S, prec)
+            pol = tu.with_model(seed=0)
+            T.set_policy(prec, pol)
+            tuning[prec] = {"policy_schur_by_bucket": pol[1].tolist(), "timings": tu.timings,
+                            "pairs": len(tu.pairs()), "tune_s": time.perf_counter() - t0,
+                            "schur_launches_ms_default": tu.policy_time(np.zeros_like(pol)) * 1e3,
+                            "schur_launches_ms_tuned": tu.policy_time(pol) * 1e3}
     results, Xs = {}, {}
     for path in paths:
         prec, relaxed = path.split("+")[0], path.endswith("+relax")
@@ -362,6 +377,7 @@ def main():
         "speedup_vs_fp64_same_analysis": (results[same_fp64]["seconds"] / R["seconds"]) if same_fp64 in results
         else None,
         "roofline": roof,
+        "tuning": tuning,
     }
     if args.batch:
         nmat, kb, nrb = (int(v) for v in args.batch.split(","))

@@ -234,6 +234,11 @@ int mf_gemm_tile_info(int32_t *ntiles, int32_t *tile_m, int32_t *tile_n, int32_t
 int mf_time_gemm(int32_t precision, int64_t m, int64_t n, int64_t k, int64_t nprob, int32_t tile, int32_t reps,
                  double *seconds);
 
+/* average seconds of one launch of the nprob problems mnk[3q..3q+2] = (m, n, k) (independent
+ * C -= A B, as a plan groups a level's Schur complements) with tile shape `tile`. */
+int mf_time_gemm_list(int32_t precision, int64_t nprob, const int32_t *mnk, int32_t tile, int32_t reps,
+                      double *seconds);
+
 /* Copies `bytes` (a multiple of 16) between two fresh device buffers with a plain 16-byte
  * copy kernel: the known traffic for calibrating hardware byte counters. */
 int mf_copy_probe(int64_t bytes, double *seconds);

@@ -2142,6 +2142,64 @@ extern "C" int mf_time_gemm(int32_t precision, int64_t m, int64_t n, int64_t k, 
     return MF_OK;
 }
 
+// Times one launch of a list of independent problems C(m x n) -= A(m x k) B(k x n) (mnk: nprob
+// triples) with tile shape `tile`: the autotuner's measurement on a plan's real gemm launches.
+extern "C" int mf_time_gemm_list(int32_t precision, int64_t nprob, const int32_t *mnk, int32_t tile, int32_t reps,
+                                 double *seconds)
+{
+    if (!seconds || !mnk || nprob < 1 || reps < 1) return fail(MF_BADARG, "bad gemm timing request");
+    if (tile < 0 || tile >= GEMM_NTILES) return fail(MF_BADARG, "tile index out of range");
+    const size_t esz = precision == MF_FP64 ? 8 : 4;
+    std::vector<GemmProb> pv;
+    std::vector<GemmTile> tv;
+    int64_t off = 0;
+    for (int64_t q = 0; q < nprob; q++) {
+        const int64_t m = mnk[3 * q], n = mnk[3 * q + 1], k = mnk[3 * q + 2];
+        if (m < 1 || n < 1 || k < 1 || m > INT32_MAX / 2 || n > INT32_MAX / 2 || k > INT32_MAX / 2)
+            return fail(MF_BADARG, "bad gemm problem size");
+        GemmProb g{};
+        g.aoff = off;
+        g.boff = off + m * k;
+        g.coff = off + m * k + k * n;
+        g.m = (int32_t)m; g.n = (int32_t)n; g.k = (int32_t)k;
+        g.lda = (int32_t)m; g.ldb = (int32_t)k; g.ldc = (int32_t)m;
+        off += m * k + k * n + m * n;
+        for (int64_t tn = 0; tn < n; tn += GEMM_TILE_N[tile])
+            for (int64_t tm = 0; tm < m; tm += GEMM_TILE_M[tile]) tv.push_back({(int32_t)q, (int32_t)tm, (int32_t)tn, 0});
+        pv.push_back(g);
+    }
+    xcd_order(tv, 0);
+    DevBuf pool, dprobs, dtiles;
+    HIPCHK(pool.alloc((size_t)off * esz));
+    HIPCHK(hipMemset(pool.p, 0, (size_t)off * esz));
+    HIPCHK(upload(dprobs, pv));
+    HIPCHK(upload(dtiles, tv));
+    hipStream_t st;
+    HIPCHK(hipStreamCreate(&st));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    auto launch = [&]() {
+        return esz == 8 ? launch_gemm_tile<double>(tile, 1, dprobs.as<GemmProb>(), dtiles.as<GemmTile>(), (int)tv.size(),
+                                                    pool.p, 0, 1, nullptr, st)
+                        : launch_gemm_tile<float>(tile, 1, dprobs.as<GemmProb>(), dtiles.as<GemmTile>(), (int)tv.size(),
+                                                   pool.p, 0, 1, nullptr, st);
+    };
+    hipError_t err = launch();
+    if (err == hipSuccess) err = hipEventRecord(e0, st);
+    for (int r = 0; r < reps && err == hipSuccess; r++) err = launch();
+    if (err == hipSuccess) err = hipEventRecord(e1, st);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    float ms = 0;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    HIPCHK(err);
+    *seconds = ms * 1e-3 / reps;
+    return MF_OK;
+}
+
 // A plain copy of `bytes` bytes between two fresh device buffers (16-byte vectors): the known
 // traffic against which hardware byte counters are calibrated.
 extern "C" int mf_copy_probe(int64_t bytes, double *seconds)

@@ -77,6 +77,8 @@ SIGNATURES = {
     "mf_gemm_tile_info": (ctypes.c_int, [c_i32p, c_i32p, c_i32p, c_i32p, c_i32p]),
     "mf_time_gemm": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                     ctypes.c_int32, ctypes.c_int32, c_f64p]),
+    "mf_time_gemm_list": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, c_i32p, ctypes.c_int32, ctypes.c_int32,
+                                         c_f64p]),
     "mf_mlp_create": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
     "mf_mlp_destroy": (None, [ctypes.c_void_p]),
     "mf_mlp_num_params": (ctypes.c_int64, [ctypes.c_void_p]),

@@ -9,7 +9,7 @@ the same order, so a policy changes speed only, never a result bit.
   every bucket with mf_time_gemm and keep the fastest.
 * Learned (`Autotuner.with_model`): time a fraction of the (problem, shape) pairs, train the
   small MLP (mf_mlp_*: forward, backward and Adam run on the GPU) on log-time, predict the
-  rest and pick the predicted fastest; measured timing stays the judge: each pick is timed
+  rest and pick the predicted fastest; measured timing decides: each pick is timed
   against the default shape and loses to it when slower.
 """
 from __future__ import annotations
@@ -236,7 +236,7 @@ class Autotuner:
             pred = [sum(model.predict([(*sh, self.fp64, t)])[0] for sh in shs) for t in range(self.nt)]
             pick = int(np.argmin(pred))
             if pick != 0:
-                # the judge: the measured pick must beat the measured default
+                # the measured pick must beat the measured default
                 mp = sum(self.measure(sh, pick) for sh in shs)
                 md = sum(self.measure(sh, 0) for sh in shs)
                 if mp >= md:
@@ -249,3 +249,134 @@ class Autotuner:
         r = reps or self.reps
         return sum(time_gemm(self.precision, *sh, int(pol[use, b]), r)
                    for (use, b), shs in self.shapes.items() for sh in shs)
+
+
+def time_gemm_list(precision, mnk, tile, reps=10):
+    """Seconds per launch of the independent problems mnk = [(m, n, k), ...] with one tile shape."""
+    lib = _lib.load()
+    a = np.ascontiguousarray(np.asarray(mnk, np.int32).reshape(-1, 3))
+    sec = ctypes.c_double()
+    _check(lib.mf_time_gemm_list(PREC[precision], a.shape[0], a.ctypes.data_as(c_i32p), int(tile), int(reps),
+                                 ctypes.byref(sec)), "mf_time_gemm_list")
+    return sec.value
+
+
+def schur_launches(S, small_nf=64):
+    """The Schur-complement gemm launches a plan on analysis S issues, by size bucket: for every
+    etree level, the (m, n, k) = (nu, nu, np) of its large fronts (nf > small_nf, nu > 0),
+    split like the plan into fronts with nf >= 1024 and the rest and grouped by bucket.
+    Returns {bucket: [launch, ...]}, a launch being a tuple of (m, n, k)."""
+    _, _, hi = tile_info()
+    npv = np.diff(np.asarray(S.sstart))
+    nu = np.diff(np.asarray(S.upd_ptr))
+    nf = npv + nu
+    cp, ci = np.asarray(S.child_ptr), np.asarray(S.child_idx)
+    level = np.zeros(S.nsuper, np.int64)
+    for s in range(S.nsuper):
+        for c in ci[cp[s]:cp[s + 1]]:
+            level[s] = max(level[s], level[c] + 1)
+    out = {}
+    for lv in range(int(level.max()) + 1 if S.nsuper else 0):
+        idx = np.nonzero((level == lv) & (nf > small_nf) & (nu > 0))[0]
+        for big in (True, False):
+            sel = [s for s in idx if (nf[s] >= 1024) == big]
+            by_b = {}
+            for s in sel:
+                by_b.setdefault(bucket_of(int(nu[s]), int(nu[s]), hi), []).append((int(nu[s]), int(nu[s]), int(npv[s])))
+            for b, probs in by_b.items():
+                out.setdefault(b, []).append(tuple(probs))
+    return out
+
+
+def launch_features(probs, fp64, tile, tile_m, tile_n):
+    """Feature vector of one (launch, tile) pair: the launch's size (flops, problems, largest and
+    mean dimensions), the workgroups and the fill of the edge tiles under the shape, the
+    precision and the shape (one-hot)."""
+    tm, tn = tile_m[tile], tile_n[tile]
+    m = np.array([p[0] for p in probs], float)
+    n = np.array([p[1] for p in probs], float)
+    k = np.array([p[2] for p in probs], float)
+    wg = float((np.ceil(m / tm) * np.ceil(n / tn)).sum())
+    fill = float((m * n).sum() / (np.ceil(m / tm) * tm * np.ceil(n / tn) * tn).sum())
+    f = [math.log2(2.0 * (m * n * k).sum()), math.log2(len(probs)), math.log2(m.max()), math.log2(m.mean()),
+         math.log2(k.mean()), math.log2(wg), fill, float(fp64)]
+    f += [1.0 if t == tile else 0.0 for t in range(len(tile_m))]
+    return f
+
+
+class PlanTuner:
+    """Tile policy for the Schur complements (use 1) from the plan's own gemm launches.
+
+    `exhaustive` times every launch of every bucket under every tile shape; `with_model` times a
+    fraction of the (launch, shape) pairs, trains the GPU MLP on log-time, predicts the rest,
+    and keeps a predicted pick only when measurement confirms it beats the default shape.  Panel
+    updates (use 0) keep the default."""
+
+    def __init__(self, S, precision="fp32", reps=5, small_nf=64):
+        self.precision = precision
+        self.fp64 = precision == "fp64"
+        self.reps = reps
+        self.tile_m, self.tile_n, self.hi = tile_info()
+        self.nt = len(self.tile_m)
+        self.nb = len(self.hi) + 1
+        self.launches = schur_launches(S, small_nf)
+        self.timings = 0
+        self._cache = {}
+
+    def measure(self, probs, tile):
+        key = (probs, tile)
+        if key not in self._cache:
+            self._cache[key] = time_gemm_list(self.precision, probs, tile, self.reps)
+            self.timings += 1
+        return self._cache[key]
+
+    def bucket_time(self, b, tile):
+        return sum(self.measure(p, tile) for p in self.launches.get(b, []))
+
+    def pairs(self):
+        return [(b, p, t) for b, ls in self.launches.items() for p in ls for t in range(self.nt)]
+
+    def _base(self):
+        pol = np.zeros((2, self.nb), np.int32)
+        pol[0] = get_policy(self.precision)[0]   # panel updates keep the installed shapes
+        return pol
+
+    def exhaustive(self):
+        pol = self._base()
+        for b in self.launches:
+            pol[1, b] = int(np.argmin([self.bucket_time(b, t) for t in range(self.nt)]))
+        return pol
+
+    def with_model(self, frac=0.35, seed=0, epochs=1500, lr=3e-3, hidden=32):
+        rng = np.random.default_rng(seed)
+        allp = self.pairs()
+        take = rng.permutation(len(allp))[:max(self.nt * 3, int(frac * len(allp)))]
+        rows = [launch_features(allp[i][1], self.fp64, allp[i][2], self.tile_m, self.tile_n) for i in take]
+        secs = [self.measure(allp[i][1], allp[i][2]) for i in take]
+        X = np.array(rows, np.float64)
+        self.mu, self.sd = X.mean(axis=0), X.std(axis=0)
+        self.sd[self.sd < 1e-6] = 1.0
+        y = np.log(np.asarray(secs, np.float64))
+        ymu, ysd = float(y.mean()), float(y.std() or 1.0)
+        net = MLP(X.shape[1], hidden, seed)
+        self.train_loss = net.train(((X - self.mu) / self.sd).astype(np.float32),
+                                    ((y - ymu) / ysd).astype(np.float32), epochs, lr)
+
+        def predict(probs, t):
+            f = (np.array([launch_features(probs, self.fp64, t, self.tile_m, self.tile_n)]) - self.mu) / self.sd
+            return float(np.exp(net.forward(f.astype(np.float32))[0] * ysd + ymu))
+
+        pol = self._base()
+        self.model_picks = {}
+        for b, ls in self.launches.items():
+            pred = [sum(predict(p, t) for p in ls) for t in range(self.nt)]
+            pick = int(np.argmin(pred))
+            self.model_picks[b] = pick
+            if pick != 0 and self.bucket_time(b, pick) >= self.bucket_time(b, 0):
+                pick = 0   # measurement overrules a wrong prediction
+            pol[1, b] = pick
+        return pol
+
+    def policy_time(self, pol):
+        """Measured time of all Schur launches under a policy (cached timings)."""
+        return sum(self.bucket_time(b, int(pol[1, b])) for b in self.launches)
