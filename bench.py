@@ -141,7 +141,8 @@ def rel_error(X, Xref):
     return float((np.abs(X - Xref).max(axis=0) / np.abs(Xref).max(axis=0)).max())
 
 
-def run_batch(hip, k, nmat, nrhs, precision, steps, warmup, tol, max_refine, graph=True, relax=False):
+def run_batch(hip, k, nmat, nrhs, precision, steps, warmup, tol, max_refine, graph=True, relax=False,
+              solve_path="auto"):
     """Same-pattern batch: nmat matrices with the poisson3d(k) pattern and unsymmetric values
     (diag 6 + U[0,1], off-diagonals -U[0.5,1.5]), nrhs randn right-hand sides each; one step =
     mf_refactor_batch_device + mf_solve_batch_device refined to bwd err <= tol."""
@@ -150,7 +151,7 @@ def run_batch(hip, k, nmat, nrhs, precision, steps, warmup, tol, max_refine, gra
     n = P.shape[0]
     t0 = time.perf_counter()
     S = I.symbolic_mf(P, relax=relax)
-    B = I.BatchedLU(P, nmat, precision=precision, use_graph=graph, symbolic=S)
+    B = I.BatchedLU(P, nmat, precision=precision, use_graph=graph, symbolic=S, solve_path=solve_path)
     Rs = np.stack([I.row_scaling(B.matrix(v), I.SCALE_SUM) for v in Ax])
     t_plan = time.perf_counter() - t0
     rhs = np.random.default_rng(3).standard_normal((nmat, nrhs, n))   # per matrix n x nrhs, column-major

@@ -54,6 +54,14 @@ enum mf_pivoting {
     MF_PIVOT_PARTIAL = 1  /* large fronts: always the partial-pivoting panel schedule */
 };
 
+enum mf_solve_path {
+    MF_SOLVE_AUTO = 0,    /* column-parallel sweeps from 512 right-hand-side columns (matrices
+                             times right-hand sides) when no pivot block exceeds 1024 columns,
+                             the per-front sweeps otherwise */
+    MF_SOLVE_FRONTS = 1,  /* per-front sweeps: a front's rows spread over the lanes */
+    MF_SOLVE_COLUMNS = 2  /* column-parallel sweeps: one lane per right-hand-side column */
+};
+
 typedef struct mf_options {
     int32_t precision; /* enum mf_precision */
     int32_t small_nf;  /* fronts with nf <= small_nf take the one-workgroup fused kernel;
@@ -66,7 +74,8 @@ typedef struct mf_options {
                           |l_ij| <= 1/pivot_tol.  0 = default: 1 for MF_FP64 (then the result
                           is exactly that of partial pivoting, src/multifrontal.jl:248), 0.1 for
                           the mixed precisions (threshold pivoting) */
-    int32_t reserved[2];
+    int32_t solve_path; /* enum mf_solve_path: how mf_solve* sweep the factors */
+    int32_t reserved;
 } mf_options;
 
 typedef struct mf_solve_report {

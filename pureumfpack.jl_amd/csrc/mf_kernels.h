@@ -177,6 +177,17 @@ hipError_t launch_prep_rhs(const double *B, int64_t ldb, const double *rs, int64
 template <typename T>
 hipError_t launch_finish_x(const T *x, const int32_t *qf, double *X, int64_t ldx, int64_t n, int64_t nrhs,
                            int accumulate, int64_t x_mat_stride, int nmat, hipStream_t st);
+// Column-parallel sweeps (one lane per right-hand-side column of the batch, vectors
+// interleaved by column: entry i of column g = mat * nrhs + r at i * C + g, C = nmat * nrhs)
+template <typename T>
+hipError_t launch_cs_level(int fwd, const SolveArgs &a, const int32_t *lists, int list_off, int count, int64_t C,
+                           int nrhs, hipStream_t st);
+template <typename T>
+hipError_t launch_cs_prep(const double *B, int64_t ldb, int64_t b_mat_stride, const double *rs, int64_t rs_stride,
+                          const int32_t *qinv, T *w0, int64_t n, int nrhs, int64_t C, hipStream_t st);
+template <typename T>
+hipError_t launch_cs_finish(const T *x, const int32_t *qinv, double *X, int64_t ldx, int64_t x_mat_stride, int64_t n,
+                            int nrhs, int64_t C, int accumulate, hipStream_t st);
 hipError_t launch_residual(const int32_t *rp, const int32_t *ci, const int32_t *vmap, const double *ax,
                            int64_t ax_stride, const double *X, int64_t ldx, const double *B, int64_t ldb, double *R,
                            int64_t n, int64_t nrhs, int64_t msx, int64_t msb, int nmat, hipStream_t st);

@@ -29,6 +29,13 @@ using namespace mf;
 static constexpr int64_t SOLVE_CK_HOST = 64;    // column chunk of the split-K solve products (k_rect_part)
 static constexpr int64_t SOLVE_FUSE_MIN = 128;  // fewer mid-size fronts in a level: split pipeline
 static constexpr int64_t FRONT_MAXNF_HOST = 1024;  // largest front of the fused per-front sweeps
+static constexpr int64_t CS_CH_HOST = 64;         // register chunk of the column-parallel sweeps
+// MF_SOLVE_AUTO takes the column-parallel sweeps from this many right-hand-side columns (the
+// batch's matrices times their right-hand sides) when no pivot block is larger than
+// CS_MAXNP: each lane runs a front's substitution by itself, so a wide pivot block is a long
+// sequential chain
+static constexpr int64_t CS_MIN_COLS = 512;
+static constexpr int64_t CS_MAXNP = 1024;
 
 static thread_local std::string g_err;
 // gemm tile policy per precision (0: FP64, 1: FP32/BF16), use (panel update / Schur) and size
@@ -135,13 +142,16 @@ struct mf_plan {
     // 64 < nf <= FRONT_MAXNF; the level lists above hold only the other fronts
     std::vector<int32_t> lvl_s64o, lvl_s64c, lvl_s1ko, lvl_s1kc;
     std::vector<int32_t> lvl_s1k_small;   // every such front of the level has nf <= 256
+    std::vector<int32_t> lvl_allo, lvl_allc;   // every front of the level (column-parallel sweeps)
+    int64_t maxnp = 0;                         // largest pivot block
+    int solve_path = MF_SOLVE_AUTO;
     int64_t factor_elems = 0, inv_elems = 0, tmp_elems = 0;  // store = factors | inverses | gemm temps
     int64_t ws_sum_elems = 0;  // what every front kept resident would take (for the stats)
     int64_t part_slots = 0;  // partial-sum slots (256 rows x nrhs each) the solve needs
     std::vector<double> lvl_lflops, lvl_uflops, lvl_lbytes, lvl_ubytes, lvl_vec;
     // device
     DevBuf d_sn, d_upd, d_rmap, d_child, d_orow, d_ocol, d_osrc, d_ocolptr, d_ecolptr, d_ext, d_eupd, d_arow;
-    DevBuf d_lists, d_probs, d_tiles, d_solvelist, d_qf;
+    DevBuf d_lists, d_probs, d_tiles, d_solvelist, d_qf, d_qinv;
     DevBuf d_csr_rp, d_csr_ci, d_csr_vmap;
     DevBuf d_store, d_ws, d_prow, d_info, d_ax, d_rs, d_pinfo, d_flag, d_spec, d_redo;
     int64_t spec_slots = 0;         // most fronts in one speculative diagonal launch
@@ -1215,6 +1225,12 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
         P.lvl_lbytes.push_back(lb);
         P.lvl_ubytes.push_back(ub);
         P.lvl_vec.push_back(vec);
+        P.lvl_allo.push_back((int32_t)slist.size());
+        for (int64_t s : bylevel[lv]) {
+            slist.push_back((int32_t)s);
+            P.maxnp = std::max(P.maxnp, np[s]);
+        }
+        P.lvl_allc.push_back((int32_t)bylevel[lv].size());
     }
     // CSC export offsets (src/multifrontal.jl:73-104, :277-317)
     P.Lp.assign(n + 1, 0);
@@ -1277,12 +1293,18 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
     HIPCHK(upload(P.d_tiles, tiles));
     HIPCHK(upload(P.d_solvelist, slist));
     HIPCHK(upload(P.d_qf, qf32));
+    {
+        std::vector<int32_t> qinv(n);
+        for (int64_t j = 0; j < n; j++) qinv[P.qf[j]] = (int32_t)j;
+        HIPCHK(upload(P.d_qinv, qinv));
+    }
     HIPCHK(upload(P.d_csr_rp, rp));
     HIPCHK(upload(P.d_csr_ci, ci));
     HIPCHK(upload(P.d_csr_vmap, vmap));
     HIPCHK(upload(P.d_uoff, P.uoff));
-    HIPCHK(P.d_store.alloc((size_t)P.store_elems * P.esize() * P.nbatch));
-    HIPCHK(hipMemset(P.d_store.p, 0, (size_t)P.store_elems * P.esize() * P.nbatch));
+    // the column-parallel sweeps read up to CS_CH_HOST elements past a front's factor blocks
+    HIPCHK(P.d_store.alloc(((size_t)P.store_elems * P.nbatch + CS_CH_HOST) * P.esize()));
+    HIPCHK(hipMemset(P.d_store.p, 0, ((size_t)P.store_elems * P.nbatch + CS_CH_HOST) * P.esize()));
     HIPCHK(P.d_ws.alloc((size_t)std::max<int64_t>(P.ws_elems, 16) * P.esize() * P.nbatch));
     HIPCHK(hipMemset(P.d_ws.p, 0, (size_t)std::max<int64_t>(P.ws_elems, 16) * P.esize() * P.nbatch));
     HIPCHK(P.d_redo.alloc((size_t)4 * P.nbatch));
@@ -1313,6 +1335,9 @@ static int plan_options(mf_plan &P, const mf_options *opt)
         P.pivoting = opt->pivoting;
         if (opt->pivot_tol < 0.f || opt->pivot_tol > 1.f) return fail(MF_BADARG, "pivot_tol must be in [0, 1]");
         P.pivot_tol = opt->pivot_tol;
+        if (opt->solve_path < MF_SOLVE_AUTO || opt->solve_path > MF_SOLVE_COLUMNS)
+            return fail(MF_BADARG, "unknown solve_path");
+        P.solve_path = opt->solve_path;
     }
     if (P.pivot_tol == 0.f) P.pivot_tol = P.prec == MF_FP64 ? 1.f : 0.1f;
     if (P.small_nf > 128) return fail(MF_BADARG, "small_nf must be <= 128");
@@ -1586,9 +1611,57 @@ extern "C" int mf_refactor_batch_device(mf_plan *P, int64_t nmat, const double *
 }
 
 // ------------------------------------------------------------------ solve
+static bool use_columns(const mf_plan &P, int64_t nrhs, int nmat)
+{
+    if (P.solve_path == MF_SOLVE_COLUMNS) return true;
+    if (P.solve_path == MF_SOLVE_FRONTS) return false;
+    return nrhs * nmat >= CS_MIN_COLS && P.maxnp <= CS_MAXNP;
+}
+
+// Column-parallel solve of every column of the batch: two launches per level.
+template <typename T>
+static int solve_columns(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumulate, int nmat)
+{
+    const int64_t n = P.n, C = nrhs * nmat;
+    const double *rs = P.rs_given ? P.d_rs.as<double>() : nullptr;
+    HIPCHK(launch_cs_prep<T>(d_rhs, n, n * nrhs, rs, n, P.d_qinv.as<int32_t>(), reinterpret_cast<T *>(P.sb.w0), n,
+                             (int)nrhs, C, P.stream));
+    SolveArgs a{};
+    a.sn = P.d_sn.as<SnDev>();
+    a.upd = P.d_upd.as<int32_t>();
+    a.store = P.d_store.p;
+    a.prow = P.d_prow.as<int32_t>();
+    a.w0 = P.sb.w0;
+    a.y = P.sb.y;
+    a.x = P.sb.x;
+    a.uv = P.sb.uv;
+    a.zbuf = P.sb.zbuf;
+    a.ecolptr = P.d_ecolptr.as<int32_t>();
+    a.eupd = P.d_eupd.as<int32_t>();
+    a.n = n;
+    a.nupd = P.nupd;
+    a.nrhs = nrhs;
+    a.store_stride = P.store_elems;
+    a.n_stride = n;
+    const int32_t *sl = P.d_solvelist.as<int32_t>();
+    const double e = (double)P.esize();
+    for (int64_t lv = 0; lv < P.nlevels; lv++)
+        HIPCHK(profiled(P, 5, P.lvl_lflops[lv] * C, (P.lvl_lbytes[lv] * nmat + 3.0 * e * P.lvl_vec[lv] * C), [&] {
+            return launch_cs_level<T>(1, a, sl, P.lvl_allo[lv], P.lvl_allc[lv], C, (int)nrhs, P.stream);
+        }));
+    for (int64_t lv = P.nlevels - 1; lv >= 0; lv--)
+        HIPCHK(profiled(P, 6, P.lvl_uflops[lv] * C, (P.lvl_ubytes[lv] * nmat + 3.0 * e * P.lvl_vec[lv] * C), [&] {
+            return launch_cs_level<T>(0, a, sl, P.lvl_allo[lv], P.lvl_allc[lv], C, (int)nrhs, P.stream);
+        }));
+    HIPCHK(launch_cs_finish<T>(reinterpret_cast<const T *>(P.sb.x), P.d_qinv.as<int32_t>(), P.sb.X, n, n * nrhs, n,
+                               (int)nrhs, C, accumulate, P.stream));
+    return MF_OK;
+}
+
 template <typename T>
 static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumulate, int nmat)
 {
+    if (use_columns(P, nrhs, nmat)) return solve_columns<T>(P, d_rhs, nrhs, accumulate, nmat);
     // d_rhs: n x nrhs per matrix (ld n); result into P.sb.X (ld n), overwritten or accumulated
     const int64_t n = P.n;
     const double *rs = P.rs_given ? P.d_rs.as<double>() : nullptr;

@@ -40,6 +40,7 @@ hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st)
 
 constexpr int SOLVE_RC = 8;   // right-hand sides processed together per pass
 constexpr int SOLVE_NB = 64;  // triangular block size (one wave solves a diagonal block)
+constexpr int CS_CH = 64;     // rows per register chunk of the column-parallel sweeps
 
 // One wave: v <- L^-1 v for the unit lower bs x bs block staged in sd (leading dimension
 // ldd, lane = row; rows and columns >= bs of sd and v are zero).  Columns in groups of 8,
@@ -174,9 +175,10 @@ __global__ __launch_bounds__(256) void k_fwd_tri(SolveArgs a)
     const int64_t src = tid < np ? prow[c1 + tid] : 0;
     for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
-        if (tid < np) {
+        if (tid < TRI_NP) {
 #pragma unroll
-            for (int r = 0; r < SOLVE_RC; r++) g[r * TRI_NP + tid] = r < rc ? w0[(int64_t)(r0 + r) * a.n + src] : T(0);
+            for (int r = 0; r < SOLVE_RC; r++)
+                g[r * TRI_NP + tid] = (r < rc && tid < np) ? w0[(int64_t)(r0 + r) * a.n + src] : T(0);
         }
 #pragma unroll
         for (int h = 0; h < DPT; h++) {
@@ -486,9 +488,13 @@ __global__ __launch_bounds__(256) void k_bwd_tri(SolveArgs a)
     }
     for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
-        if (tid < np) {
+        // every row of g is written (zero past np): the U11 coupling product below reads the
+        // whole second block, with zero factor entries past np, and 0 * (stale LDS) need
+        // not be 0
+        if (tid < TRI_NP) {
 #pragma unroll
-            for (int r = 0; r < SOLVE_RC; r++) g[r * TRI_NP + tid] = r < rc ? x[(int64_t)(r0 + r) * a.n + c1 + tid] : T(0);
+            for (int r = 0; r < SOLVE_RC; r++)
+                g[r * TRI_NP + tid] = (r < rc && tid < np) ? x[(int64_t)(r0 + r) * a.n + c1 + tid] : T(0);
         }
         if (bs1 > 0) {
 #pragma unroll
@@ -1042,38 +1048,54 @@ __global__ void k_finish_x(const T *__restrict__ x, const int32_t *__restrict__ 
     }
 }
 
-// R = B - A X with A in CSR (row-parallel, fixed summation order per row).
-__global__ void k_residual(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                           const int32_t *__restrict__ vmap, const double *__restrict__ ax,
-                           int64_t ax_stride, const double *__restrict__ X, int64_t ldx,
-                           const double *__restrict__ B, int64_t ldb, double *__restrict__ R,
-                           int64_t n, int64_t nrhs, int64_t mat_stride_x, int64_t mat_stride_b)
+// R = B - A X with A in CSR: one thread per row (of one matrix), its entries read once for
+// eight right-hand sides at a time; each R(i, c) sums its row in entry order from 0.
+constexpr int RES_RC = 8;
+__global__ __launch_bounds__(256) void k_residual(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                  const int32_t *__restrict__ vmap, const double *__restrict__ ax,
+                                                  int64_t ax_stride, const double *__restrict__ X, int64_t ldx,
+                                                  const double *__restrict__ B, int64_t ldb, double *__restrict__ R,
+                                                  int64_t n, int64_t nrhs, int64_t mat_stride_x, int64_t mat_stride_b)
 {
     const int mat = blockIdx.y;
     const double *axm = ax + mat * ax_stride;
-    const int64_t tot = n * nrhs;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
-        int64_t i = e % n, c = e / n;
-        const double *x = X + mat * mat_stride_x + c * ldx;
-        double s = 0.0;
-        for (int p = rp[i]; p < rp[i + 1]; p++) s += axm[vmap[p]] * x[ci[p]];
-        R[mat * mat_stride_b + i + c * n] = B[mat * mat_stride_b + i + c * ldb] - s;
+    const double *Xm = X + mat * mat_stride_x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int p0 = rp[i], p1 = rp[i + 1];
+        for (int64_t c0 = 0; c0 < nrhs; c0 += RES_RC) {
+            const int rc = (int)min((int64_t)RES_RC, nrhs - c0);
+            double s[RES_RC];
+#pragma unroll
+            for (int c = 0; c < RES_RC; c++) s[c] = 0.0;
+            for (int p = p0; p < p1; p++) {
+                const double a = axm[vmap[p]];
+                const double *xj = Xm + ci[p] + c0 * ldx;
+#pragma unroll
+                for (int c = 0; c < RES_RC; c++)
+                    if (c < rc) s[c] += a * xj[c * ldx];
+            }
+#pragma unroll
+            for (int c = 0; c < RES_RC; c++)
+                if (c < rc)
+                    R[mat * mat_stride_b + i + (c0 + c) * n] = B[mat * mat_stride_b + i + (c0 + c) * ldb] - s[c];
+        }
     }
 }
 
-// Norms for the backward error, two passes.  Pass 1: grid (NORM_BLOCKS, nrhs, nmat); block b
-// of column c reduces its slice to part[((mat * nrhs + c) * NORM_BLOCKS + b) * 5 + q] with
-// q = 0 max|R|, 1 max|X|, 2 max|B|, 3 sum R^2, 4 sum B^2.  Pass 2 folds the blocks in order.
+// Norms for the backward error, two passes.  Pass 1: grid (nb, nrhs, nmat), nb <= NORM_BLOCKS
+// blocks per column (about 2048 entries each); block b of column c reduces its slice to
+// part[((mat * nrhs + c) * nb + b) * 5 + q] with q = 0 max|R|, 1 max|X|, 2 max|B|, 3 sum R^2,
+// 4 sum B^2.  Pass 2 folds the blocks in order.
 constexpr int NORM_BLOCKS = 64;
 
 __global__ __launch_bounds__(256) void k_norms_part(const double *__restrict__ R, const double *__restrict__ X,
                                                     const double *__restrict__ B, int64_t n, int64_t nrhs,
                                                     double *__restrict__ part)
 {
-    const int b = blockIdx.x, col = blockIdx.y, mat = blockIdx.z;
+    const int b = blockIdx.x, col = blockIdx.y, mat = blockIdx.z, nb = gridDim.x;
     const int64_t off = ((int64_t)mat * nrhs + col) * n;
     double mr = 0, mx = 0, mb = 0, sr = 0, sb = 0;
-    for (int64_t i = (int64_t)b * blockDim.x + threadIdx.x; i < n; i += (int64_t)NORM_BLOCKS * blockDim.x) {
+    for (int64_t i = (int64_t)b * blockDim.x + threadIdx.x; i < n; i += (int64_t)nb * blockDim.x) {
         const double rv = R[off + i], xv = X[off + i], bv = B[off + i];
         mr = fmax(mr, fabs(rv)); mx = fmax(mx, fabs(xv)); mb = fmax(mb, fabs(bv));
         sr += rv * rv; sb += bv * bv;
@@ -1089,16 +1111,16 @@ __global__ __launch_bounds__(256) void k_norms_part(const double *__restrict__ R
         }
         __syncthreads();
     }
-    if (threadIdx.x < 5) part[(((int64_t)mat * nrhs + col) * NORM_BLOCKS + b) * 5 + threadIdx.x] = sh[threadIdx.x][0];
+    if (threadIdx.x < 5) part[(((int64_t)mat * nrhs + col) * nb + b) * 5 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-__global__ void k_norms_fold(const double *__restrict__ part, int64_t ncols, double *__restrict__ out)
+__global__ void k_norms_fold(const double *__restrict__ part, int64_t ncols, int nb, double *__restrict__ out)
 {
     const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (c >= ncols) return;
     double v[5] = {0, 0, 0, 0, 0};
-    for (int b = 0; b < NORM_BLOCKS; b++) {
-        const double *p = part + (c * NORM_BLOCKS + b) * 5;
+    for (int b = 0; b < nb; b++) {
+        const double *p = part + (c * nb + b) * 5;
         for (int q = 0; q < 3; q++) v[q] = fmax(v[q], p[q]);
         for (int q = 3; q < 5; q++) v[q] += p[q];
     }
@@ -1181,6 +1203,230 @@ __global__ __launch_bounds__(256) void k_export(const SnDev *__restrict__ sn, co
             Ui[up + b] = c1 + b + 1;
             Ux[up + b] = (double)U12[b + (int64_t)t * np];
         }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Column-parallel sweeps, for solves with many right-hand-side columns (a batch of matrices
+// times their right-hand sides).  Column g = mat * nrhs + r of the batch is one lane; a wave
+// holds 64 consecutive columns and a work item is (front, wave of columns).  Each lane runs
+// the front's substitution for its own column in the order of the one-column sweeps:
+//   forward:  z = own entries + the children's update rows (extend-add list order), pivot part
+//             permuted by the block pivoting, y_j = z_j and z_i -= L(i, j) y_j for j ascending;
+//   backward: s = y_piv - U12 x_upd, then x_j = s_j / U(j, j) and s_i -= U(i, j) x_j for j
+//             descending.
+// The vectors are interleaved by column (entry i of column g at i * C + g, C = nmat * nrhs),
+// so each vector access of a wave is 64 consecutive words.  When nrhs is a multiple of 64 a
+// wave holds one matrix and every factor entry is one scalar load for the 64 lanes.  Rows
+// pass through registers in chunks of CS_CH; entries of a chunk beyond the front are read
+// from the store's slack (CS_CH elements past each matrix's factors) and never written back.
+// ------------------------------------------------------------------------------------------
+template <typename T, bool UNI>
+__device__ __forceinline__ void cs_fwd_front(const SolveArgs &a, int slot, int64_t gw, int64_t C, int nrhs)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t g = gw + lane;
+    const bool on = g < C;
+    const int64_t gc = on ? g : C - 1;                 // loads of idle lanes stay in bounds
+    const int mat = UNI ? (int)(gw / nrhs) : (int)(gc / nrhs);
+    const SnDev d = a.sn[a.list[slot]];
+    const int nf = d.nf, np = d.np, c1 = d.c1;
+    const T *F = reinterpret_cast<const T *>(a.store) + (int64_t)mat * a.store_stride + d.foff;
+    const int32_t *prow = a.prow + (int64_t)mat * a.n_stride;
+    const T *w0 = reinterpret_cast<const T *>(a.w0);
+    const T *uvr = reinterpret_cast<const T *>(a.uv);
+    T *y = reinterpret_cast<T *>(a.y);
+    T *uv = reinterpret_cast<T *>(a.uv);
+    T *zs = reinterpret_cast<T *>(a.zbuf);            // the gathered pivot part, V-row order
+    // gather the pivot part (own entries plus the children's update rows)
+    for (int p = 0; p < np; p++) {
+        T v = w0[(int64_t)(c1 + p) * C + gc];
+        const int q1 = a.ecolptr[d.fcol + p + 1];
+        for (int q = a.ecolptr[d.fcol + p]; q < q1; q++) v += uvr[(int64_t)a.eupd[q] * C + gc];
+        if (on) zs[(int64_t)(c1 + p) * C + g] = v;
+    }
+    // pivot part, chunk by chunk: earlier chunks' y, then the chunk's own unit lower triangle
+    for (int b0 = 0; b0 < np; b0 += CS_CH) {
+        const int nb = min(CS_CH, np - b0);
+        T z[CS_CH];
+#pragma unroll
+        for (int r = 0; r < CS_CH; r++) {
+            const int i = min(b0 + r, np - 1);
+            z[r] = zs[(int64_t)(prow[c1 + i]) * C + gc];   // factor position i takes V-row prow[c1 + i]
+        }
+        for (int j = 0; j < b0; j++) {
+            const T yj = y[(int64_t)(c1 + j) * C + gc];
+            const T *col = F + (int64_t)j * nf + b0;
+#pragma unroll
+            for (int r = 0; r < CS_CH; r++) z[r] -= col[r] * yj;
+        }
+#pragma unroll
+        for (int jj = 0; jj < CS_CH; jj++) {
+            if (jj >= nb) continue;
+            const T yj = z[jj];
+            const T *col = F + (int64_t)(b0 + jj) * nf + b0;
+#pragma unroll
+            for (int r = jj + 1; r < CS_CH; r++) z[r] -= col[r] * yj;
+        }
+        if (on) {
+#pragma unroll
+            for (int r = 0; r < CS_CH; r++)
+                if (r < nb) y[(int64_t)(c1 + b0 + r) * C + g] = z[r];
+        }
+    }
+    // update rows: the children's contributions minus L21 y
+    for (int u0 = np; u0 < nf; u0 += CS_CH) {
+        const int nb = min(CS_CH, nf - u0);
+        T z[CS_CH];
+#pragma unroll
+        for (int r = 0; r < CS_CH; r++) {
+            T v = T(0);
+            if (r < nb) {
+                const int q1 = a.ecolptr[d.fcol + u0 + r + 1];
+                for (int q = a.ecolptr[d.fcol + u0 + r]; q < q1; q++) v += uvr[(int64_t)a.eupd[q] * C + gc];
+            }
+            z[r] = v;
+        }
+        for (int j = 0; j < np; j++) {
+            const T yj = y[(int64_t)(c1 + j) * C + gc];
+            const T *col = F + (int64_t)j * nf + u0;
+#pragma unroll
+            for (int r = 0; r < CS_CH; r++) z[r] -= col[r] * yj;
+        }
+        if (on) {
+#pragma unroll
+            for (int r = 0; r < CS_CH; r++)
+                if (r < nb) uv[(int64_t)(d.upd_off + u0 - np + r) * C + g] = z[r];
+        }
+    }
+}
+
+template <typename T, bool UNI>
+__device__ __forceinline__ void cs_bwd_front(const SolveArgs &a, int slot, int64_t gw, int64_t C, int nrhs)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t g = gw + lane;
+    const bool on = g < C;
+    const int64_t gc = on ? g : C - 1;
+    const int mat = UNI ? (int)(gw / nrhs) : (int)(gc / nrhs);
+    const SnDev d = a.sn[a.list[slot]];
+    const int nf = d.nf, np = d.np, nu = d.nu, c1 = d.c1;
+    const T *F = reinterpret_cast<const T *>(a.store) + (int64_t)mat * a.store_stride + d.foff;
+    const T *U12 = reinterpret_cast<const T *>(a.store) + (int64_t)mat * a.store_stride + d.u12off;   // ld np
+    const int32_t *upd = a.upd + d.upd_off;
+    const T *y = reinterpret_cast<const T *>(a.y);
+    T *x = reinterpret_cast<T *>(a.x);
+    for (int b0 = ((np - 1) / CS_CH) * CS_CH; b0 >= 0; b0 -= CS_CH) {
+        const int nb = min(CS_CH, np - b0);
+        T z[CS_CH];
+#pragma unroll
+        for (int r = 0; r < CS_CH; r++) z[r] = y[(int64_t)(c1 + min(b0 + r, np - 1)) * C + gc];
+        for (int t = 0; t < nu; t++) {
+            const T xt = x[(int64_t)upd[t] * C + gc];
+            const T *col = U12 + (int64_t)t * np + b0;
+#pragma unroll
+            for (int r = 0; r < CS_CH; r++) z[r] -= col[r] * xt;
+        }
+        for (int j = b0 + CS_CH; j < np; j++) {
+            const T xj = x[(int64_t)(c1 + j) * C + gc];
+            const T *col = F + (int64_t)j * nf + b0;
+#pragma unroll
+            for (int r = 0; r < CS_CH; r++) z[r] -= col[r] * xj;
+        }
+#pragma unroll
+        for (int jj = CS_CH - 1; jj >= 0; jj--) {
+            if (jj >= nb) continue;
+            const T *col = F + (int64_t)(b0 + jj) * nf + b0;
+            const T xj = z[jj] / col[jj];
+            z[jj] = xj;
+#pragma unroll
+            for (int r = 0; r < jj; r++) z[r] -= col[r] * xj;
+        }
+        if (on) {
+#pragma unroll
+            for (int r = 0; r < CS_CH; r++)
+                if (r < nb) x[(int64_t)(c1 + b0 + r) * C + g] = z[r];
+        }
+    }
+}
+
+// One level of a column-parallel sweep: items (front, wave of columns) in a grid-stride loop,
+// four waves per workgroup; consecutive waves take the same front.
+template <typename T, bool UNI, bool FWD>
+__global__ __launch_bounds__(256) void k_cs_sweep(SolveArgs a, int count, int64_t C, int nrhs)
+{
+    const int64_t ncg = (C + 63) / 64;
+    const int64_t items = (int64_t)count * ncg;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int64_t it = (int64_t)blockIdx.x * 4 + wave; it < items; it += (int64_t)gridDim.x * 4) {
+        const int slot = (int)(it / ncg);
+        const int64_t gw = (it - (int64_t)slot * ncg) * 64;
+        if (FWD)
+            cs_fwd_front<T, UNI>(a, slot, gw, C, nrhs);
+        else
+            cs_bwd_front<T, UNI>(a, slot, gw, C, nrhs);
+    }
+}
+
+// Interleaved right-hand sides: w0[i * C + g] = Rs[qf[i]] * B[qf[i], r] of matrix mat (column g
+// = mat * nrhs + r), through a 64 x 64 LDS tile: B is read down its columns (consecutive
+// original rows q), w0 written along its rows (consecutive columns g) at i = qinv[q].
+template <typename T>
+__global__ __launch_bounds__(256) void k_cs_prep(const double *__restrict__ B, int64_t ldb, int64_t b_mat_stride,
+                                                 const double *__restrict__ rs, int64_t rs_stride,
+                                                 const int32_t *__restrict__ qinv, T *__restrict__ w0, int64_t n,
+                                                 int nrhs, int64_t C)
+{
+    __shared__ double tile[64][65];
+    const int64_t g0 = (int64_t)blockIdx.x * 64;
+    for (int64_t q0 = (int64_t)blockIdx.y * 64; q0 < n; q0 += (int64_t)gridDim.y * 64) {
+        for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+            const int ql = e & 63, gl = e >> 6;
+            const int64_t g = g0 + gl, q = q0 + ql;
+            double v = 0.0;
+            if (g < C && q < n) {
+                const int mat = (int)(g / nrhs), r = (int)(g - (int64_t)mat * nrhs);
+                v = B[mat * b_mat_stride + q + r * ldb];
+                if (rs) v *= rs[mat * rs_stride + q];
+            }
+            tile[gl][ql] = v;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+            const int gl = e & 63, ql = e >> 6;
+            const int64_t g = g0 + gl, q = q0 + ql;
+            if (g < C && q < n) w0[(int64_t)qinv[q] * C + g] = (T)tile[gl][ql];
+        }
+        __syncthreads();
+    }
+}
+
+// X[qf[k], r] of matrix mat = x[k * C + g] (accumulate: +=, the FP64 refinement update), through
+// the same tiles read along x's rows and written down X's columns.
+template <typename T>
+__global__ __launch_bounds__(256) void k_cs_finish(const T *__restrict__ x, const int32_t *__restrict__ qinv,
+                                                   double *__restrict__ X, int64_t ldx, int64_t x_mat_stride,
+                                                   int64_t n, int nrhs, int64_t C, int accumulate)
+{
+    __shared__ double tile[64][65];
+    const int64_t g0 = (int64_t)blockIdx.x * 64;
+    for (int64_t q0 = (int64_t)blockIdx.y * 64; q0 < n; q0 += (int64_t)gridDim.y * 64) {
+        for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+            const int gl = e & 63, ql = e >> 6;
+            const int64_t g = g0 + gl, q = q0 + ql;
+            tile[gl][ql] = (g < C && q < n) ? (double)x[(int64_t)qinv[q] * C + g] : 0.0;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+            const int ql = e & 63, gl = e >> 6;
+            const int64_t g = g0 + gl, q = q0 + ql;
+            if (g < C && q < n) {
+                const int mat = (int)(g / nrhs), r = (int)(g - (int64_t)mat * nrhs);
+                double *dst = X + mat * x_mat_stride + q + r * ldx;
+                *dst = accumulate ? *dst + tile[gl][ql] : tile[gl][ql];
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -1308,11 +1554,50 @@ hipError_t launch_finish_x(const T *x, const int32_t *qf, double *X, int64_t ldx
                        x_mat_stride);
     return hipGetLastError();
 }
+template <typename T>
+hipError_t launch_cs_level(int fwd, const SolveArgs &a, const int32_t *lists, int list_off, int count, int64_t C,
+                           int nrhs, hipStream_t st)
+{
+    if (count <= 0) return hipSuccess;
+    SolveArgs b = a;
+    b.list = lists + list_off;
+    const int64_t items = (int64_t)count * ((C + 63) / 64);
+    const dim3 g((unsigned)std::min<int64_t>((items + 3) / 4, 16384)), blk(256);
+    const bool uni = nrhs % 64 == 0;
+    if (fwd) {
+        if (uni) hipLaunchKernelGGL((k_cs_sweep<T, true, true>), g, blk, 0, st, b, count, C, nrhs);
+        else hipLaunchKernelGGL((k_cs_sweep<T, false, true>), g, blk, 0, st, b, count, C, nrhs);
+    } else {
+        if (uni) hipLaunchKernelGGL((k_cs_sweep<T, true, false>), g, blk, 0, st, b, count, C, nrhs);
+        else hipLaunchKernelGGL((k_cs_sweep<T, false, false>), g, blk, 0, st, b, count, C, nrhs);
+    }
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_cs_prep(const double *B, int64_t ldb, int64_t b_mat_stride, const double *rs, int64_t rs_stride,
+                          const int32_t *qinv, T *w0, int64_t n, int nrhs, int64_t C, hipStream_t st)
+{
+    const int64_t gx = (C + 63) / 64, gy = std::max<int64_t>(1, std::min<int64_t>((n + 63) / 64, 65535));
+    if (gx > INT32_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_cs_prep<T>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, B, ldb, b_mat_stride, rs,
+                       rs_stride, qinv, w0, n, nrhs, C);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_cs_finish(const T *x, const int32_t *qinv, double *X, int64_t ldx, int64_t x_mat_stride, int64_t n,
+                            int nrhs, int64_t C, int accumulate, hipStream_t st)
+{
+    const int64_t gx = (C + 63) / 64, gy = std::max<int64_t>(1, std::min<int64_t>((n + 63) / 64, 65535));
+    if (gx > INT32_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_cs_finish<T>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, x, qinv, X, ldx,
+                       x_mat_stride, n, nrhs, C, accumulate);
+    return hipGetLastError();
+}
 hipError_t launch_residual(const int32_t *rp, const int32_t *ci, const int32_t *vmap, const double *ax,
                            int64_t ax_stride, const double *X, int64_t ldx, const double *B, int64_t ldb, double *R,
                            int64_t n, int64_t nrhs, int64_t msx, int64_t msb, int nmat, hipStream_t st)
 {
-    int blocks = (int)std::min<int64_t>((n * nrhs + 255) / 256, 8192);
+    int blocks = (int)std::min<int64_t>((n + 255) / 256, 8192);
     hipLaunchKernelGGL(k_residual, dim3(blocks, nmat), dim3(256), 0, st, rp, ci, vmap, ax, ax_stride, X, ldx, B, ldb,
                        R, n, nrhs, msx, msb);
     return hipGetLastError();
@@ -1320,9 +1605,10 @@ hipError_t launch_residual(const int32_t *rp, const int32_t *ci, const int32_t *
 hipError_t launch_norms(const double *R, const double *X, const double *B, int64_t n, int64_t nrhs, double *part,
                         double *out, int nmat, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_norms_part, dim3(NORM_BLOCKS, (unsigned)nrhs, nmat), dim3(256), 0, st, R, X, B, n, nrhs, part);
+    const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(NORM_BLOCKS, (n + 2047) / 2048));
+    hipLaunchKernelGGL(k_norms_part, dim3(nb, (unsigned)nrhs, nmat), dim3(256), 0, st, R, X, B, n, nrhs, part);
     const int64_t ncols = nrhs * nmat;
-    hipLaunchKernelGGL(k_norms_fold, dim3((unsigned)((ncols + 63) / 64)), dim3(64), 0, st, part, ncols, out);
+    hipLaunchKernelGGL(k_norms_fold, dim3((unsigned)((ncols + 63) / 64)), dim3(64), 0, st, part, ncols, nb, out);
     return hipGetLastError();
 }
 hipError_t launch_anorm(const int32_t *rp, const int32_t *vmap, const double *ax, int64_t ax_stride, int64_t n,
@@ -1357,6 +1643,12 @@ hipError_t launch_export(const SnDev *sn, int64_t nsuper, const T *store, const 
                                            T *, int64_t, int64_t, int64_t, int, hipStream_t);                   \
     template hipError_t launch_finish_x<T>(const T *, const int32_t *, double *, int64_t, int64_t, int64_t,     \
                                            int, int64_t, int, hipStream_t);                                     \
+    template hipError_t launch_cs_level<T>(int, const SolveArgs &, const int32_t *, int, int, int64_t, int,      \
+                                           hipStream_t);                                                        \
+    template hipError_t launch_cs_prep<T>(const double *, int64_t, int64_t, const double *, int64_t,            \
+                                          const int32_t *, T *, int64_t, int, int64_t, hipStream_t);            \
+    template hipError_t launch_cs_finish<T>(const T *, const int32_t *, double *, int64_t, int64_t, int64_t,    \
+                                            int, int64_t, int, hipStream_t);                                    \
     template hipError_t launch_export<T>(const SnDev *, int64_t, const T *, const int32_t *, const int32_t *,   \
                                          const int64_t *, const int64_t *, const int64_t *, int64_t *, double *, \
                                          int64_t *, double *, hipStream_t);

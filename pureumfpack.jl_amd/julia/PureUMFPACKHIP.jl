@@ -22,10 +22,11 @@ struct MFOptions
     nbatch::Int32
     pivoting::Int32
     pivot_tol::Float32
-    reserved::NTuple{2, Int32}
+    solve_path::Int32
+    reserved::Int32
 end
-MFOptions(; precision = 0, small_nf = 0, use_graph = 0, nbatch = 1, pivoting = 0, pivot_tol = 0.0f0) =
-    MFOptions(precision, small_nf, use_graph, nbatch, pivoting, pivot_tol, (0, 0))
+MFOptions(; precision = 0, small_nf = 0, use_graph = 0, nbatch = 1, pivoting = 0, pivot_tol = 0.0f0, solve_path = 0) =
+    MFOptions(precision, small_nf, use_graph, nbatch, pivoting, pivot_tol, solve_path, 0)
 
 struct MFSolveReport
     iterations::Int32

@@ -13,6 +13,7 @@ MF_OK, MF_SINGULAR, MF_DIM, MF_BADARG, MF_HIP, MF_OOM = range(6)
 MF_FP64, MF_FP32, MF_BF16 = 0, 1, 2
 MF_ORDER_AMD, MF_ORDER_COLAMD, MF_ORDER_GIVEN = 0, 1, 2
 MF_PIVOT_AUTO, MF_PIVOT_PARTIAL = 0, 1
+MF_SOLVE_AUTO, MF_SOLVE_FRONTS, MF_SOLVE_COLUMNS = 0, 1, 2
 
 c_i64p = ctypes.POINTER(ctypes.c_int64)
 c_f64p = ctypes.POINTER(ctypes.c_double)
@@ -24,7 +25,7 @@ c_void_pp = ctypes.POINTER(ctypes.c_void_p)
 class mf_options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("small_nf", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("nbatch", ctypes.c_int32), ("pivoting", ctypes.c_int32), ("pivot_tol", ctypes.c_float),
-                ("reserved", ctypes.c_int32 * 2)]
+                ("solve_path", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class mf_solve_report(ctypes.Structure):

@@ -24,6 +24,7 @@ from ._lib import MF_BADARG, MF_DIM, MF_OK, MF_SINGULAR, c_f64p, c_i64p
 SCALE_NONE, SCALE_SUM, SCALE_MAX = 0, 1, 2
 _PREC = {"fp64": _lib.MF_FP64, "fp32": _lib.MF_FP32, "bf16": _lib.MF_BF16}
 _PIVOT = {"auto": _lib.MF_PIVOT_AUTO, "partial": _lib.MF_PIVOT_PARTIAL}
+_SOLVE = {"auto": _lib.MF_SOLVE_AUTO, "fronts": _lib.MF_SOLVE_FRONTS, "columns": _lib.MF_SOLVE_COLUMNS}
 
 
 class DimensionMismatch(ValueError):
@@ -173,19 +174,21 @@ class _Plan:
     """Owns an mf_plan (device schedule + front storage); destroyed with the Python object."""
 
     def __init__(self, A, S: SymbolicMF, precision="fp64", small_nf=0, use_graph=False, nbatch=1,
-                 pivoting="auto", pivot_tol=0.0):
+                 pivoting="auto", pivot_tol=0.0, solve_path="auto"):
         lib = _lib.load()
         if precision not in _PREC:
             raise ArgumentError(f"unknown precision {precision}")
         if pivoting not in _PIVOT:
             raise ArgumentError(f"unknown pivoting {pivoting}")
+        if solve_path not in _SOLVE:
+            raise ArgumentError(f"unknown solve_path {solve_path}")
         self.lib = lib
         self.n = A.shape[0]
         self.nnz = A.nnz
         self.S = S
         self.precision = precision
         opt = _lib.mf_options(_PREC[precision], int(small_nf), int(bool(use_graph)), int(nbatch),
-                              _PIVOT[pivoting], float(pivot_tol))
+                              _PIVOT[pivoting], float(pivot_tol), _SOLVE[solve_path])
         Ap, Ai = _pattern_1based(A)
         arrs = [np.ascontiguousarray(x, dtype=np.int64) + 1 for x in
                 (S.qf, S.sstart, S.upd_ptr, S.upd_idx, S.child_ptr, S.child_idx)]
@@ -359,7 +362,7 @@ def multifrontal_lu(A, q=None, tol=None, check=True, *, precision="fp64", symbol
 
 
 def splu(A, method="gplu", ordering="amd", tol=0.1, scale=SCALE_SUM, check=True, *, precision="fp64",
-         symbolic=None, small_nf=0, use_graph=False, pivoting="auto", pivot_tol=0.0):
+         symbolic=None, small_nf=0, use_graph=False, pivoting="auto", pivot_tol=0.0, solve_path="auto"):
     """High-level factorization (src/interface.jl:93-120).  Only method="multifrontal" is served
     by this package; method="gplu" (the reference default, a different algorithm) raises
     NotImplementedError, any other method raises ArgumentError.  `ordering` and `tol` are
@@ -372,7 +375,7 @@ def splu(A, method="gplu", ordering="amd", tol=0.1, scale=SCALE_SUM, check=True,
         Rs = row_scaling(A, scale)
         S = symbolic if symbolic is not None else symbolic_mf(A)
         plan = _Plan(A, S, precision=precision, small_nf=small_nf, use_graph=use_graph, pivoting=pivoting,
-                     pivot_tol=pivot_tol)
+                     pivot_tol=pivot_tol, solve_path=solve_path)
         plan.factor(A.data, None if scale == SCALE_NONE else Rs, check=check)
         return PureLU(plan, Rs, A)
     if method == "gplu":
@@ -389,7 +392,7 @@ class BatchedLU:
     values: nmat x nnz in the CSC order of `pattern` (the pattern's own data are ignored)."""
 
     def __init__(self, pattern, nbatch, precision="fp32", scale=SCALE_SUM, symbolic=None, use_graph=True,
-                 pivoting="auto", pivot_tol=0.0):
+                 pivoting="auto", pivot_tol=0.0, solve_path="auto"):
         A = _csc(pattern)
         if A.shape[0] != A.shape[1]:
             raise DimensionMismatch("BatchedLU requires square matrices")
@@ -397,7 +400,7 @@ class BatchedLU:
         self.scale = scale
         self.S = symbolic if symbolic is not None else symbolic_mf(A)
         self._plan = _Plan(A, self.S, precision=precision, use_graph=use_graph, nbatch=nbatch,
-                           pivoting=pivoting, pivot_tol=pivot_tol)
+                           pivoting=pivoting, pivot_tol=pivot_tol, solve_path=solve_path)
         self.nbatch = int(nbatch)
         self.Rs = None
 
