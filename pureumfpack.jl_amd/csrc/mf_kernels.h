@@ -145,6 +145,7 @@ struct SolveArgs {
     void *zbuf;    // large fronts: permuted pivot part (forward) / U12-corrected part (backward)
     const int32_t *ecolptr;  // extend-add lists of the factorization (per front position)
     const int32_t *eupd;     // their entries as offsets of the child's update rows in uv
+    const int32_t *egpos;    // the front position (ecolptr index) of every entry
     int64_t n, nupd, nrhs;
     int64_t store_stride, n_stride, part_stride;
 };
@@ -181,7 +182,7 @@ hipError_t launch_finish_x(const T *x, const int32_t *qf, double *X, int64_t ldx
 // interleaved by column: entry i of column g = mat * nrhs + r at i * C + g, C = nmat * nrhs)
 template <typename T>
 hipError_t launch_cs_level(int fwd, const SolveArgs &a, const int32_t *lists, int list_off, int count, int64_t C,
-                           int nrhs, hipStream_t st);
+                           int nrhs, int ch, hipStream_t st);
 template <typename T>
 hipError_t launch_cs_prep(const double *B, int64_t ldb, int64_t b_mat_stride, const double *rs, int64_t rs_stride,
                           const int32_t *qinv, T *w0, int64_t n, int nrhs, int64_t C, hipStream_t st);

@@ -143,6 +143,7 @@ struct mf_plan {
     std::vector<int32_t> lvl_s64o, lvl_s64c, lvl_s1ko, lvl_s1kc;
     std::vector<int32_t> lvl_s1k_small;   // every such front of the level has nf <= 256
     std::vector<int32_t> lvl_allo, lvl_allc;   // every front of the level (column-parallel sweeps)
+    std::vector<int32_t> lvl_csch;             // their register chunk: 75th percentile of nf, 8..64
     int64_t maxnp = 0;                         // largest pivot block
     int solve_path = MF_SOLVE_AUTO;
     int64_t factor_elems = 0, inv_elems = 0, tmp_elems = 0;  // store = factors | inverses | gemm temps
@@ -151,7 +152,7 @@ struct mf_plan {
     std::vector<double> lvl_lflops, lvl_uflops, lvl_lbytes, lvl_ubytes, lvl_vec;
     // device
     DevBuf d_sn, d_upd, d_rmap, d_child, d_orow, d_ocol, d_osrc, d_ocolptr, d_ecolptr, d_ext, d_eupd, d_arow;
-    DevBuf d_lists, d_probs, d_tiles, d_solvelist, d_qf, d_qinv;
+    DevBuf d_lists, d_probs, d_tiles, d_solvelist, d_qf, d_qinv, d_egpos;
     DevBuf d_csr_rp, d_csr_ci, d_csr_vmap;
     DevBuf d_store, d_ws, d_prow, d_info, d_ax, d_rs, d_pinfo, d_flag, d_spec, d_redo;
     int64_t spec_slots = 0;         // most fronts in one speculative diagonal launch
@@ -636,6 +637,10 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                 }
             }
     }
+    // the front position (index into ecolptr) each extend-add entry belongs to
+    std::vector<int32_t> egpos(std::max<size_t>(eupd.size(), 1), 0);
+    for (int64_t c = 0; c < fcol[nsuper]; c++)
+        for (int32_t q = ecolptr[c]; q < ecolptr[c + 1]; q++) egpos[q] = (int32_t)c;
     // children in reverse order (the order src/multifrontal.jl:236 absorbs them)
     std::vector<int32_t> child_rev;
     std::vector<int64_t> child_off(nsuper);
@@ -1226,11 +1231,22 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
         P.lvl_ubytes.push_back(ub);
         P.lvl_vec.push_back(vec);
         P.lvl_allo.push_back((int32_t)slist.size());
+        std::vector<int64_t> nfs;
         for (int64_t s : bylevel[lv]) {
             slist.push_back((int32_t)s);
             P.maxnp = std::max(P.maxnp, np[s]);
+            nfs.push_back(nf[s]);
         }
         P.lvl_allc.push_back((int32_t)bylevel[lv].size());
+        {
+            int32_t ch = 8;
+            if (!nfs.empty()) {
+                std::sort(nfs.begin(), nfs.end());
+                const int64_t p75 = nfs[std::min(nfs.size() - 1, (nfs.size() * 3) / 4)];
+                while (ch < 64 && ch < p75) ch *= 2;
+            }
+            P.lvl_csch.push_back(ch);
+        }
     }
     // CSC export offsets (src/multifrontal.jl:73-104, :277-317)
     P.Lp.assign(n + 1, 0);
@@ -1287,6 +1303,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
     HIPCHK(upload(P.d_ecolptr, ecolptr));
     HIPCHK(upload(P.d_ext, ext));
     HIPCHK(upload(P.d_eupd, eupd));
+    HIPCHK(upload(P.d_egpos, egpos));
     HIPCHK(upload(P.d_arow, arow));
     HIPCHK(upload(P.d_lists, lists));
     HIPCHK(upload(P.d_probs, probs));
@@ -1638,6 +1655,7 @@ static int solve_columns(mf_plan &P, const double *d_rhs, int64_t nrhs, int accu
     a.zbuf = P.sb.zbuf;
     a.ecolptr = P.d_ecolptr.as<int32_t>();
     a.eupd = P.d_eupd.as<int32_t>();
+    a.egpos = P.d_egpos.as<int32_t>();
     a.n = n;
     a.nupd = P.nupd;
     a.nrhs = nrhs;
@@ -1647,11 +1665,13 @@ static int solve_columns(mf_plan &P, const double *d_rhs, int64_t nrhs, int accu
     const double e = (double)P.esize();
     for (int64_t lv = 0; lv < P.nlevels; lv++)
         HIPCHK(profiled(P, 5, P.lvl_lflops[lv] * C, (P.lvl_lbytes[lv] * nmat + 3.0 * e * P.lvl_vec[lv] * C), [&] {
-            return launch_cs_level<T>(1, a, sl, P.lvl_allo[lv], P.lvl_allc[lv], C, (int)nrhs, P.stream);
+            return launch_cs_level<T>(1, a, sl, P.lvl_allo[lv], P.lvl_allc[lv], C, (int)nrhs, P.lvl_csch[lv],
+                                      P.stream);
         }));
     for (int64_t lv = P.nlevels - 1; lv >= 0; lv--)
         HIPCHK(profiled(P, 6, P.lvl_uflops[lv] * C, (P.lvl_ubytes[lv] * nmat + 3.0 * e * P.lvl_vec[lv] * C), [&] {
-            return launch_cs_level<T>(0, a, sl, P.lvl_allo[lv], P.lvl_allc[lv], C, (int)nrhs, P.stream);
+            return launch_cs_level<T>(0, a, sl, P.lvl_allo[lv], P.lvl_allc[lv], C, (int)nrhs, P.lvl_csch[lv],
+                                      P.stream);
         }));
     HIPCHK(launch_cs_finish<T>(reinterpret_cast<const T *>(P.sb.x), P.d_qinv.as<int32_t>(), P.sb.X, n, n * nrhs, n,
                                (int)nrhs, C, accumulate, P.stream));

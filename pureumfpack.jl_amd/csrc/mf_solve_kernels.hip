@@ -40,7 +40,6 @@ hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st)
 
 constexpr int SOLVE_RC = 8;   // right-hand sides processed together per pass
 constexpr int SOLVE_NB = 64;  // triangular block size (one wave solves a diagonal block)
-constexpr int CS_CH = 64;     // rows per register chunk of the column-parallel sweeps
 
 // One wave: v <- L^-1 v for the unit lower bs x bs block staged in sd (leading dimension
 // ldd, lane = row; rows and columns >= bs of sd and v are zero).  Columns in groups of 8,
@@ -1048,8 +1047,9 @@ __global__ void k_finish_x(const T *__restrict__ x, const int32_t *__restrict__ 
     }
 }
 
-// R = B - A X with A in CSR: one thread per row (of one matrix), its entries read once for
-// eight right-hand sides at a time; each R(i, c) sums its row in entry order from 0.
+// R = B - A X with A in CSR: one thread per (row, chunk of eight right-hand sides) of one
+// matrix, the row's entries read once per chunk; each R(i, c) sums its row in entry order
+// from 0.  Grid x = row blocks times chunks, y = matrix.
 constexpr int RES_RC = 8;
 __global__ __launch_bounds__(256) void k_residual(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                                   const int32_t *__restrict__ vmap, const double *__restrict__ ax,
@@ -1058,28 +1058,39 @@ __global__ __launch_bounds__(256) void k_residual(const int32_t *__restrict__ rp
                                                   int64_t n, int64_t nrhs, int64_t mat_stride_x, int64_t mat_stride_b)
 {
     const int mat = blockIdx.y;
+    const int64_t nrb = (n + 255) / 256;
+    const int64_t i = (int64_t)(blockIdx.x % nrb) * 256 + threadIdx.x;
+    const int64_t c0 = (int64_t)(blockIdx.x / nrb) * RES_RC;
+    if (i >= n || c0 >= nrhs) return;
     const double *axm = ax + mat * ax_stride;
     const double *Xm = X + mat * mat_stride_x;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int p0 = rp[i], p1 = rp[i + 1];
-        for (int64_t c0 = 0; c0 < nrhs; c0 += RES_RC) {
-            const int rc = (int)min((int64_t)RES_RC, nrhs - c0);
-            double s[RES_RC];
+    const int p0 = rp[i], p1 = rp[i + 1];
+    const int rc = (int)min((int64_t)RES_RC, nrhs - c0);
+    double s[RES_RC];
 #pragma unroll
-            for (int c = 0; c < RES_RC; c++) s[c] = 0.0;
-            for (int p = p0; p < p1; p++) {
-                const double a = axm[vmap[p]];
-                const double *xj = Xm + ci[p] + c0 * ldx;
+    for (int c = 0; c < RES_RC; c++) s[c] = 0.0;
+    if (rc == RES_RC) {   // full chunk: the eight loads of an entry go out together
+        for (int p = p0; p < p1; p++) {
+            const double a = axm[vmap[p]];
+            const double *xj = Xm + ci[p] + c0 * ldx;
+            double xv[RES_RC];
 #pragma unroll
-                for (int c = 0; c < RES_RC; c++)
-                    if (c < rc) s[c] += a * xj[c * ldx];
-            }
+            for (int c = 0; c < RES_RC; c++) xv[c] = xj[c * ldx];
+#pragma unroll
+            for (int c = 0; c < RES_RC; c++) s[c] += a * xv[c];
+        }
+    } else {
+        for (int p = p0; p < p1; p++) {
+            const double a = axm[vmap[p]];
+            const double *xj = Xm + ci[p] + c0 * ldx;
 #pragma unroll
             for (int c = 0; c < RES_RC; c++)
-                if (c < rc)
-                    R[mat * mat_stride_b + i + (c0 + c) * n] = B[mat * mat_stride_b + i + (c0 + c) * ldb] - s[c];
+                if (c < rc) s[c] += a * xj[c * ldx];
         }
     }
+#pragma unroll
+    for (int c = 0; c < RES_RC; c++)
+        if (c < rc) R[mat * mat_stride_b + i + (c0 + c) * n] = B[mat * mat_stride_b + i + (c0 + c) * ldb] - s[c];
 }
 
 // Norms for the backward error, two passes.  Pass 1: grid (nb, nrhs, nmat), nb <= NORM_BLOCKS
@@ -1218,11 +1229,151 @@ __global__ __launch_bounds__(256) void k_export(const SnDev *__restrict__ sn, co
 // The vectors are interleaved by column (entry i of column g at i * C + g, C = nmat * nrhs),
 // so each vector access of a wave is 64 consecutive words.  When nrhs is a multiple of 64 a
 // wave holds one matrix and every factor entry is one scalar load for the 64 lanes.  Rows
-// pass through registers in chunks of CS_CH; entries of a chunk beyond the front are read
-// from the store's slack (CS_CH elements past each matrix's factors) and never written back.
+// pass through registers in chunks of CH <= 64; factor entries of a chunk beyond the front are
+// read from the store's slack (64 elements past each matrix's factors) and never written back.
 // ------------------------------------------------------------------------------------------
-template <typename T, bool UNI>
-__device__ __forceinline__ void cs_fwd_front(const SolveArgs &a, int slot, int64_t gw, int64_t C, int nrhs)
+// z[r] -= col_j[r] * v_j for the columns j in [j0, j1) of a chunk, four columns per step:
+// the four vector values are loaded before any of their products
+// z[r] -= col_j[r] * v_j for the columns j in [j0, j1) of a chunk, eight columns per step,
+// their vector values and factor columns loaded before any of their products.  UNI (the
+// wave's lanes share one matrix): lane l loads row l of each column segment with one vector
+// load and row r reaches every lane by v_readlane, so eight columns are in flight without
+// tying up scalar registers; otherwise each lane reads its own matrix's entries.
+template <typename T, int CH, bool UNI, typename VF, typename CF>
+__device__ __forceinline__ void cs_cols(T (&z)[CH], int j0, int j1, VF vat, CF colat)
+{
+    const int lane = threadIdx.x & 63;
+    int j = j0;
+    for (; j + 8 <= j1; j += 8) {
+        T v[8];
+        if constexpr (UNI) {
+            T f[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) f[q] = colat(j + q)[lane];
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = vat(j + q);
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+#pragma unroll
+                for (int r = 0; r < CH; r++) z[r] -= lane_value(f[q], r) * v[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = vat(j + q);
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const T *col = colat(j + q);
+#pragma unroll
+                for (int r = 0; r < CH; r++) z[r] -= col[r] * v[q];
+            }
+        }
+    }
+    for (; j < j1; j++) {
+        const T vj = vat(j);
+        if constexpr (UNI) {
+            const T f = colat(j)[lane];
+#pragma unroll
+            for (int r = 0; r < CH; r++) z[r] -= lane_value(f, r) * vj;
+        } else {
+            const T *col = colat(j);
+#pragma unroll
+            for (int r = 0; r < CH; r++) z[r] -= col[r] * vj;
+        }
+    }
+}
+
+// the chunk's own unit lower triangle: y_j = z_j, z_r -= L(r, j) y_j for r > j
+template <typename T, int CH, bool UNI>
+__device__ __forceinline__ void cs_lower(T (&z)[CH], const T *D, int ld, int nb)
+{
+    const int lane = threadIdx.x & 63;
+    if constexpr (UNI) {
+        T f[CH];
+#pragma unroll
+        for (int jj = 0; jj < CH; jj++) f[jj] = jj < nb ? D[(int64_t)jj * ld + lane] : T(0);
+#pragma unroll
+        for (int jj = 0; jj < CH; jj++) {
+            if (jj >= nb) continue;
+            const T yj = z[jj];
+#pragma unroll
+            for (int r = jj + 1; r < CH; r++) z[r] -= lane_value(f[jj], r) * yj;
+        }
+    } else {
+#pragma unroll
+        for (int jj = 0; jj < CH; jj++) {
+            if (jj >= nb) continue;
+            const T yj = z[jj];
+            const T *col = D + (int64_t)jj * ld;
+#pragma unroll
+            for (int r = jj + 1; r < CH; r++) z[r] -= col[r] * yj;
+        }
+    }
+}
+
+// the chunk's upper triangle, last column first: x_j = z_j / U(j, j), z_r -= U(r, j) x_j for r < j
+template <typename T, int CH, bool UNI>
+__device__ __forceinline__ void cs_upper(T (&z)[CH], const T *D, int ld, int nb)
+{
+    const int lane = threadIdx.x & 63;
+    if constexpr (UNI) {
+        T f[CH];
+#pragma unroll
+        for (int jj = 0; jj < CH; jj++) f[jj] = jj < nb ? D[(int64_t)jj * ld + lane] : T(1);
+#pragma unroll
+        for (int jj = CH - 1; jj >= 0; jj--) {
+            if (jj >= nb) continue;
+            const T xj = z[jj] / lane_value(f[jj], jj);
+            z[jj] = xj;
+#pragma unroll
+            for (int r = 0; r < jj; r++) z[r] -= lane_value(f[jj], r) * xj;
+        }
+    } else {
+#pragma unroll
+        for (int jj = CH - 1; jj >= 0; jj--) {
+            if (jj >= nb) continue;
+            const T *col = D + (int64_t)jj * ld;
+            const T xj = z[jj] / col[jj];
+            z[jj] = xj;
+#pragma unroll
+            for (int r = 0; r < jj; r++) z[r] -= col[r] * xj;
+        }
+    }
+}
+
+// z[r] = base(r) + the children's update rows of front positions p0 + r (r < nb): the
+// extend-add entries of those positions are one contiguous range of the lists, walked eight
+// at a time with their loads in flight and summed per position in list order through this
+// wave's LDS rows acc[r * 64 + lane].
+template <typename T, int CH, typename BF>
+__device__ __forceinline__ void cs_gather(T (&z)[CH], const SolveArgs &a, const SnDev &d, int p0, int nb, int64_t C,
+                                          int64_t gc, T *acc, BF base)
+{
+    const int lane = threadIdx.x & 63;
+    const T *uvr = reinterpret_cast<const T *>(a.uv);
+#pragma unroll
+    for (int r = 0; r < CH; r++) acc[r * 64 + lane] = r < nb ? base(r) : T(0);
+    const int qa = a.ecolptr[d.fcol + p0], qb = a.ecolptr[d.fcol + p0 + nb];
+    const int pbase = d.fcol + p0;
+    for (int q = qa; q < qb; q += 8) {
+        int off[8], pos[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int qq = min(q + k, qb - 1);
+            off[k] = a.eupd[qq];
+            pos[k] = a.egpos[qq] - pbase;
+        }
+        T v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = uvr[(int64_t)off[k] * C + gc];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (q + k < qb) acc[pos[k] * 64 + lane] += v[k];
+    }
+#pragma unroll
+    for (int r = 0; r < CH; r++) z[r] = acc[r * 64 + lane];
+}
+
+template <typename T, bool UNI, int CH>
+__device__ __forceinline__ void cs_fwd_front(const SolveArgs &a, int slot, int64_t gw, int64_t C, int nrhs, T *acc)
 {
     const int lane = threadIdx.x & 63;
     const int64_t g = gw + lane;
@@ -1234,74 +1385,51 @@ __device__ __forceinline__ void cs_fwd_front(const SolveArgs &a, int slot, int64
     const T *F = reinterpret_cast<const T *>(a.store) + (int64_t)mat * a.store_stride + d.foff;
     const int32_t *prow = a.prow + (int64_t)mat * a.n_stride;
     const T *w0 = reinterpret_cast<const T *>(a.w0);
-    const T *uvr = reinterpret_cast<const T *>(a.uv);
     T *y = reinterpret_cast<T *>(a.y);
     T *uv = reinterpret_cast<T *>(a.uv);
     T *zs = reinterpret_cast<T *>(a.zbuf);            // the gathered pivot part, V-row order
-    // gather the pivot part (own entries plus the children's update rows)
-    for (int p = 0; p < np; p++) {
-        T v = w0[(int64_t)(c1 + p) * C + gc];
-        const int q1 = a.ecolptr[d.fcol + p + 1];
-        for (int q = a.ecolptr[d.fcol + p]; q < q1; q++) v += uvr[(int64_t)a.eupd[q] * C + gc];
-        if (on) zs[(int64_t)(c1 + p) * C + g] = v;
-    }
-    // pivot part, chunk by chunk: earlier chunks' y, then the chunk's own unit lower triangle
-    for (int b0 = 0; b0 < np; b0 += CS_CH) {
-        const int nb = min(CS_CH, np - b0);
-        T z[CS_CH];
-#pragma unroll
-        for (int r = 0; r < CS_CH; r++) {
-            const int i = min(b0 + r, np - 1);
-            z[r] = zs[(int64_t)(prow[c1 + i]) * C + gc];   // factor position i takes V-row prow[c1 + i]
-        }
-        for (int j = 0; j < b0; j++) {
-            const T yj = y[(int64_t)(c1 + j) * C + gc];
-            const T *col = F + (int64_t)j * nf + b0;
-#pragma unroll
-            for (int r = 0; r < CS_CH; r++) z[r] -= col[r] * yj;
-        }
-#pragma unroll
-        for (int jj = 0; jj < CS_CH; jj++) {
-            if (jj >= nb) continue;
-            const T yj = z[jj];
-            const T *col = F + (int64_t)(b0 + jj) * nf + b0;
-#pragma unroll
-            for (int r = jj + 1; r < CS_CH; r++) z[r] -= col[r] * yj;
-        }
+    const auto yat = [&](int j) { return y[(int64_t)(c1 + j) * C + gc]; };
+    // gather the pivot part (own entries plus the children's update rows), chunk by chunk
+    for (int p0 = 0; p0 < np; p0 += CH) {
+        const int nb = min(CH, np - p0);
+        T z[CH];
+        cs_gather<T, CH>(z, a, d, p0, nb, C, gc, acc, [&](int r) { return w0[(int64_t)(c1 + p0 + r) * C + gc]; });
         if (on) {
 #pragma unroll
-            for (int r = 0; r < CS_CH; r++)
+            for (int r = 0; r < CH; r++)
+                if (r < nb) zs[(int64_t)(c1 + p0 + r) * C + g] = z[r];
+        }
+    }
+    // pivot part, chunk by chunk: earlier chunks' y, then the chunk's own unit lower triangle
+    for (int b0 = 0; b0 < np; b0 += CH) {
+        const int nb = min(CH, np - b0);
+        T z[CH];
+#pragma unroll
+        for (int r = 0; r < CH; r++)   // factor position b0 + r takes V-row prow[c1 + b0 + r]
+            z[r] = r < nb ? zs[(int64_t)(prow[c1 + b0 + r]) * C + gc] : T(0);
+        cs_cols<T, CH, UNI>(z, 0, b0, yat, [&](int j) { return F + (int64_t)j * nf + b0; });
+        cs_lower<T, CH, UNI>(z, F + (int64_t)b0 * nf + b0, nf, nb);
+        if (on) {
+#pragma unroll
+            for (int r = 0; r < CH; r++)
                 if (r < nb) y[(int64_t)(c1 + b0 + r) * C + g] = z[r];
         }
     }
     // update rows: the children's contributions minus L21 y
-    for (int u0 = np; u0 < nf; u0 += CS_CH) {
-        const int nb = min(CS_CH, nf - u0);
-        T z[CS_CH];
-#pragma unroll
-        for (int r = 0; r < CS_CH; r++) {
-            T v = T(0);
-            if (r < nb) {
-                const int q1 = a.ecolptr[d.fcol + u0 + r + 1];
-                for (int q = a.ecolptr[d.fcol + u0 + r]; q < q1; q++) v += uvr[(int64_t)a.eupd[q] * C + gc];
-            }
-            z[r] = v;
-        }
-        for (int j = 0; j < np; j++) {
-            const T yj = y[(int64_t)(c1 + j) * C + gc];
-            const T *col = F + (int64_t)j * nf + u0;
-#pragma unroll
-            for (int r = 0; r < CS_CH; r++) z[r] -= col[r] * yj;
-        }
+    for (int u0 = np; u0 < nf; u0 += CH) {
+        const int nb = min(CH, nf - u0);
+        T z[CH];
+        cs_gather<T, CH>(z, a, d, u0, nb, C, gc, acc, [&](int) { return T(0); });
+        cs_cols<T, CH, UNI>(z, 0, np, yat, [&](int j) { return F + (int64_t)j * nf + u0; });
         if (on) {
 #pragma unroll
-            for (int r = 0; r < CS_CH; r++)
+            for (int r = 0; r < CH; r++)
                 if (r < nb) uv[(int64_t)(d.upd_off + u0 - np + r) * C + g] = z[r];
         }
     }
 }
 
-template <typename T, bool UNI>
+template <typename T, bool UNI, int CH>
 __device__ __forceinline__ void cs_bwd_front(const SolveArgs &a, int slot, int64_t gw, int64_t C, int nrhs)
 {
     const int lane = threadIdx.x & 63;
@@ -1316,45 +1444,39 @@ __device__ __forceinline__ void cs_bwd_front(const SolveArgs &a, int slot, int64
     const int32_t *upd = a.upd + d.upd_off;
     const T *y = reinterpret_cast<const T *>(a.y);
     T *x = reinterpret_cast<T *>(a.x);
-    for (int b0 = ((np - 1) / CS_CH) * CS_CH; b0 >= 0; b0 -= CS_CH) {
-        const int nb = min(CS_CH, np - b0);
-        T z[CS_CH];
+    for (int b0 = ((np - 1) / CH) * CH; b0 >= 0; b0 -= CH) {
+        const int nb = min(CH, np - b0);
+        T z[CH];
 #pragma unroll
-        for (int r = 0; r < CS_CH; r++) z[r] = y[(int64_t)(c1 + min(b0 + r, np - 1)) * C + gc];
-        for (int t = 0; t < nu; t++) {
-            const T xt = x[(int64_t)upd[t] * C + gc];
-            const T *col = U12 + (int64_t)t * np + b0;
-#pragma unroll
-            for (int r = 0; r < CS_CH; r++) z[r] -= col[r] * xt;
-        }
-        for (int j = b0 + CS_CH; j < np; j++) {
-            const T xj = x[(int64_t)(c1 + j) * C + gc];
-            const T *col = F + (int64_t)j * nf + b0;
-#pragma unroll
-            for (int r = 0; r < CS_CH; r++) z[r] -= col[r] * xj;
-        }
-#pragma unroll
-        for (int jj = CS_CH - 1; jj >= 0; jj--) {
-            if (jj >= nb) continue;
-            const T *col = F + (int64_t)(b0 + jj) * nf + b0;
-            const T xj = z[jj] / col[jj];
-            z[jj] = xj;
-#pragma unroll
-            for (int r = 0; r < jj; r++) z[r] -= col[r] * xj;
-        }
+        for (int r = 0; r < CH; r++) z[r] = r < nb ? y[(int64_t)(c1 + b0 + r) * C + gc] : T(0);
+        cs_cols<T, CH, UNI>(z, 0, nu, [&](int t) { return x[(int64_t)upd[t] * C + gc]; },
+                       [&](int t) { return U12 + (int64_t)t * np + b0; });
+        cs_cols<T, CH, UNI>(z, b0 + CH, np, [&](int j) { return x[(int64_t)(c1 + j) * C + gc]; },
+                       [&](int j) { return F + (int64_t)j * nf + b0; });
+        cs_upper<T, CH, UNI>(z, F + (int64_t)b0 * nf + b0, nf, nb);
         if (on) {
 #pragma unroll
-            for (int r = 0; r < CS_CH; r++)
+            for (int r = 0; r < CH; r++)
                 if (r < nb) x[(int64_t)(c1 + b0 + r) * C + g] = z[r];
         }
     }
 }
 
 // One level of a column-parallel sweep: items (front, wave of columns) in a grid-stride loop,
-// four waves per workgroup; consecutive waves take the same front.
-template <typename T, bool UNI, bool FWD>
-__global__ __launch_bounds__(256) void k_cs_sweep(SolveArgs a, int count, int64_t C, int nrhs)
+// four waves per workgroup; consecutive waves take the same front.  CH: rows per register
+// chunk, chosen per level from its front sizes (small chunks keep more waves resident).
+// Waves per SIMD the register budget must allow: the sweeps of small fronts are chains of
+// dependent loads, hidden only by many resident waves.
+template <typename T, int CH>
+constexpr int cs_min_waves()
 {
+    return sizeof(T) == 4 ? (CH <= 16 ? 8 : (CH <= 32 ? 5 : 3)) : (CH <= 16 ? 5 : 3);
+}
+
+template <typename T, bool UNI, bool FWD, int CH>
+__global__ __launch_bounds__(256, (cs_min_waves<T, CH>())) void k_cs_sweep(SolveArgs a, int count, int64_t C, int nrhs)
+{
+    __shared__ T acc[FWD ? 4 * CH * 64 : 1];   // per wave: the gather rows of one chunk
     const int64_t ncg = (C + 63) / 64;
     const int64_t items = (int64_t)count * ncg;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1362,9 +1484,9 @@ __global__ __launch_bounds__(256) void k_cs_sweep(SolveArgs a, int count, int64_
         const int slot = (int)(it / ncg);
         const int64_t gw = (it - (int64_t)slot * ncg) * 64;
         if (FWD)
-            cs_fwd_front<T, UNI>(a, slot, gw, C, nrhs);
+            cs_fwd_front<T, UNI, CH>(a, slot, gw, C, nrhs, acc + wave * CH * 64);
         else
-            cs_bwd_front<T, UNI>(a, slot, gw, C, nrhs);
+            cs_bwd_front<T, UNI, CH>(a, slot, gw, C, nrhs);
     }
 }
 
@@ -1385,7 +1507,7 @@ __global__ __launch_bounds__(256) void k_cs_prep(const double *__restrict__ B, i
             const int64_t g = g0 + gl, q = q0 + ql;
             double v = 0.0;
             if (g < C && q < n) {
-                const int mat = (int)(g / nrhs), r = (int)(g - (int64_t)mat * nrhs);
+                const int mat = (int)g / nrhs, r = (int)g - mat * nrhs;   // C < 2^31 (launcher)
                 v = B[mat * b_mat_stride + q + r * ldb];
                 if (rs) v *= rs[mat * rs_stride + q];
             }
@@ -1421,7 +1543,7 @@ __global__ __launch_bounds__(256) void k_cs_finish(const T *__restrict__ x, cons
             const int ql = e & 63, gl = e >> 6;
             const int64_t g = g0 + gl, q = q0 + ql;
             if (g < C && q < n) {
-                const int mat = (int)(g / nrhs), r = (int)(g - (int64_t)mat * nrhs);
+                const int mat = (int)g / nrhs, r = (int)g - mat * nrhs;   // C < 2^31 (launcher)
                 double *dst = X + mat * x_mat_stride + q + r * ldx;
                 *dst = accumulate ? *dst + tile[gl][ql] : tile[gl][ql];
             }
@@ -1554,22 +1676,37 @@ hipError_t launch_finish_x(const T *x, const int32_t *qf, double *X, int64_t ldx
                        x_mat_stride);
     return hipGetLastError();
 }
+template <typename T, int CH>
+static void launch_cs_ch(int fwd, bool uni, dim3 g, const SolveArgs &b, int count, int64_t C, int nrhs, hipStream_t st)
+{
+    const dim3 blk(256);
+    if (fwd) {
+        if (uni) hipLaunchKernelGGL((k_cs_sweep<T, true, true, CH>), g, blk, 0, st, b, count, C, nrhs);
+        else hipLaunchKernelGGL((k_cs_sweep<T, false, true, CH>), g, blk, 0, st, b, count, C, nrhs);
+    } else {
+        if (uni) hipLaunchKernelGGL((k_cs_sweep<T, true, false, CH>), g, blk, 0, st, b, count, C, nrhs);
+        else hipLaunchKernelGGL((k_cs_sweep<T, false, false, CH>), g, blk, 0, st, b, count, C, nrhs);
+    }
+}
 template <typename T>
 hipError_t launch_cs_level(int fwd, const SolveArgs &a, const int32_t *lists, int list_off, int count, int64_t C,
-                           int nrhs, hipStream_t st)
+                           int nrhs, int ch, hipStream_t st)
 {
     if (count <= 0) return hipSuccess;
     SolveArgs b = a;
     b.list = lists + list_off;
     const int64_t items = (int64_t)count * ((C + 63) / 64);
-    const dim3 g((unsigned)std::min<int64_t>((items + 3) / 4, 16384)), blk(256);
+    const dim3 g((unsigned)std::min<int64_t>((items + 3) / 4, 16384));
     const bool uni = nrhs % 64 == 0;
-    if (fwd) {
-        if (uni) hipLaunchKernelGGL((k_cs_sweep<T, true, true>), g, blk, 0, st, b, count, C, nrhs);
-        else hipLaunchKernelGGL((k_cs_sweep<T, false, true>), g, blk, 0, st, b, count, C, nrhs);
-    } else {
-        if (uni) hipLaunchKernelGGL((k_cs_sweep<T, true, false>), g, blk, 0, st, b, count, C, nrhs);
-        else hipLaunchKernelGGL((k_cs_sweep<T, false, false>), g, blk, 0, st, b, count, C, nrhs);
+    switch (ch) {
+    case 8: launch_cs_ch<T, 8>(fwd, uni, g, b, count, C, nrhs, st); break;
+    case 16: launch_cs_ch<T, 16>(fwd, uni, g, b, count, C, nrhs, st); break;
+    case 32: launch_cs_ch<T, 32>(fwd, uni, g, b, count, C, nrhs, st); break;
+    case 64:   // FP64 stays at 32 rows: 64 would take 128 KB of gather rows per workgroup
+        if constexpr (sizeof(T) == 4) launch_cs_ch<T, 64>(fwd, uni, g, b, count, C, nrhs, st);
+        else launch_cs_ch<T, 32>(fwd, uni, g, b, count, C, nrhs, st);
+        break;
+    default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
@@ -1578,7 +1715,7 @@ hipError_t launch_cs_prep(const double *B, int64_t ldb, int64_t b_mat_stride, co
                           const int32_t *qinv, T *w0, int64_t n, int nrhs, int64_t C, hipStream_t st)
 {
     const int64_t gx = (C + 63) / 64, gy = std::max<int64_t>(1, std::min<int64_t>((n + 63) / 64, 65535));
-    if (gx > INT32_MAX) return hipErrorInvalidValue;
+    if (C + 64 > INT32_MAX) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_cs_prep<T>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, B, ldb, b_mat_stride, rs,
                        rs_stride, qinv, w0, n, nrhs, C);
     return hipGetLastError();
@@ -1588,7 +1725,7 @@ hipError_t launch_cs_finish(const T *x, const int32_t *qinv, double *X, int64_t 
                             int nrhs, int64_t C, int accumulate, hipStream_t st)
 {
     const int64_t gx = (C + 63) / 64, gy = std::max<int64_t>(1, std::min<int64_t>((n + 63) / 64, 65535));
-    if (gx > INT32_MAX) return hipErrorInvalidValue;
+    if (C + 64 > INT32_MAX) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_cs_finish<T>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, x, qinv, X, ldx,
                        x_mat_stride, n, nrhs, C, accumulate);
     return hipGetLastError();
@@ -1597,9 +1734,10 @@ hipError_t launch_residual(const int32_t *rp, const int32_t *ci, const int32_t *
                            int64_t ax_stride, const double *X, int64_t ldx, const double *B, int64_t ldb, double *R,
                            int64_t n, int64_t nrhs, int64_t msx, int64_t msb, int nmat, hipStream_t st)
 {
-    int blocks = (int)std::min<int64_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_residual, dim3(blocks, nmat), dim3(256), 0, st, rp, ci, vmap, ax, ax_stride, X, ldx, B, ldb,
-                       R, n, nrhs, msx, msb);
+    const int64_t blocks = (n + 255) / 256 * ((nrhs + RES_RC - 1) / RES_RC);
+    if (blocks > INT32_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_residual, dim3((unsigned)blocks, nmat), dim3(256), 0, st, rp, ci, vmap, ax, ax_stride, X, ldx,
+                       B, ldb, R, n, nrhs, msx, msb);
     return hipGetLastError();
 }
 hipError_t launch_norms(const double *R, const double *X, const double *B, int64_t n, int64_t nrhs, double *part,
@@ -1643,7 +1781,7 @@ hipError_t launch_export(const SnDev *sn, int64_t nsuper, const T *store, const 
                                            T *, int64_t, int64_t, int64_t, int, hipStream_t);                   \
     template hipError_t launch_finish_x<T>(const T *, const int32_t *, double *, int64_t, int64_t, int64_t,     \
                                            int, int64_t, int, hipStream_t);                                     \
-    template hipError_t launch_cs_level<T>(int, const SolveArgs &, const int32_t *, int, int, int64_t, int,      \
+    template hipError_t launch_cs_level<T>(int, const SolveArgs &, const int32_t *, int, int, int64_t, int, int, \
                                            hipStream_t);                                                        \
     template hipError_t launch_cs_prep<T>(const double *, int64_t, int64_t, const double *, int64_t,            \
                                           const int32_t *, T *, int64_t, int, int64_t, hipStream_t);            \
