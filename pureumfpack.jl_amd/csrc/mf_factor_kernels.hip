@@ -973,13 +973,23 @@ __device__ __forceinline__ void gemm_tile(const GemmProb *__restrict__ probs, co
             Bs[(buf * BK + e % BK) * LB + e / BK] = rb[i];
         }
     };
-    const int nk = (pr.k + BK - 1) / BK;
-    gload(0);
-    sstore(0);
+    // k range that can hold nonzero products for this tile (triangular operands); the slices
+    // skipped contribute exact zeros, so the sums are unchanged
+    int kbeg = 0, kend = pr.k;
+    if (pr.flags & GEMM_A_LOWER) kend = min(kend, m0 + TM);
+    if (pr.flags & GEMM_A_UPPER) kbeg = max(kbeg, m0);
+    if (pr.flags & GEMM_B_LOWER) kbeg = max(kbeg, n0);
+    if (pr.flags & GEMM_B_UPPER) kend = min(kend, n0 + TN);
+    kbeg = kbeg / BK * BK;
+    const int nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+    if (nk > 0) {
+        gload(kbeg);
+        sstore(0);
+    }
     __syncthreads();
     for (int s = 0; s < nk; s++) {
         const int cur = s & 1;
-        if (s + 1 < nk) gload((s + 1) * BK);
+        if (s + 1 < nk) gload(kbeg + (s + 1) * BK);
         const T *Ac = As + (cur * BK + kl) * LA + wm + FM * slot;
         const T *Bc = Bs + (cur * BK + kl) * LB + wn + FN * slot;
 #pragma unroll

@@ -51,7 +51,11 @@ struct SnDev {
 
 // Grouped gemm problem, column-major, offsets in elements into one matrix's pool:
 //   C -= A B (flags 0), C += A B (GEMM_ADD), C = -A B (GEMM_SET), C = A B (GEMM_SET|GEMM_ADD).
-enum GemmFlags : int32_t { GEMM_ADD = 1, GEMM_SET = 2 };
+// A triangular operand (A_LOWER: A[i][k] = 0 for k > i, A_UPPER: for k < i, B_LOWER: B[k][j] = 0
+// for k < j, B_UPPER: for k > j, zeros stored) lets a tile skip the k-slices it knows are zero.
+enum GemmFlags : int32_t {
+    GEMM_ADD = 1, GEMM_SET = 2, GEMM_A_LOWER = 4, GEMM_A_UPPER = 8, GEMM_B_LOWER = 16, GEMM_B_UPPER = 32
+};
 struct GemmProb {
     i64 aoff, boff, coff;
     int32_t m, n, k, lda, ldb, ldc;

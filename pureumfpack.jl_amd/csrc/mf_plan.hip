@@ -761,8 +761,11 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             L.on_store = schur == 2 ? 1 : 0;   // the pivot-block inversions work in the store
             L.tile = tile;
             for (const GemmProb *g : sel) {
-                L.flops += 2.0 * g->m * (double)g->n * g->k;
-                L.bytes += ((double)g->m * g->k + (double)g->k * g->n + 2.0 * g->m * (double)g->n) * ebytes;
+                // a triangular operand halves the products that can be nonzero (and its bytes)
+                const bool ta = g->flags & (GEMM_A_LOWER | GEMM_A_UPPER), tb = g->flags & (GEMM_B_LOWER | GEMM_B_UPPER);
+                L.flops += (ta || tb ? 1.0 : 2.0) * g->m * (double)g->n * g->k;
+                L.bytes += ((double)g->m * g->k * (ta ? 0.5 : 1.0) + (double)g->k * g->n * (tb ? 0.5 : 1.0) +
+                            2.0 * g->m * (double)g->n) * ebytes;
             }
             L.prob_off = (int32_t)probs.size();
             L.list_off = (int32_t)tiles.size();
@@ -1084,13 +1087,13 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                         g.aoff = f + m + a0 * nf[s]; g.lda = (int32_t)nf[s];
                         g.boff = Li + a0 + a0 * ld; g.ldb = (int32_t)ld;
                         g.coff = toff; g.ldc = (int32_t)s2;
-                        g.m = (int32_t)s2; g.n = (int32_t)s1; g.k = (int32_t)s1; g.flags = GEMM_SET;
+                        g.m = (int32_t)s2; g.n = (int32_t)s1; g.k = (int32_t)s1; g.flags = GEMM_SET | GEMM_B_LOWER;
                         t1.push_back(g);
                         GemmProb h{};
                         h.aoff = Li + m + m * ld; h.lda = (int32_t)ld;
                         h.boff = toff; h.ldb = (int32_t)s2;
                         h.coff = Li + m + a0 * ld; h.ldc = (int32_t)ld;
-                        h.m = (int32_t)s2; h.n = (int32_t)s1; h.k = (int32_t)s2; h.flags = GEMM_SET | GEMM_ADD;
+                        h.m = (int32_t)s2; h.n = (int32_t)s1; h.k = (int32_t)s2; h.flags = GEMM_SET | GEMM_ADD | GEMM_A_LOWER;
                         t2.push_back(h);
                         toff += s2 * s1;
                         // U: T = -N inv(U2) (s1 x s2), X12 = inv(U1) T
@@ -1098,13 +1101,13 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                         u.aoff = f + a0 + m * nf[s]; u.lda = (int32_t)nf[s];
                         u.boff = Ui + m + m * ld; u.ldb = (int32_t)ld;
                         u.coff = toff; u.ldc = (int32_t)s1;
-                        u.m = (int32_t)s1; u.n = (int32_t)s2; u.k = (int32_t)s2; u.flags = GEMM_SET;
+                        u.m = (int32_t)s1; u.n = (int32_t)s2; u.k = (int32_t)s2; u.flags = GEMM_SET | GEMM_B_UPPER;
                         t1.push_back(u);
                         GemmProb v{};
                         v.aoff = Ui + a0 + a0 * ld; v.lda = (int32_t)ld;
                         v.boff = toff; v.ldb = (int32_t)s1;
                         v.coff = Ui + a0 + m * ld; v.ldc = (int32_t)ld;
-                        v.m = (int32_t)s1; v.n = (int32_t)s2; v.k = (int32_t)s1; v.flags = GEMM_SET | GEMM_ADD;
+                        v.m = (int32_t)s1; v.n = (int32_t)s2; v.k = (int32_t)s1; v.flags = GEMM_SET | GEMM_ADD | GEMM_A_UPPER;
                         t2.push_back(v);
                         toff += s1 * s2;
                     }
