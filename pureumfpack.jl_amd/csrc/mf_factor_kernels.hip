@@ -904,8 +904,9 @@ template <> struct Mfma<float> {
 // accumulated over k in the same order (k steps of 4, ascending) for every tile shape, so the
 // choice of tile never changes a result bit.
 template <typename T> struct GemmCfg {
-    static constexpr int BK = sizeof(T) == 8 ? 16 : 32;   // k per LDS slice
+    static constexpr int BK = 16;                          // k per LDS slice
     static constexpr int PAD = sizeof(T) == 8 ? 2 : 4;    // row padding: keeps 16-byte rows, staggers banks
+    static constexpr int V = 16 / sizeof(T);              // elements per 16-byte global load
 };
 // LDS elements a gemm tile needs (two buffers of the A and B slices)
 template <typename T, int TM, int TN>
@@ -919,15 +920,21 @@ __device__ __forceinline__ void lds_run(const T *p, T (&v)[F])
     for (int i = 0; i < F; i++) v[i] = p[i];
 }
 
+// Global loads are 16 bytes wide (V elements): A slices as V consecutive rows of a column, B
+// slices as V consecutive k of a column.  Fronts have any leading dimension, so these loads are
+// only element-aligned (the GPU's unaligned access mode serves them); a slice that reaches past
+// the problem's edge is loaded element by element with bounds checks instead.
 template <typename T, int TM, int TN>
 __device__ __forceinline__ void gemm_tile(const GemmProb *__restrict__ probs, const GemmTile tl, void *pool_v,
                                           int64_t pool_stride, T *lds)
 {
     typedef typename Mfma<T>::acc_t acc_t;
-    constexpr int BK = GemmCfg<T>::BK, PAD = GemmCfg<T>::PAD;
+    constexpr int BK = GemmCfg<T>::BK, PAD = GemmCfg<T>::PAD, V = GemmCfg<T>::V;
+    typedef T vec_t __attribute__((ext_vector_type(V), aligned(sizeof(T))));
     constexpr int LA = TM + PAD, LB = TN + PAD;
     constexpr int WM = TM / 2, WN = TN / 2, FM = WM / 16, FN = WN / 16;
-    constexpr int NA = TM * BK / 256, NB = TN * BK / 256;
+    constexpr int AV = TM * BK / V / 256, BV = TN * BK / V / 256;
+    static_assert(AV >= 1 && BV >= 1 && AV * V * 256 == TM * BK && BV * V * 256 == TN * BK, "tile and slice shapes");
     T *As = lds;                      // [2][BK][LA]
     T *Bs = lds + 2 * BK * LA;        // [2][BK][LB]
     const GemmProb pr = probs[tl.prob];
@@ -944,33 +951,46 @@ __device__ __forceinline__ void gemm_tile(const GemmProb *__restrict__ probs, co
     for (int i = 0; i < FM; i++)
 #pragma unroll
         for (int j = 0; j < FN; j++) acc[i][j] = acc_t{0, 0, 0, 0};
-    T ra[NA], rb[NB];
-    // A slice TM x BK: consecutive threads walk a column (coalesced); B slice BK x TN:
-    // consecutive threads walk k within a column
+    vec_t ra[AV], rb[BV];
+    const bool mfull = m0 + TM <= pr.m, nfull = n0 + TN <= pr.n;
     auto gload = [&](int k0) {
+        const bool kfull = k0 + BK <= pr.k;
 #pragma unroll
-        for (int i = 0; i < NA; i++) {
-            const int e = tid + 256 * i, r = e % TM, kk = e / TM;
-            const int gr = m0 + r, gk = k0 + kk;
-            ra[i] = (gr < pr.m && gk < pr.k) ? A[gr + (int64_t)gk * pr.lda] : T(0);
+        for (int i = 0; i < AV; i++) {
+            const int e = tid + 256 * i, rv = e % (TM / V), kk = e / (TM / V);
+            const int gr = m0 + V * rv, gk = k0 + kk;
+            const T *p = A + gr + (int64_t)gk * pr.lda;
+            if (mfull && kfull) {
+                ra[i] = *reinterpret_cast<const vec_t *>(p);
+            } else {
+#pragma unroll
+                for (int q = 0; q < V; q++) ra[i][q] = (gr + q < pr.m && gk < pr.k) ? p[q] : T(0);
+            }
         }
 #pragma unroll
-        for (int i = 0; i < NB; i++) {
-            const int e = tid + 256 * i, kk = e % BK, cc = e / BK;
-            const int gc = n0 + cc, gk = k0 + kk;
-            rb[i] = (gc < pr.n && gk < pr.k) ? B[gk + (int64_t)gc * pr.ldb] : T(0);
+        for (int i = 0; i < BV; i++) {
+            const int e = tid + 256 * i, kv = e % (BK / V), cc = e / (BK / V);
+            const int gc = n0 + cc, gk = k0 + V * kv;
+            const T *p = B + gk + (int64_t)gc * pr.ldb;
+            if (nfull && kfull) {
+                rb[i] = *reinterpret_cast<const vec_t *>(p);
+            } else {
+#pragma unroll
+                for (int q = 0; q < V; q++) rb[i][q] = (gc < pr.n && gk + q < pr.k) ? p[q] : T(0);
+            }
         }
     };
     auto sstore = [&](int buf) {
 #pragma unroll
-        for (int i = 0; i < NA; i++) {
-            const int e = tid + 256 * i;
-            As[(buf * BK + e / TM) * LA + e % TM] = ra[i];
+        for (int i = 0; i < AV; i++) {
+            const int e = tid + 256 * i, rv = e % (TM / V), kk = e / (TM / V);
+            *reinterpret_cast<vec_t *>(As + (buf * BK + kk) * LA + V * rv) = ra[i];
         }
 #pragma unroll
-        for (int i = 0; i < NB; i++) {
-            const int e = tid + 256 * i;
-            Bs[(buf * BK + e % BK) * LB + e / BK] = rb[i];
+        for (int i = 0; i < BV; i++) {
+            const int e = tid + 256 * i, kv = e % (BK / V), cc = e / (BK / V);
+#pragma unroll
+            for (int q = 0; q < V; q++) Bs[(buf * BK + V * kv + q) * LB + cc] = rb[i][q];
         }
     };
     // k range that can hold nonzero products for this tile (triangular operands); the slices
