@@ -550,9 +550,13 @@ __device__ __forceinline__ void spec_diag_steps4(T (&x)[KB / 4], int lane, int w
                                                  T &myrcp, T *lbuf)
 {
     constexpr int CW = KB / 4;
+    // the owner wave's column groups in a loop (16 steps unrolled per group): a quarter of the
+    // code of a fully unrolled factorization, which the instruction cache holds
+#pragma unroll 1
+    for (int ow = 0; ow < 4; ow++)
 #pragma unroll
-    for (int j = 0; j < KB; j++) {
-        const int ow = j / CW, jl = j % CW;
+    for (int jl = 0; jl < CW; jl++) {
+        const int j = ow * CW + jl;
         if (w == ow) {
             const T piv = lane_value(x[jl], j);
             const bool act = lane > j;
