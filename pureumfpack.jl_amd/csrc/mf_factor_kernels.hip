@@ -131,13 +131,29 @@ __global__ __launch_bounds__(256) void k_asm_cols(FactorArgs a)
     for (int q = a.orig_colptr[fc] + tid; q < a.orig_colptr[fc + 1]; q += nt)
         col[a.orig_row[q]] = scaled_value<T>(ax, rs, a.arow, a.orig_src[q]);
     __syncthreads();
-    for (int q = a.ext_colptr[fc]; q < a.ext_colptr[fc + 1]; q++) {
-        const int2 e = a.ext[q];
-        const SnDev ch = a.sn[e.x];
-        const T *src = ws + ch.cboff + (int64_t)e.y * ch.cbld;
-        const int32_t *rm = a.rmap + ch.upd_off;
-        for (int r = tid; r < ch.nu; r += nt) col[rm[r]] += src[r];
-        __syncthreads();
+    // child contributions in list order, four at a time: their records and first rows are
+    // loaded together, then added one contribution per barrier (rows of two children may meet)
+    const int q1 = a.ext_colptr[fc + 1];
+    for (int q = a.ext_colptr[fc]; q < q1; q += 4) {
+        ExtRec e[4];
+        T v[4];
+        int32_t ri[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) e[j] = q + j < q1 ? a.ext[q + j] : ExtRec{0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const bool on = tid < e[j].nu;
+            v[j] = on ? ws[e[j].src + tid] : T(0);
+            ri[j] = on ? a.rmap[e[j].roff + tid] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (q + j < q1) {   // the same for every thread
+                if (tid < e[j].nu) col[ri[j]] += v[j];
+                for (int r = tid + nt; r < e[j].nu; r += nt) col[a.rmap[e[j].roff + r]] += ws[e[j].src + r];
+                __syncthreads();
+            }
+        }
     }
     T *dst = ws + d.woff + (int64_t)c * nf;
     for (int i = tid; i < nf; i += nt) dst[i] = col[i];

@@ -64,6 +64,13 @@ struct GemmProb {
     i64 apk, bpk;   // BF16 Schur complements: first packed tile of A and of B (tile index in the pack buffer)
 };
 
+// One child contribution to one parent front column (extend-add): the child's contribution-block
+// column at src (arena offset), its nu rows mapped into the parent by rmap[roff .. roff + nu).
+struct ExtRec {
+    i64 src;
+    int32_t roff, nu;
+};
+
 // Grouped gemm tile: problem index and tile origin.
 struct GemmTile {
     int32_t prob, tm, tn, pad;

@@ -110,7 +110,7 @@ struct FactorArgs {
     const int32_t *orig_row, *orig_col, *orig_src, *orig_colptr;
     const int32_t *child_list, *rmap;
     const int32_t *ext_colptr;
-    const int2 *ext;
+    const ExtRec *ext;
     const int32_t *arow;
     const double *ax;
     const double *rs;

@@ -1304,7 +1304,14 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
     HIPCHK(upload(P.d_osrc, osrc));
     HIPCHK(upload(P.d_ocolptr, ocolptr));
     HIPCHK(upload(P.d_ecolptr, ecolptr));
-    HIPCHK(upload(P.d_ext, ext));
+    {
+        std::vector<ExtRec> er(std::max<size_t>(ext.size(), 1));
+        for (size_t q = 0; q < ext.size(); q++) {
+            const int64_t c = ext[q].x, t = ext[q].y;
+            er[q] = {cboff[c] + t * cbld[c], (int32_t)uoff0[c], (int32_t)nu[c]};
+        }
+        HIPCHK(upload(P.d_ext, er));
+    }
     HIPCHK(upload(P.d_eupd, eupd));
     HIPCHK(upload(P.d_egpos, egpos));
     HIPCHK(upload(P.d_arow, arow));
@@ -1471,7 +1478,7 @@ static int run_factor(mf_plan &P, const double *d_ax, const double *d_rs, int nm
     a.child_list = P.d_child.as<int32_t>();
     a.rmap = P.d_rmap.as<int32_t>();
     a.ext_colptr = P.d_ecolptr.as<int32_t>();
-    a.ext = P.d_ext.as<int2>();
+    a.ext = P.d_ext.as<ExtRec>();
     a.arow = P.d_arow.as<int32_t>();
     a.ax = d_ax;
     a.rs = d_rs;
