@@ -603,9 +603,9 @@ __device__ __forceinline__ void spec_diag_front(const FactorArgs &a, int k, floa
     // One block per 256-thread workgroup.  One wave: lane i owns row i of the block in
     // registers; step j reads the pivot row from lane j with v_readlane (no memory round
     // trip).  Control flow depends only on the pivot, which is uniform: rows a step must not
-    // change (finished rows, the pivot row) get multiplier 0.  A full block is first factored
-    // by four waves assuming normal pivots (spec_diag_steps4); a zero or tiny pivot, or a
-    // partial block, goes through the one-wave form with getf2's rules.  The factored block
+    // change (finished rows, the pivot row) get multiplier 0.  A block is first factored by
+    // four waves assuming normal pivots (spec_diag_steps4); a zero or tiny pivot sends it
+    // through the one-wave form with getf2's rules.  The factored block
     // goes back to the front, where the trsm reads it; this front's slot of a.spec takes the
     // reciprocal pivots and a flag for a zero or tiny pivot (which getf2 treats apart: no
     // scaling, or a division).
@@ -617,19 +617,29 @@ __device__ __forceinline__ void spec_diag_front(const FactorArgs &a, int k, floa
     T *F = reinterpret_cast<T *>(a.ws) + mat * a.ws_stride + d.woff;
     T *S = reinterpret_cast<T *>(a.spec) + mat * a.spec_stride + (int64_t)slot * SPEC_SLOT;
     const T itau = T(1) / T(tau);
-    if (kb == KB) {
+    {
+        // A partial block (kb < KB) is padded with the identity: the padded rows get zero
+        // multipliers, the padded columns of the real rows stay zero and the padded pivots stay 1,
+        // so the real entries receive exactly the updates of the one-wave form, which skips the
+        // steps j >= kb.
         constexpr int CW = KB / 4;
         T xs[CW];
 #pragma unroll
-        for (int c = 0; c < CW; c++) xs[c] = F[(k + lane) + (int64_t)(k + CW * w + c) * nf];
+        for (int c = 0; c < CW; c++) {
+            const int col = CW * w + c;
+            xs[c] = (lane < kb && col < kb) ? F[(k + lane) + (int64_t)(k + col) * nf] : T(lane == col ? 1 : 0);
+        }
         int bad = 0, special = 0;
         T myrcp = T(1);
         spec_diag_steps4<T, KB>(xs, lane, w, itau, bad, special, myrcp, lbuf);
         if (!__syncthreads_or(special)) {
             if (bad) atomicOr(a.flag + mat, 1);
+            if (lane < kb) {
 #pragma unroll
-            for (int c = 0; c < CW; c++) F[(k + lane) + (int64_t)(k + CW * w + c) * nf] = xs[c];
-            if ((lane >> 4) == w) S[lane] = myrcp;   // lane j's pivot belongs to wave j / 16
+                for (int c = 0; c < CW; c++)
+                    if (CW * w + c < kb) F[(k + lane) + (int64_t)(k + CW * w + c) * nf] = xs[c];
+            }
+            if ((lane >> 4) == w) S[lane] = myrcp;   // lane j's pivot belongs to wave j / 16 (1 when padded)
             if (threadIdx.x == 0) S[KB] = T(0);
             return;
         }
@@ -698,18 +708,103 @@ __device__ __forceinline__ T quad_lane(T v, int o)
     }
 }
 
+// LDS layout of the trsm's triangle: M[j][c] (strictly c > j, zero elsewhere) at row j, group
+// c % 4, position c / 4; a group holds 16 values and a pad of one 16-byte chunk, so that a lane's
+// four 16-byte reads of a row hit distinct banks for the four groups and stay aligned.
+template <typename T, int KB> struct TrsmLds {
+    static constexpr int GS = KB / 4 + 16 / (int)sizeof(T);   // group stride (elements)
+    static constexpr int RS = 4 * GS;                         // row stride
+    static constexpr int ELEMS = KB * RS + 3 * KB;            // M, its diagonal, reciprocals + flag
+};
+
 // Right-looking substitution on one row (kind 0) or column (kind 1) of the block, shared by a
 // group of 4 lanes: lane g of the group holds the entries c = 4 t + g (t = 0..15) in x[t].  Step
 // j: the owner lane of column j finishes x_j (kind 0: multiplier check and scaling), the group
-// takes it by a DPP broadcast, and every lane updates its entries c > j with M[j][c] from LDS.  Each x_c
-// receives its updates in the order j = 0, 1, ..., as in the row-by-row dot-product form.
-// SPECIAL: some pivot of the block is zero or below the smallest normal number, where getf2
-// leaves the column unscaled or divides instead of scaling by the reciprocal.
-template <typename T, int KB, bool LROWS, bool SPECIAL>
-__device__ __forceinline__ int spec_solve_group(T (&x)[KB / 4], const T *M, const T *rc, int g,
+// takes it by a DPP broadcast, and every lane updates its entries c > j with row j of M, read
+// from LDS one step ahead.  Each x_c receives its updates in the order j = 0, 1, ..., as in the
+// row-by-row dot-product form.  dg: the diagonal of U (kind 0), rc: the reciprocal pivots.
+// Normal pivots only (spec_solve_group_special covers the others).
+template <typename T, int KB, bool LROWS>
+__device__ __forceinline__ int spec_solve_group(T (&x)[KB / 4], const T *M, const T *dg, const T *rc, int g,
                                                 bool cand, T itau)
 {
-    constexpr int LDM = KB + 1;   // M[j][c] at M[j * LDM + c]
+    constexpr int NT = KB / 4, RS = TrsmLds<T, KB>::RS, VW = 16 / (int)sizeof(T);
+    typedef T vec_t __attribute__((ext_vector_type(VW)));
+    const T *Mg = M + g * TrsmLds<T, KB>::GS;
+    int bad = 0;
+    T cur[NT], nxt[NT];
+    T ucur = T(0), unxt = T(0), rcur = T(0), rnxt = T(0);
+#pragma unroll
+    for (int t = 0; t < NT; t += VW) {
+        const vec_t q = *reinterpret_cast<const vec_t *>(Mg + t);
+#pragma unroll
+        for (int e = 0; e < VW; e++) cur[t + e] = q[e];
+    }
+    if (LROWS) {
+        ucur = dg[0];
+        rcur = rc[0];
+    }
+#pragma unroll
+    for (int jc = 0; jc < KB / 32; jc++)
+#pragma unroll
+    for (int jj = 0; jj < 32; jj++) {
+        const int j = 32 * jc + jj, owner = j & 3, tj = j >> 2;
+        if (j + 1 < KB) {   // the next step's operands go out before this step's arithmetic
+            // entries t >= (j + 1) / 4 only (a loaded register that is never read can be
+            // reused while its load is in flight, which makes the next write wait for it)
+            const int t0 = (j + 1) >> 2, t1 = (t0 + VW - 1) / VW * VW;
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+                if (t >= t0 && t < t1) nxt[t] = Mg[(j + 1) * RS + t];
+#pragma unroll
+            for (int t = 0; t < NT; t += VW)
+                if (t >= t1) {
+                    const vec_t q = *reinterpret_cast<const vec_t *>(Mg + (j + 1) * RS + t);
+#pragma unroll
+                    for (int e = 0; e < VW; e++) nxt[t + e] = q[e];
+                }
+            if (LROWS) {
+                unxt = dg[j + 1];
+                rnxt = rc[j + 1];
+            }
+        }
+        T v = x[tj];
+        if (LROWS) {
+            const T u = ucur;
+            bad |= (int)(g == owner && cand && fabs(v) > fabs(u) * itau);
+            v = v * rcur;   // reciprocal scaling (getf2, trsm R/U/N/N)
+            if (g == owner) x[tj] = v;
+        }
+        v = quad_lane(v, owner);   // from the group's owner lane
+        // entries c = 4 t + g > j: every lane for t > tj, lanes g > owner for t = tj (a select)
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            if (t > tj) {
+                x[t] -= v * cur[t];
+            } else if (t == tj) {
+                const T y = x[t] - v * cur[t];
+                x[t] = g > owner ? y : x[t];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NT; t++) cur[t] = nxt[t];
+        ucur = unxt;
+        rcur = rnxt;
+        // keep the schedule one step deep: loads hoisted further ahead only add registers
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    return bad;
+}
+
+// Kind 0 when some pivot of the block is zero or below the smallest normal number, where getf2
+// leaves the column unscaled or divides instead of scaling by the reciprocal: the same steps as
+// spec_solve_group, its operands read at their use (a compact form for a rare case).
+template <typename T, int KB>
+__device__ __forceinline__ int spec_solve_group_special(T (&x)[KB / 4], const T *M, const T *dg, const T *rc,
+                                                        int g, bool cand, T itau)
+{
+    constexpr int RS = TrsmLds<T, KB>::RS;
+    const T *Mg = M + g * TrsmLds<T, KB>::GS;
     int bad = 0;
 #pragma unroll
     for (int jc = 0; jc < KB / 32; jc++)
@@ -717,24 +812,17 @@ __device__ __forceinline__ int spec_solve_group(T (&x)[KB / 4], const T *M, cons
     for (int jj = 0; jj < 32; jj++) {
         const int j = 32 * jc + jj, owner = j & 3, tj = j >> 2;
         T v = x[tj];
-        if (LROWS) {
-            const T u = M[j * LDM + j];
-            bad |= (int)(g == owner && cand && fabs(v) > fabs(u) * itau);
-            if (SPECIAL) {
-                if (u == T(0)) v = cand ? v : v * rc[j];
-                else if (fabs(u) < Tiny<T>::v()) v = cand ? v / u : v * rc[j];
-                else v = v * rc[j];
-            } else {
-                v = v * rc[j];   // reciprocal scaling (getf2, trsm R/U/N/N)
-            }
-            if (g == owner) x[tj] = v;
-        }
-        v = quad_lane(v, owner);   // from the group's owner lane
+        const T u = dg[j];
+        bad |= (int)(g == owner && cand && fabs(v) > fabs(u) * itau);
+        if (u == T(0)) v = cand ? v : v * rc[j];
+        else if (fabs(u) < Tiny<T>::v()) v = cand ? v / u : v * rc[j];
+        else v = v * rc[j];
+        if (g == owner) x[tj] = v;
+        v = quad_lane(v, owner);
 #pragma unroll
         for (int t = 0; t < KB / 4; t++)
             if (4 * t + 3 > j) {
-                const int c = 4 * t + g;
-                if (c > j) x[t] -= v * M[j * LDM + c];
+                if (4 * t + g > j) x[t] -= v * Mg[j * RS + t];
             }
     }
     return bad;
@@ -745,12 +833,14 @@ __global__ __launch_bounds__(4 * SPEC_ROWS) void k_spec_trsm(FactorArgs a, int k
                                                              const T *__restrict__ spec)
 {
     // SPEC_ROWS rows (kind 0) or columns (kind 1) per workgroup, 4 lanes each (see
-    // spec_solve_group), M[j][c] = U(j, c) for kind 0 and L(c, j) for kind 1 (zero outside the
-    // triangle) staged in LDS (row stride KB + 1) from the factored diagonal block in the
-    // front, read down its columns; the reciprocal pivots and the special-pivot flag come
-    // from this front's slot of a.spec.
-    constexpr int LDM = KB + 1;
-    __shared__ T sm[KB * LDM + 2 * KB];
+    // spec_solve_group), M[j][c] = U(j, c) for kind 0 and L(c, j) for kind 1 (c > j, zero
+    // elsewhere) staged in LDS (TrsmLds layout) from the factored diagonal block in the front,
+    // read down its columns; the diagonal of U, the reciprocal pivots and the special-pivot
+    // flag (from this front's slot of a.spec) follow it.
+    constexpr int GS = TrsmLds<T, KB>::GS, RS = TrsmLds<T, KB>::RS;
+    __shared__ __align__(16) T sm[TrsmLds<T, KB>::ELEMS];
+    T *const dg = sm + KB * RS;
+    T *const rc = dg + KB;
     const int mat = blockIdx.y;
     if (skip_matrix(a.only, mat)) return;
     const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
@@ -784,25 +874,32 @@ __global__ __launch_bounds__(4 * SPEC_ROWS) void k_spec_trsm(FactorArgs a, int k
             const int e = threadIdx.x + h * 4 * SPEC_ROWS, i = e % KB, jj = e / KB;   // block entry (i, jj)
             v[h] = (i < kb && jj < kb) ? Fd[i + (int64_t)jj * nf] : T(0);
         }
+        if (lrows) {
 #pragma unroll
-        for (int h = 0; h < PT; h++) {
-            const int e = threadIdx.x + h * 4 * SPEC_ROWS, i = e % KB, jj = e / KB;
-            if (lrows)
-                sm[i * LDM + jj] = jj >= i ? v[h] : T(0);   // U(i, jj)
-            else
-                sm[jj * LDM + i] = i > jj ? v[h] : T(0);    // L(i, jj)
+            for (int h = 0; h < PT; h++) {   // M[i][jj] = U(i, jj)
+                const int e = threadIdx.x + h * 4 * SPEC_ROWS, i = e % KB, jj = e / KB;
+                sm[i * RS + (jj & 3) * GS + (jj >> 2)] = jj > i ? v[h] : T(0);
+                if (jj == i) dg[i] = v[h];
+            }
+        } else {
+#pragma unroll
+            for (int h = 0; h < PT; h++) {   // M[jj][i] = L(i, jj)
+                const int e = threadIdx.x + h * 4 * SPEC_ROWS, i = e % KB, jj = e / KB;
+                sm[jj * RS + (i & 3) * GS + (i >> 2)] = i > jj ? v[h] : T(0);
+            }
         }
-        for (int e = threadIdx.x; e < 2 * KB; e += 4 * SPEC_ROWS) sm[KB * LDM + e] = S[e];
+        for (int e = threadIdx.x; e < 2 * KB; e += 4 * SPEC_ROWS) rc[e] = S[e];
     }
     __syncthreads();
-    const T *rc = sm + KB * LDM;
     int bad = 0;
-    if (!lrows)
-        spec_solve_group<T, KB, false, false>(x, sm, rc, g, false, itau);
-    else if (rc[KB] == T(0))
-        bad = spec_solve_group<T, KB, true, false>(x, sm, rc, g, cand, itau);
-    else
-        bad = spec_solve_group<T, KB, true, true>(x, sm, rc, g, cand, itau);
+    if (!lrows) {
+        spec_solve_group<T, KB, false>(x, sm, dg, rc, g, false, itau);
+    } else {
+        if (rc[KB] == T(0))
+            bad = spec_solve_group<T, KB, true>(x, sm, dg, rc, g, cand, itau);
+        else
+            bad = spec_solve_group_special<T, KB>(x, sm, dg, rc, g, cand, itau);
+    }
     if (bad) atomicOr(a.flag + mat, 1);
     if (on) {
 #pragma unroll
