@@ -555,50 +555,83 @@ __device__ __forceinline__ void spec_diag_steps(T (&x)[KB], int lane, int kb, T 
     }
 }
 
-// Four waves on a full block with normal pivots: wave w holds columns 16w .. 16w+15 of every
-// row (lane = row).  Step j: the owner wave (j / 16) takes the pivot from lane j, checks and
-// scales its multipliers, publishes them in LDS (two buffers by step parity, one barrier per
-// step) and updates its own later columns; the waves to its right update all their columns.
+// Multiplier slots of the four-wave factorization: one LDS word per step and row, EMPTY until
+// the owner wave stores the multiplier.  EMPTY is a signaling NaN, which no arithmetic result
+// is (operations quiet a signaling NaN operand), so a stored multiplier never looks empty.
+template <typename T> struct SlotBits;
+template <> struct SlotBits<float> {
+    typedef uint32_t U;
+    static constexpr U EMPTY = 0x7fa00000u;
+    static __device__ __forceinline__ U to(float v) { return __float_as_uint(v); }
+    static __device__ __forceinline__ float from(U u) { return __uint_as_float(u); }
+};
+template <> struct SlotBits<double> {
+    typedef unsigned long long U;
+    static constexpr U EMPTY = 0x7ff4000000000000ull;
+    static __device__ __forceinline__ U to(double v) { return (U)__double_as_longlong(v); }
+    static __device__ __forceinline__ double from(U u) { return __longlong_as_double((long long)u); }
+};
+
+// Four waves on a block with normal pivots: wave w holds columns 16w .. 16w+15 of every row
+// (lane = row).  The owner wave of steps 16w .. 16w+15 takes each pivot from lane j, checks and
+// scales its multipliers, stores them in the step's slots (KB x KB words in LDS, EMPTY at entry)
+// and updates its own later columns; a wave to its right applies step j as soon as its slot
+// words are filled (each lane polls its own word), so no wave waits at a barrier and the owner
+// runs ahead.  A poll that would exceed its bound (it cannot: each wave waits only on waves to
+// its left) reports a multiplier violation, which sends the matrix to the pivoted schedule.
 // Every element receives the same updates (explicit fused multiply-adds) in the same order as
 // in spec_diag_steps, so with normal pivots the results are bit-identical to the one-wave form.
 template <typename T, int KB>
 __device__ __forceinline__ void spec_diag_steps4(T (&x)[KB / 4], int lane, int w, T itau, int &bad, int &special,
-                                                 T &myrcp, T *lbuf)
+                                                 T &myrcp, typename SlotBits<T>::U *slots)
 {
+    typedef typename SlotBits<T>::U U;
+    typedef __attribute__((address_space(3))) U lds_u;
     constexpr int CW = KB / 4;
-    // the owner wave's column groups in a loop (16 steps unrolled per group): a quarter of the
-    // code of a fully unrolled factorization, which the instruction cache holds
+    lds_u *sl = (lds_u *)slots;
+    const int wu = __builtin_amdgcn_readfirstlane(w);
+    int stuck = 0;
+    // the column groups of the waves to the left, in order (16 steps unrolled per group)
 #pragma unroll 1
-    for (int ow = 0; ow < 4; ow++)
+    for (int ow = 0; ow < wu; ow++) {
 #pragma unroll
-    for (int jl = 0; jl < CW; jl++) {
-        const int j = ow * CW + jl;
-        if (w == ow) {
-            const T piv = lane_value(x[jl], j);
-            const bool act = lane > j;
-            T v = x[jl];
-            bad |= (int)(act && fabs(v) > fabs(piv) * itau);
-            const T rp = T(1) / piv;
-            special |= (int)!(fabs(piv) >= Tiny<T>::v());
-            v = v * rp;
-            myrcp = lane == j ? rp : myrcp;
-            const T l = act ? v : T(0);
-            x[jl] = act ? v : x[jl];
-            lbuf[(j & 1) * KB + lane] = l;
-#pragma unroll
-            for (int c = jl + 1; c < CW; c++) x[c] = fma(-l, lane_value(x[c], j), x[c]);
-        }
-        __syncthreads();
-        if (w > ow) {
-            const T l = lbuf[(j & 1) * KB + lane];
+        for (int jl = 0; jl < CW; jl++) {
+            const int j = ow * CW + jl;
+            U b = __atomic_load_n(sl + j * KB + lane, __ATOMIC_RELAXED);
+            for (int it = 0; b == SlotBits<T>::EMPTY && !stuck; it++) {
+                if (it > (1 << 16)) stuck = 1;
+                __builtin_amdgcn_s_sleep(1);
+                b = __atomic_load_n(sl + j * KB + lane, __ATOMIC_RELAXED);
+            }
+            const T l = stuck ? T(0) : SlotBits<T>::from(b);
 #pragma unroll
             for (int c = 0; c < CW; c++) x[c] = fma(-l, lane_value(x[c], j), x[c]);
         }
     }
+    bad |= stuck;
+    // this wave's own group
+#pragma unroll
+    for (int jl = 0; jl < CW; jl++) {
+        const int j = wu * CW + jl;
+        const T piv = lane_value(x[jl], j);
+        const bool act = lane > j;
+        T v = x[jl];
+        bad |= (int)(act && fabs(v) > fabs(piv) * itau);
+        const T rp = T(1) / piv;
+        special |= (int)!(fabs(piv) >= Tiny<T>::v());
+        v = v * rp;
+        myrcp = lane == j ? rp : myrcp;
+        const T l = act ? v : T(0);
+        x[jl] = act ? v : x[jl];
+        __atomic_store_n(sl + j * KB + lane, SlotBits<T>::to(l), __ATOMIC_RELAXED);
+#pragma unroll
+        for (int c = jl + 1; c < CW; c++) x[c] = fma(-l, lane_value(x[c], j), x[c]);
+    }
 }
 
 template <typename T, int KB>
-__device__ __forceinline__ void spec_diag_front(const FactorArgs &a, int k, float tau, int slot, T *lbuf)
+__device__ __forceinline__ void spec_diag_front(const FactorArgs &a, int k, float tau, int slot,
+                                                typename SlotBits<T>::U *slots)
 {
     // One block per 256-thread workgroup.  One wave: lane i owns row i of the block in
     // registers; step j reads the pivot row from lane j with v_readlane (no memory round
@@ -629,9 +662,11 @@ __device__ __forceinline__ void spec_diag_front(const FactorArgs &a, int k, floa
             const int col = CW * w + c;
             xs[c] = (lane < kb && col < kb) ? F[(k + lane) + (int64_t)(k + col) * nf] : T(lane == col ? 1 : 0);
         }
+        for (int e = threadIdx.x; e < KB * KB; e += 256) slots[e] = SlotBits<T>::EMPTY;
+        __syncthreads();
         int bad = 0, special = 0;
         T myrcp = T(1);
-        spec_diag_steps4<T, KB>(xs, lane, w, itau, bad, special, myrcp, lbuf);
+        spec_diag_steps4<T, KB>(xs, lane, w, itau, bad, special, myrcp, slots);
         if (!__syncthreads_or(special)) {
             if (bad) atomicOr(a.flag + mat, 1);
             if (lane < kb) {
@@ -665,9 +700,9 @@ __device__ __forceinline__ void spec_diag_front(const FactorArgs &a, int k, floa
 template <typename T, int KB>
 __global__ __launch_bounds__(256) void k_spec_diag(FactorArgs a, int k, float tau)
 {
-    __shared__ T lbuf[2 * KB];
+    __shared__ typename SlotBits<T>::U slots[KB * KB];
     if (skip_matrix(a.only, blockIdx.y)) return;
-    spec_diag_front<T, KB>(a, k, tau, blockIdx.x, lbuf);
+    spec_diag_front<T, KB>(a, k, tau, blockIdx.x, slots);
 }
 
 
@@ -1397,7 +1432,7 @@ __global__ __launch_bounds__(256) void k_store(FactorArgs a)
 // the bulk of block k-1's trailing update (64 x 64 gemm tiles the next panel does not read).
 // One LDS area serves either role.
 template <typename T, int KB>
-__global__ __launch_bounds__(256) void k_spec_diag_gemm(FactorArgs a, int k, float tau,
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 2 : 4) void k_spec_diag_gemm(FactorArgs a, int k, float tau,
                                                         const GemmProb *__restrict__ probs,
                                                         const GemmTile *__restrict__ tiles, int ndiag, int dprob)
 {
@@ -1412,7 +1447,8 @@ __global__ __launch_bounds__(256) void k_spec_diag_gemm(FactorArgs a, int k, flo
         gemm_tile<T, 64, 64>(probs, GemmTile{dprob + (int)blockIdx.x, 0, 0, 0}, a.ws, a.ws_stride, lds);
         __syncthreads();
     }
-    spec_diag_front<T, KB>(a, k, tau, (int)blockIdx.x, lds);
+    static_assert(gemm_lds_elems<T, 64, 64>() >= KB * KB, "the multiplier slots fit in the tile's LDS");
+    spec_diag_front<T, KB>(a, k, tau, (int)blockIdx.x, reinterpret_cast<typename SlotBits<T>::U *>(lds));
 }
 
 template <typename T>
