@@ -150,7 +150,21 @@ __global__ __launch_bounds__(256) void k_asm_cols(FactorArgs a)
         for (int j = 0; j < 4; j++) {
             if (q + j < q1) {   // the same for every thread
                 if (tid < e[j].nu) col[ri[j]] += v[j];
-                for (int r = tid + nt; r < e[j].nu; r += nt) col[a.rmap[e[j].roff + r]] += ws[e[j].src + r];
+                // the remaining rows eight per thread at a time: all loads before the adds
+                for (int r0 = tid + nt; r0 < e[j].nu; r0 += 8 * nt) {
+                    T vv[8];
+                    int32_t rr[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int r = r0 + u * nt;
+                        const bool on = r < e[j].nu;
+                        vv[u] = on ? ws[e[j].src + r] : T(0);
+                        rr[u] = on ? a.rmap[e[j].roff + r] : -1;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; u++)
+                        if (rr[u] >= 0) col[rr[u]] += vv[u];
+                }
                 __syncthreads();
             }
         }
