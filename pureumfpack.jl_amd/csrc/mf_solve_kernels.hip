@@ -357,6 +357,17 @@ __global__ __launch_bounds__(256) void k_rect_sum(SolveArgs a, int kind)
     for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
         const T *pp = part + (int64_t)r0 * 256 + rr;
+        // the values the sums are applied to (kinds 0 and 1) do not depend on the partial sums:
+        // they are read first
+        T base[SOLVE_RC];
+#pragma unroll
+        for (int r = 0; r < SOLVE_RC; r++) {
+            base[r] = T(0);
+            if (g == 0 && on && r < rc) {
+                if (kind == 0) base[r] = uv[(int64_t)(r0 + r) * a.nupd + dst];
+                else if (kind == 1) base[r] = y[(int64_t)(r0 + r) * a.n + dst];
+            }
+        }
         T s[SOLVE_RC] = {};
         if (on) {
             int c = c0;
@@ -391,8 +402,8 @@ __global__ __launch_bounds__(256) void k_rect_sum(SolveArgs a, int kind)
                               red[(3 * SOLVE_RC + r) * 64 + l];
                 const int64_t ro = (int64_t)(r0 + r) * a.n + dst;
                 switch (kind) {
-                case 0: uv[(int64_t)(r0 + r) * a.nupd + dst] -= sum; break;
-                case 1: (to_zb ? zb : x)[ro] = y[ro] - sum; break;   // inv(U11) next
+                case 0: uv[(int64_t)(r0 + r) * a.nupd + dst] = base[r] - sum; break;
+                case 1: (to_zb ? zb : x)[ro] = base[r] - sum; break;   // inv(U11) next
                 case 2: y[ro] = sum; break;
                 default: x[ro] = sum; break;
                 }
