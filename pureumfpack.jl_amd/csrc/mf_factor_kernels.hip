@@ -1168,6 +1168,22 @@ __device__ __forceinline__ void gemm_tile(const GemmProb *__restrict__ probs, co
     if (pr.flags & GEMM_B_UPPER) kend = min(kend, n0 + TN);
     kbeg = kbeg / BK * BK;
     const int nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+    // C of a small tile is read before the k loop (its loads overlap the slices), not after it
+    constexpr bool CPRE = FM * FN <= 4;
+    const bool add = pr.flags & GEMM_ADD, set = pr.flags & GEMM_SET;
+    const int gr0 = m0 + wm + FM * slot;
+    T cpre[CPRE ? FN : 1][4][CPRE ? FM : 1];
+    if (CPRE && !set) {
+#pragma unroll
+        for (int j = 0; j < FN; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int gc = n0 + wn + FN * Mfma<T>::row(lane, q) + j;
+                const T *cp = C + (int64_t)gc * pr.ldc;
+#pragma unroll
+                for (int i = 0; i < FM; i++) cpre[j][q][i] = (gc < pr.n && gr0 + i < pr.m) ? cp[gr0 + i] : T(0);
+            }
+    }
     if (nk > 0) {
         gload(kbeg);
         sstore(0);
@@ -1192,8 +1208,6 @@ __device__ __forceinline__ void gemm_tile(const GemmProb *__restrict__ probs, co
         __syncthreads();
     }
     // acc[i][j][q] = C[wm + FM (lane % 16) + i][wn + FN row(lane, q) + j]
-    const bool add = pr.flags & GEMM_ADD, set = pr.flags & GEMM_SET;
-    const int gr0 = m0 + wm + FM * slot;
 #pragma unroll
     for (int j = 0; j < FN; j++)
 #pragma unroll
@@ -1206,7 +1220,8 @@ __device__ __forceinline__ void gemm_tile(const GemmProb *__restrict__ probs, co
                 const int gr = gr0 + i;
                 if (gr < pr.m) {
                     const T v = add ? acc[i][j][q] : -acc[i][j][q];
-                    cp[gr] = set ? v : cp[gr] + v;
+                    const T c0 = CPRE ? cpre[CPRE ? j : 0][q][CPRE ? i : 0] : cp[gr];
+                    cp[gr] = set ? v : c0 + v;
                 }
             }
         }
