@@ -1090,7 +1090,7 @@ __device__ __forceinline__ void lds_run(const T *p, T (&v)[F])
 // slices as V consecutive k of a column.  Fronts have any leading dimension, so these loads are
 // only element-aligned (the GPU's unaligned access mode serves them); a slice that reaches past
 // the problem's edge is loaded element by element with bounds checks instead.
-template <typename T, int TM, int TN>
+template <typename T, int TM, int TN, int PF = 1>
 __device__ __forceinline__ void gemm_tile(const GemmProb *__restrict__ probs, const GemmTile tl, void *pool_v,
                                           int64_t pool_stride, T *lds)
 {
@@ -1117,9 +1117,9 @@ __device__ __forceinline__ void gemm_tile(const GemmProb *__restrict__ probs, co
     for (int i = 0; i < FM; i++)
 #pragma unroll
         for (int j = 0; j < FN; j++) acc[i][j] = acc_t{0, 0, 0, 0};
-    vec_t ra[AV], rb[BV];
+    vec_t ra[PF][AV], rb[PF][BV];
     const bool mfull = m0 + TM <= pr.m, nfull = n0 + TN <= pr.n;
-    auto gload = [&](int k0) {
+    auto gload = [&](int k0, vec_t (&ra)[AV], vec_t (&rb)[BV]) {
         const bool kfull = k0 + BK <= pr.k;
 #pragma unroll
         for (int i = 0; i < AV; i++) {
@@ -1146,7 +1146,7 @@ __device__ __forceinline__ void gemm_tile(const GemmProb *__restrict__ probs, co
             }
         }
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&](int buf, const vec_t (&ra)[AV], const vec_t (&rb)[BV]) {
 #pragma unroll
         for (int i = 0; i < AV; i++) {
             const int e = tid + 256 * i, rv = e % (TM / V), kk = e / (TM / V);
@@ -1184,28 +1184,37 @@ __device__ __forceinline__ void gemm_tile(const GemmProb *__restrict__ probs, co
                 for (int i = 0; i < FM; i++) cpre[j][q][i] = (gc < pr.n && gr0 + i < pr.m) ? cp[gr0 + i] : T(0);
             }
     }
-    if (nk > 0) {
-        gload(kbeg);
-        sstore(0);
-    }
+    // PF slices are in flight in registers: slice s + PF is requested while slice s is multiplied,
+    // so each load has PF slice times to arrive before it is stored to LDS
+#pragma unroll
+    for (int u = 0; u < PF; u++)
+        if (u < nk) gload(kbeg + u * BK, ra[u], rb[u]);
+    if (nk > 0) sstore(0, ra[0], rb[0]);
     __syncthreads();
-    for (int s = 0; s < nk; s++) {
-        const int cur = s & 1;
-        if (s + 1 < nk) gload(kbeg + (s + 1) * BK);
-        const T *Ac = As + (cur * BK + kl) * LA + wm + FM * slot;
-        const T *Bc = Bs + (cur * BK + kl) * LB + wn + FN * slot;
+    for (int s0 = 0; s0 < nk; s0 += PF) {
 #pragma unroll
-        for (int kq = 0; kq < BK; kq += 4) {
-            T av[FM], bv[FN];
-            lds_run<T, FM>(Ac + kq * LA, av);
-            lds_run<T, FN>(Bc + kq * LB, bv);
+        for (int u = 0; u < PF; u++) {
+            const int s = s0 + u;
+            if (s < nk) {
+                const int cur = s & 1;
+                // register set u held slice s, already in LDS: reuse it for slice s + PF
+                if (s + PF < nk) gload(kbeg + (s + PF) * BK, ra[u], rb[u]);
+                const T *Ac = As + (cur * BK + kl) * LA + wm + FM * slot;
+                const T *Bc = Bs + (cur * BK + kl) * LB + wn + FN * slot;
 #pragma unroll
-            for (int i = 0; i < FM; i++)
+                for (int kq = 0; kq < BK; kq += 4) {
+                    T av[FM], bv[FN];
+                    lds_run<T, FM>(Ac + kq * LA, av);
+                    lds_run<T, FN>(Bc + kq * LB, bv);
 #pragma unroll
-                for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::mma(bv[j], av[i], acc[i][j]);
+                    for (int i = 0; i < FM; i++)
+#pragma unroll
+                        for (int j = 0; j < FN; j++) acc[i][j] = Mfma<T>::mma(bv[j], av[i], acc[i][j]);
+                }
+                if (s + 1 < nk) sstore(cur ^ 1, ra[(u + 1) % PF], rb[(u + 1) % PF]);
+                __syncthreads();
+            }
         }
-        if (s + 1 < nk) sstore(cur ^ 1);
-        __syncthreads();
     }
     // acc[i][j][q] = C[wm + FM (lane % 16) + i][wn + FN row(lane, q) + j]
 #pragma unroll
@@ -1227,6 +1236,19 @@ __device__ __forceinline__ void gemm_tile(const GemmProb *__restrict__ probs, co
         }
 }
 
+// slices in flight per tile shape: the 64 x 64 FP32 tile runs few MFMAs per slice, so its loads
+// get three slice times to arrive (measured on the Schur launches of 3D Poisson 48^3: 2.5% less
+// time than one slice; the FP64 tile, with twice the load registers per slice, keeps one)
+#ifndef MF_GEMM_PF
+#define MF_GEMM_PF 3
+#endif
+#ifndef MF_GEMM_PF64
+#define MF_GEMM_PF64 1
+#endif
+template <typename T, int TM, int TN> struct GemmPf {
+    static constexpr int value = TM * TN <= 64 * 64 ? (sizeof(T) == 4 ? MF_GEMM_PF : MF_GEMM_PF64) : 1;
+};
+
 template <typename T, int TM, int TN, int USE>
 __global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs,
                                               const GemmTile *__restrict__ tiles, void *pool_v,
@@ -1234,7 +1256,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmProb *__restrict__ probs
 {
     __shared__ __align__(16) T lds[gemm_lds_elems<T, TM, TN>()];
     if (skip_matrix(only, blockIdx.y)) return;
-    gemm_tile<T, TM, TN>(probs, tiles[blockIdx.x], pool_v, pool_stride, lds);
+    gemm_tile<T, TM, TN, GemmPf<T, TM, TN>::value>(probs, tiles[blockIdx.x], pool_v, pool_stride, lds);
 }
 
 // ------------------------------------------------------------------------------------------

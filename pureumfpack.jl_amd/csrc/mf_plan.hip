@@ -55,6 +55,43 @@ static void xcd_order(std::vector<GemmTile> &t, size_t begin)
     for (size_t p = 0; p < per * NX; p++) t[begin + p] = o[(p % NX) * per + p / NX];
 }
 
+// Launches whose problems differ in k (the Schur complements of a level's fronts, the pivot-block
+// inversions) would put all of one front's heavy tiles on one or two XCDs under xcd_order.  Here
+// every XCD receives a contiguous eighth of every problem's tiles (so each XCD carries an eighth
+// of each problem's work, and its tiles of one problem still share rows of A or columns of B),
+// and runs the problems in order of decreasing k, heaviest first.  Tiles [begin, end) must be
+// grouped by problem; kw[prob] is the k of problem `prob` (the index stored in the tile).
+static void xcd_order_by_problem(std::vector<GemmTile> &t, size_t begin, const std::vector<int64_t> &kw)
+{
+    constexpr size_t NX = 8;
+    const size_t cnt = t.size() - begin;
+    if (cnt == 0) return;
+    std::vector<std::pair<size_t, size_t>> runs;   // [first, last) of each problem's tiles
+    for (size_t i = begin; i < t.size(); i++)
+        if (i == begin || t[i].prob != t[i - 1].prob) runs.push_back({i, i + 1});
+        else runs.back().second = i + 1;
+    std::stable_sort(runs.begin(), runs.end(), [&](const std::pair<size_t, size_t> &a, const std::pair<size_t, size_t> &b) {
+        return kw[t[a.first].prob] > kw[t[b.first].prob];
+    });
+    std::vector<GemmTile> q[NX];
+    size_t c = 0;   // next XCD to receive an extra tile: keeps the queue lengths within one
+    for (const auto &r : runs) {
+        const size_t T = r.second - r.first, base = T / NX, rem = T % NX;
+        size_t pos = r.first;
+        for (size_t j = 0; j < NX; j++) {
+            const size_t take = base + (j < rem ? 1 : 0);
+            std::vector<GemmTile> &dst = q[(c + j) % NX];
+            dst.insert(dst.end(), t.begin() + pos, t.begin() + pos + take);
+            pos += take;
+        }
+        c = (c + rem) % NX;
+    }
+    // workgroup i runs on XCD i % 8: queue x has exactly ceil((cnt - x) / 8) tiles
+    std::vector<GemmTile> o(cnt);
+    for (size_t i = 0; i < cnt; i++) o[i] = q[i % NX][i / NX];
+    std::copy(o.begin(), o.end(), t.begin() + begin);
+}
+
 static int fail(int code, const std::string &msg)
 {
     g_err = msg;
@@ -745,7 +782,11 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             Pk.count = (int32_t)(items.size() / 4);
             Pk.level = L.level = lvl;
             L.count = (int32_t)(tiles.size() - L.list_off);
-            xcd_order(tiles, L.list_off);
+            {
+                std::vector<int64_t> kw;
+                for (const GemmProb &g : pv) kw.push_back(g.k);
+                xcd_order_by_problem(tiles, L.list_off, kw);
+            }
             gemm_dst->push_back(Pk);
             gemm_dst->push_back(L);
             return;
@@ -777,7 +818,11 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             }
             L.count = (int32_t)(tiles.size() - L.list_off);
             L.level = lvl;
-            xcd_order(tiles, L.list_off);
+            {
+                std::vector<int64_t> kw;
+                for (const GemmProb *g : sel) kw.push_back(g->k);
+                xcd_order_by_problem(tiles, L.list_off, kw);
+            }
             if (L.count > 0) gemm_dst->push_back(L);
         }
     };
@@ -2271,7 +2316,11 @@ extern "C" int mf_time_gemm_list(int32_t precision, int64_t nprob, const int32_t
             for (int64_t tm = 0; tm < m; tm += GEMM_TILE_M[tile]) tv.push_back({(int32_t)q, (int32_t)tm, (int32_t)tn, 0});
         pv.push_back(g);
     }
-    xcd_order(tv, 0);
+    {
+        std::vector<int64_t> kw;
+        for (const GemmProb &g : pv) kw.push_back(g.k);
+        xcd_order_by_problem(tv, 0, kw);
+    }
     DevBuf pool, dprobs, dtiles;
     HIPCHK(pool.alloc((size_t)off * esz));
     HIPCHK(hipMemset(pool.p, 0, (size_t)off * esz));
