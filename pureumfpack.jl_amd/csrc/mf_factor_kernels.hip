@@ -1236,14 +1236,15 @@ __device__ __forceinline__ void gemm_tile(const GemmProb *__restrict__ probs, co
         }
 }
 
-// slices in flight per tile shape: the 64 x 64 FP32 tile runs few MFMAs per slice, so its loads
-// get three slice times to arrive (measured on the Schur launches of 3D Poisson 48^3: 2.5% less
-// time than one slice; the FP64 tile, with twice the load registers per slice, keeps one)
+// slices in flight per tile shape: the 64 x 64 tile runs few MFMAs per slice, so its loads get
+// more than one slice time to arrive (measured on the Schur launches of 3D Poisson 48^3: three
+// slices in FP32, 2.5% less time than one; two in FP64, whose slices hold twice the registers,
+// 4% less)
 #ifndef MF_GEMM_PF
 #define MF_GEMM_PF 3
 #endif
 #ifndef MF_GEMM_PF64
-#define MF_GEMM_PF64 1
+#define MF_GEMM_PF64 2
 #endif
 template <typename T, int TM, int TN> struct GemmPf {
     static constexpr int value = TM * TN <= 64 * 64 ? (sizeof(T) == 4 ? MF_GEMM_PF : MF_GEMM_PF64) : 1;
