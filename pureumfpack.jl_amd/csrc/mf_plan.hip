@@ -16,6 +16,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <tuple>
 #include <new>
 #include <string>
 #include <vector>
@@ -60,8 +61,12 @@ static void xcd_order(std::vector<GemmTile> &t, size_t begin)
 // every XCD receives a contiguous eighth of every problem's tiles (so each XCD carries an eighth
 // of each problem's work, and its tiles of one problem still share rows of A or columns of B),
 // and runs the problems in order of decreasing k, heaviest first.  Tiles [begin, end) must be
-// grouped by problem; kw[prob] is the k of problem `prob` (the index stored in the tile).
-static void xcd_order_by_problem(std::vector<GemmTile> &t, size_t begin, const std::vector<int64_t> &kw)
+// grouped by problem; kw[prob] is the k of problem `prob` (the index stored in the tile).  A
+// problem with bcyc[prob] set (a triangular B: a tile's k range grows or shrinks with its column
+// block) has its column blocks dealt out cyclically, so that each eighth holds column blocks
+// from the whole width.
+static void xcd_order_by_problem(std::vector<GemmTile> &t, size_t begin, const std::vector<int64_t> &kw,
+                                 const std::vector<char> &bcyc = {})
 {
     constexpr size_t NX = 8;
     const size_t cnt = t.size() - begin;
@@ -70,6 +75,19 @@ static void xcd_order_by_problem(std::vector<GemmTile> &t, size_t begin, const s
     for (size_t i = begin; i < t.size(); i++)
         if (i == begin || t[i].prob != t[i - 1].prob) runs.push_back({i, i + 1});
         else runs.back().second = i + 1;
+    for (const auto &r : runs) {
+        if ((size_t)t[r.first].prob >= bcyc.size() || !bcyc[t[r.first].prob]) continue;
+        std::vector<int32_t> cols;   // distinct column-block origins, ascending
+        for (size_t i = r.first; i < r.second; i++) cols.push_back(t[i].tn);
+        std::sort(cols.begin(), cols.end());
+        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+        auto key = [&](const GemmTile &x) {
+            const int64_t cb = std::lower_bound(cols.begin(), cols.end(), x.tn) - cols.begin();
+            return std::make_tuple(cb % (int64_t)NX, cb / (int64_t)NX, x.tm);
+        };
+        std::stable_sort(t.begin() + r.first, t.begin() + r.second,
+                         [&](const GemmTile &a, const GemmTile &b) { return key(a) < key(b); });
+    }
     std::stable_sort(runs.begin(), runs.end(), [&](const std::pair<size_t, size_t> &a, const std::pair<size_t, size_t> &b) {
         return kw[t[a.first].prob] > kw[t[b.first].prob];
     });
@@ -820,8 +838,12 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             L.level = lvl;
             {
                 std::vector<int64_t> kw;
-                for (const GemmProb *g : sel) kw.push_back(g->k);
-                xcd_order_by_problem(tiles, L.list_off, kw);
+                std::vector<char> bcyc;
+                for (const GemmProb *g : sel) {
+                    kw.push_back(g->k);
+                    bcyc.push_back((g->flags & (GEMM_B_LOWER | GEMM_B_UPPER)) ? 1 : 0);
+                }
+                xcd_order_by_problem(tiles, L.list_off, kw, bcyc);
             }
             if (L.count > 0) gemm_dst->push_back(L);
         }
