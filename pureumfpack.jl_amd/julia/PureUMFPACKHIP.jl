@@ -2,10 +2,11 @@
 # multifrontal entry points.  Julia keeps ordering, scaling and symbolic analysis
 # (PureUMFPACK.symbolic_mf, row_scaling); every numeric step runs on the GPU.
 #
-# Drop-in: `hip_splu(A; scale, check)` returns a `PureLU` like
-# `splu(A; method=:multifrontal)` (src/interface.jl:93-120); `solve(F, b; refine)` and
-# `F \ b` keep their semantics (src/interface.jl:170-179).  Only Float64 values with Int64
-# indices are routed here; other element types stay on the reference path.
+# Drop-in: `hip_splu(A; scale, check)` returns a `HIPLU` with the fields of the `PureLU` of
+# `splu(A; method=:multifrontal)` (src/interface.jl:93-120), L and U exported on first
+# access; `solve(F, b; refine)` and `F \ b` keep their semantics (src/interface.jl:170-179).
+# Only Float64 values with Int64 indices are routed here; other element types stay on the
+# reference path.
 module PureUMFPACKHIP
 
 using PureUMFPACK, SparseArrays, LinearAlgebra
@@ -106,9 +107,41 @@ function export_factors(P::MFPlan)
 end
 
 """
-    hip_splu(A; scale=SCALE_SUM, check=true, precision=0) -> (PureLU, MFPlan)
+    HIPLU
 
-GPU numeric phase behind `splu(A; method=:multifrontal)` (src/interface.jl:93-105).
+The factorization `hip_splu` returns.  The factors stay on the device; the fields of the
+reference's `PureLU` (src/interface.jl:38-48) are there on demand: `F.L`, `F.U`, `F.p`, `F.q`
+copy L, U, p and q to the host on first access (`mf_export_factors`, once), `F.Rs` and `F.A`
+are held.  `F \ b` and `hip_solve(F.plan, b)` use the device factors without an export.
+"""
+mutable struct HIPLU
+    plan::MFPlan
+    Rs::Vector{Float64}
+    A::SparseMatrixCSC{Float64, Int}
+    lu::Union{Nothing, PureLU}
+end
+
+function materialize!(F::HIPLU)
+    if getfield(F, :lu) === nothing
+        L, U, p, q = export_factors(getfield(F, :plan))
+        setfield!(F, :lu, PureLU(L, U, p, q, getfield(F, :Rs), getfield(F, :A)))
+    end
+    return getfield(F, :lu)
+end
+
+function Base.getproperty(F::HIPLU, s::Symbol)
+    s in (:L, :U, :p, :q) && return getproperty(materialize!(F), s)
+    return getfield(F, s)
+end
+Base.size(F::HIPLU, args...) = size(getfield(F, :A), args...)
+Base.:\(F::HIPLU, b::AbstractVecOrMat{Float64}) = hip_solve(getfield(F, :plan), b)
+PureUMFPACK.PureLU(F::HIPLU) = materialize!(F)
+
+"""
+    hip_splu(A; scale=SCALE_SUM, check=true, precision=0) -> (HIPLU, MFPlan)
+
+GPU numeric phase behind `splu(A; method=:multifrontal)` (src/interface.jl:93-105).  L and U
+are exported only when a caller reads them (`HIPLU`).
 """
 function hip_splu(A::SparseMatrixCSC{Float64, Int}; scale::ScaleKind = SCALE_SUM, check = true,
                   precision = 0)
@@ -118,8 +151,7 @@ function hip_splu(A::SparseMatrixCSC{Float64, Int}; scale::ScaleKind = SCALE_SUM
     S = symbolic_mf(A)
     P = MFPlan(A, S; opts = MFOptions(precision = precision))
     factor!(P, A, scale == SCALE_NONE ? nothing : Rs; check_singular = check)
-    L, U, p, q = export_factors(P)
-    return PureLU(L, U, p, q, Rs, A), P
+    return HIPLU(P, Rs, A, nothing), P
 end
 
 """
