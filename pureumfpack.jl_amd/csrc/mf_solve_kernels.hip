@@ -246,10 +246,11 @@ __global__ __launch_bounds__(256) void k_fwd_tri(SolveArgs a)
 // k_rect_part items are int4 {front, row0, col0, slot}: each workgroup writes its partial sums
 // (256 rows x nrhs) to partial slot `slot`.  k_rect_sum items are {dst, rows | zb << 16,
 // nchunks, slot0}: the row block's chunks own slots slot0 .. slot0 + nchunks - 1, added in a
-// fixed order (four in-order quarters, then the quarters in order: the result does not depend
-// on scheduling) and applied to `rows` entries from dst (c1 + row0, or upd_off + row0 for
-// kind 0; zb: kind 1 writes zbuf).
+// fixed order (sixteen interleaved partial sums, each in chunk order, then the sixteen in
+// order: the result does not depend on scheduling) and applied to `rows` entries from dst
+// (c1 + row0, or upd_off + row0 for kind 0; zb: kind 1 writes zbuf).
 constexpr int SOLVE_CK = 64;
+constexpr int RECT_SUM_TPR = 16;   // threads per row of a split-K sum (and its workgroups per item)
 
 // acc[r] += sum_{j < bs} M[j * ld] * s[r * sld + j], G loads in flight
 template <typename T, int G>
@@ -335,17 +336,19 @@ __global__ __launch_bounds__(256) void k_rect_part(SolveArgs a, int kind)
 template <typename T>
 __global__ __launch_bounds__(256) void k_rect_sum(SolveArgs a, int kind)
 {
-    // four workgroups per item (64 rows each); four threads per row, thread quarter g adding
-    // chunks [g nch / 4, (g + 1) nch / 4) in order; the quarters are added in order in LDS
-    __shared__ T red[4 * SOLVE_RC * 64];
+    // sixteen workgroups per item (16 rows each); sixteen threads per row, thread g adding the
+    // chunks c = g, g + 16, g + 32, ... in order, all of a thread's loads in flight together for
+    // up to 64 chunks; the sixteen partial sums of a row are added in order in LDS
+    constexpr int TPR = RECT_SUM_TPR, RPW = 256 / TPR, U = 4;
+    __shared__ T red[TPR * SOLVE_RC * RPW];
     const int mat = blockIdx.y;
-    const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x >> 2];
-    const int g = threadIdx.x & 3, rr = ((int)blockIdx.x & 3) * 64 + (threadIdx.x >> 2);
+    const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x / TPR];
+    const int lr = threadIdx.x % RPW, g = threadIdx.x / RPW;
+    const int rr = ((int)blockIdx.x % TPR) * RPW + lr;
     const bool on = rr < (w.y & 0xffff);
     const int64_t dst = (int64_t)w.x + rr;
     const bool to_zb = (w.y >> 16) != 0;
     const int nch = w.z;
-    const int c0 = g * nch / 4, c1 = (g + 1) * nch / 4;
     const int64_t vb = (int64_t)mat * a.nrhs;
     T *y = reinterpret_cast<T *>(a.y) + vb * a.n;
     T *x = reinterpret_cast<T *>(a.x) + vb * a.n;
@@ -370,36 +373,30 @@ __global__ __launch_bounds__(256) void k_rect_sum(SolveArgs a, int kind)
         }
         T s[SOLVE_RC] = {};
         if (on) {
-            int c = c0;
-            for (; c + 4 <= c1; c += 4) {
-                T v[4][SOLVE_RC];
+            for (int c = g; c < nch; c += U * TPR) {
+                T v[U][SOLVE_RC];
 #pragma unroll
-                for (int q = 0; q < 4; q++)
+                for (int q = 0; q < U; q++)
 #pragma unroll
-                    for (int r = 0; r < SOLVE_RC; r++) v[q][r] = r < rc ? pp[(c + q) * cs + r * 256] : T(0);
+                    for (int r = 0; r < SOLVE_RC; r++)
+                        v[q][r] = (c + q * TPR < nch && r < rc) ? pp[(int64_t)(c + q * TPR) * cs + r * 256] : T(0);
 #pragma unroll
-                for (int q = 0; q < 4; q++)
+                for (int q = 0; q < U; q++)
 #pragma unroll
                     for (int r = 0; r < SOLVE_RC; r++) s[r] += v[q][r];
-            }
-            for (; c < c1; c++) {
-#pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++)
-                    if (r < rc) s[r] += pp[c * cs + r * 256];
             }
         }
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < SOLVE_RC; r++) red[(g * SOLVE_RC + r) * 64 + (threadIdx.x >> 2)] = s[r];
+        for (int r = 0; r < SOLVE_RC; r++) red[(g * SOLVE_RC + r) * RPW + lr] = s[r];
         __syncthreads();
         if (g == 0 && on) {
-            const int l = threadIdx.x >> 2;
 #pragma unroll
             for (int r = 0; r < SOLVE_RC; r++) {
                 if (r >= rc) continue;
-                const T sum = ((red[(0 * SOLVE_RC + r) * 64 + l] + red[(1 * SOLVE_RC + r) * 64 + l]) +
-                               red[(2 * SOLVE_RC + r) * 64 + l]) +
-                              red[(3 * SOLVE_RC + r) * 64 + l];
+                T sum = red[r * RPW + lr];
+#pragma unroll
+                for (int q = 1; q < TPR; q++) sum += red[(q * SOLVE_RC + r) * RPW + lr];
                 const int64_t ro = (int64_t)(r0 + r) * a.n + dst;
                 switch (kind) {
                 case 0: uv[(int64_t)(r0 + r) * a.nupd + dst] = base[r] - sum; break;
@@ -1636,8 +1633,8 @@ hipError_t launch_solve_level(int which, const SolveArgs &a, const int32_t *list
     b.list = lists + list_off;
     // right-hand-side chunks of SOLVE_RC in grid z
     const int nz = (int)((a.nrhs + SOLVE_RC - 1) / SOLVE_RC);
-    const bool sum = which == 2 || which == 4 || which == 7 || which == 9;   // four workgroups per item
-    dim3 g(sum ? 4 * count : count, nmat, nz), blk(256);
+    const bool sum = which == 2 || which == 4 || which == 7 || which == 9;   // RECT_SUM_TPR workgroups per item
+    dim3 g(sum ? RECT_SUM_TPR * count : count, nmat, nz), blk(256);
     switch (which) {
     case 0: hipLaunchKernelGGL(k_fwd_tri<T>, g, blk, 0, st, b); break;
     case 1: hipLaunchKernelGGL(k_rect_part<T>, g, blk, 0, st, b, 0); break;
