@@ -243,8 +243,8 @@ __global__ __launch_bounds__(256) void k_fwd_tri(SolveArgs a)
 //           (fronts with inverses: into zbuf, inv(U11) follows)
 //   kind 2: y[c1+i] = sum_a inv(L11)[i, a] w0[prow[c1 + a]]     lower-triangle chunks only
 //   kind 3: x[c1+i] = sum_a inv(U11)[i, a] zbuf[c1 + a]         upper-triangle chunks only
-// k_rect_part items are int4 {front, row0, col0, slot}: each workgroup writes its partial sums
-// (256 rows x nrhs) to partial slot `slot`.  k_rect_sum items are {dst, rows | zb << 16,
+// k_rect_part items are int4 {front, row0, col0, slot}: two workgroups per item (128 rows
+// each) write its partial sums (256 rows x nrhs) to partial slot `slot`.  k_rect_sum items are {dst, rows | zb << 16,
 // nchunks, slot0}: the row block's chunks own slots slot0 .. slot0 + nchunks - 1, added in a
 // fixed order (sixteen interleaved partial sums, each in chunk order, then the sixteen in
 // order: the result does not depend on scheduling) and applied to `rows` entries from dst
@@ -276,17 +276,23 @@ __device__ __forceinline__ void dot_rows(const T *M, int64_t ld, int bs, const T
 template <typename T>
 __global__ __launch_bounds__(256) void k_rect_part(SolveArgs a, int kind)
 {
+    // two workgroups per item (128 rows each); two threads per row, each with half of the chunk's
+    // columns, loaded before the vector is staged; the two halves added in order in LDS
+    constexpr int HC = SOLVE_CK / 2;
     __shared__ T sv[SOLVE_CK * SOLVE_RC];
+    __shared__ T red[SOLVE_RC * 128];
     const int mat = blockIdx.y;
-    const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
+    const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x >> 1];
     const SnDev d = a.sn[w.x];
     const int nf = d.nf, np = d.np, nu = d.nu, c1 = d.c1;
     const T *F = reinterpret_cast<const T *>(a.store) + mat * a.store_stride + d.foff;
-    const int tid = threadIdx.x;
-    const int row = w.y + tid;
+    const int tid = threadIdx.x, lr = tid & 127, hf = tid >> 7;
+    const int ir = ((int)blockIdx.x & 1) * 128 + lr;   // row within the item's 256
+    const int row = w.y + ir;
     const int nrows = kind == 0 ? nu : np, ncols = kind == 1 ? nu : np;
     const int col0 = w.z;
     const int bs = max(0, min(SOLVE_CK, ncols - col0));
+    const int hb = max(0, min(HC, bs - hf * HC));   // this thread's columns
     const int64_t vb = (int64_t)mat * a.nrhs;
     const T *y = reinterpret_cast<const T *>(a.y) + vb * a.n;
     const T *x = reinterpret_cast<const T *>(a.x) + vb * a.n;
@@ -307,6 +313,13 @@ __global__ __launch_bounds__(256) void k_rect_part(SolveArgs a, int kind)
     default: rowp = store + d.ioff + (int64_t)np * np + row; break;
     }
     const int nrhs = (int)a.nrhs;
+    T l[HC];
+    {
+        const T *mrow = rowp + (int64_t)(col0 + hf * HC) * ld;
+        const bool on = row < nrows;
+#pragma unroll
+        for (int q = 0; q < HC; q++) l[q] = (on && q < hb) ? mrow[(int64_t)q * ld] : T(0);
+    }
     for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
         __syncthreads();
@@ -326,10 +339,20 @@ __global__ __launch_bounds__(256) void k_rect_part(SolveArgs a, int kind)
         }
         __syncthreads();
         T acc[SOLVE_RC] = {};
-        if (row < nrows) dot_rows<T, 32>(rowp + (int64_t)col0 * ld, ld, bs, sv, SOLVE_CK, acc);
 #pragma unroll
-        for (int r = 0; r < SOLVE_RC; r++)
-            if (r < rc) part[(int64_t)(r0 + r) * 256 + tid] = acc[r];
+        for (int q = 0; q < HC; q++)
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) acc[r] += l[q] * sv[r * SOLVE_CK + hf * HC + q];
+        if (hf == 1) {
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) red[r * 128 + lr] = acc[r];
+        }
+        __syncthreads();
+        if (hf == 0) {
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++)
+                if (r < rc) part[(int64_t)(r0 + r) * 256 + ir] = acc[r] + red[r * 128 + lr];
+        }
     }
 }
 
@@ -1634,7 +1657,8 @@ hipError_t launch_solve_level(int which, const SolveArgs &a, const int32_t *list
     // right-hand-side chunks of SOLVE_RC in grid z
     const int nz = (int)((a.nrhs + SOLVE_RC - 1) / SOLVE_RC);
     const bool sum = which == 2 || which == 4 || which == 7 || which == 9;   // RECT_SUM_TPR workgroups per item
-    dim3 g(sum ? RECT_SUM_TPR * count : count, nmat, nz), blk(256);
+    const bool prod = which == 1 || which == 3 || which == 6 || which == 8;  // two workgroups per item
+    dim3 g(sum ? RECT_SUM_TPR * count : prod ? 2 * count : count, nmat, nz), blk(256);
     switch (which) {
     case 0: hipLaunchKernelGGL(k_fwd_tri<T>, g, blk, 0, st, b); break;
     case 1: hipLaunchKernelGGL(k_rect_part<T>, g, blk, 0, st, b, 0); break;
