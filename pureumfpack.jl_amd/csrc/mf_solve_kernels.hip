@@ -433,50 +433,62 @@ __global__ __launch_bounds__(256) void k_rect_sum(SolveArgs a, int kind)
 }
 
 // Forward L21 product without split-K, for levels whose split fronts all have np <= 4 *
-// SOLVE_CK: uv[t] -= sum_a L21[t, a] y[c1 + a].  Items int4 {front, row0}: 64 rows per
-// 256-thread workgroup, wave w takes columns 64 w .. 64 w + 63 (32 loads in flight), and the
-// four partial sums of a row are added in wave order in LDS.
+// SOLVE_CK: uv[t] -= sum_a L21[t, a] y[c1 + a].  Items int4 {front, row0} of 64 rows.
 template <typename T>
 __global__ __launch_bounds__(256) void k_rect_direct(SolveArgs a)
 {
-    constexpr int NW = 4, CK = SOLVE_CK;
-    __shared__ T sv[NW * CK * SOLVE_RC];   // y, columns 0 .. 255, by right-hand side
-    __shared__ T red[NW * SOLVE_RC * 64];
+    // two workgroups per item (32 rows each); eight column groups of 32 per row, each thread's
+    // operand values loaded before the vector is staged; the eight partial sums of a row are
+    // added in group order in LDS
+    constexpr int NG = 8, CW = 32, NC = NG * CW;
+    static_assert(NC == 4 * SOLVE_CK, "the direct product covers pivot blocks of four chunks");
+    __shared__ T sv[NC * SOLVE_RC];   // y, columns 0 .. 255, by right-hand side
+    __shared__ T red[NG * SOLVE_RC * 32];
     const int mat = blockIdx.y;
-    const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x];
+    const int4 w = reinterpret_cast<const int4 *>(a.list)[blockIdx.x >> 1];
     const SnDev d = a.sn[w.x];
     const int nf = d.nf, np = d.np, nu = d.nu, c1 = d.c1;
     const T *F = reinterpret_cast<const T *>(a.store) + mat * a.store_stride + d.foff;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int row = w.y + lane;
+    const int tid = threadIdx.x, lr = tid & 31, cg = tid >> 5;
+    const int row = w.y + ((int)blockIdx.x & 1) * 32 + lr;
     const int64_t vb = (int64_t)mat * a.nrhs;
     const T *y = reinterpret_cast<const T *>(a.y) + vb * a.n;
     T *uv = reinterpret_cast<T *>(a.uv) + vb * a.nupd;
-    const int col0 = wv * CK;
-    const int bs = max(0, min(CK, np - col0));
+    const int col0 = cg * CW;
+    const int bs = max(0, min(CW, np - col0));
+    T l[CW];
+    {
+        const T *mrow = F + (np + row) + (int64_t)col0 * nf;
+        const bool on = row < nu;
+#pragma unroll
+        for (int q = 0; q < CW; q++) l[q] = (on && q < bs) ? mrow[(int64_t)q * nf] : T(0);
+    }
     const int nrhs = (int)a.nrhs;
     for (int r0 = blockIdx.z * SOLVE_RC; r0 < nrhs; r0 += gridDim.z * SOLVE_RC) {
         const int rc = min(SOLVE_RC, nrhs - r0);
         __syncthreads();
 #pragma unroll
-        for (int h = 0; h < NW * CK * SOLVE_RC / 256; h++) {
+        for (int h = 0; h < NC * SOLVE_RC / 256; h++) {
             const int e = tid + h * 256;
-            const int j = e % (NW * CK), r = e / (NW * CK);
+            const int j = e % NC, r = e / NC;
             sv[e] = (j < np && r < rc) ? y[(int64_t)(r0 + r) * a.n + c1 + j] : T(0);
         }
         __syncthreads();
         T acc[SOLVE_RC] = {};
-        if (row < nu) dot_rows<T, 32>(F + (np + row) + (int64_t)col0 * nf, nf, bs, sv + col0, NW * CK, acc);
 #pragma unroll
-        for (int r = 0; r < SOLVE_RC; r++) red[(wv * SOLVE_RC + r) * 64 + lane] = acc[r];
+        for (int q = 0; q < CW; q++)
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++) acc[r] += l[q] * sv[r * NC + col0 + q];
+#pragma unroll
+        for (int r = 0; r < SOLVE_RC; r++) red[(cg * SOLVE_RC + r) * 32 + lr] = acc[r];
         __syncthreads();
-        if (wv == 0 && row < nu) {
+        if (cg == 0 && row < nu) {
 #pragma unroll
             for (int r = 0; r < SOLVE_RC; r++) {
                 if (r >= rc) continue;
-                const T sum = ((red[(0 * SOLVE_RC + r) * 64 + lane] + red[(1 * SOLVE_RC + r) * 64 + lane]) +
-                               red[(2 * SOLVE_RC + r) * 64 + lane]) +
-                              red[(3 * SOLVE_RC + r) * 64 + lane];
+                T sum = red[r * 32 + lr];
+#pragma unroll
+                for (int q = 1; q < NG; q++) sum += red[(q * SOLVE_RC + r) * 32 + lr];
                 uv[(int64_t)(r0 + r) * a.nupd + d.upd_off + row] -= sum;
             }
         }
@@ -1657,7 +1669,7 @@ hipError_t launch_solve_level(int which, const SolveArgs &a, const int32_t *list
     // right-hand-side chunks of SOLVE_RC in grid z
     const int nz = (int)((a.nrhs + SOLVE_RC - 1) / SOLVE_RC);
     const bool sum = which == 2 || which == 4 || which == 7 || which == 9;   // RECT_SUM_TPR workgroups per item
-    const bool prod = which == 1 || which == 3 || which == 6 || which == 8;  // two workgroups per item
+    const bool prod = which == 1 || which == 3 || which == 6 || which == 8 || which == 17;   // two workgroups per item
     dim3 g(sum ? RECT_SUM_TPR * count : prod ? 2 * count : count, nmat, nz), blk(256);
     switch (which) {
     case 0: hipLaunchKernelGGL(k_fwd_tri<T>, g, blk, 0, st, b); break;
