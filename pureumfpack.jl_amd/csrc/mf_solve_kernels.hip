@@ -39,15 +39,16 @@ hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st)
 }
 
 constexpr int SOLVE_RC = 8;   // right-hand sides processed together per pass
+constexpr int RW = SOLVE_RC / 4;   // right-hand sides per wave in the diagonal-block solves
 constexpr int SOLVE_NB = 64;  // triangular block size (one wave solves a diagonal block)
 
-// One wave: v <- L^-1 v for the unit lower bs x bs block staged in sd (leading dimension
-// ldd, lane = row; rows and columns >= bs of sd and v are zero).  Columns in groups of 8,
+// One wave: v <- L^-1 v (R right-hand sides) for the unit lower bs x bs block staged in sd
+// (leading dimension ldd, lane = row; rows and columns >= bs of sd and v are zero).  Columns in groups of 8,
 // read from LDS before their 8 steps, without branches: a step past bs has a zero column and
 // changes nothing.  The value of lane j reaches the other lanes by v_readlane (j is
 // wave-uniform).
-template <typename T>
-__device__ __forceinline__ void wave_lower_solve(const T *sd, int ldd, int bs, int lane, T (&v)[SOLVE_RC])
+template <typename T, int R>
+__device__ __forceinline__ void wave_lower_solve(const T *sd, int ldd, int bs, int lane, T (&v)[R])
 {
     for (int j0 = 0; j0 < bs; j0 += 8) {
         T l[8];
@@ -59,7 +60,7 @@ __device__ __forceinline__ void wave_lower_solve(const T *sd, int ldd, int bs, i
 #pragma unroll
         for (int q = 0; q < 8; q++)
 #pragma unroll
-            for (int r = 0; r < SOLVE_RC; r++) v[r] -= l[q] * lane_value(v[r], j0 + q);
+            for (int r = 0; r < R; r++) v[r] -= l[q] * lane_value(v[r], j0 + q);
     }
 }
 
@@ -67,8 +68,8 @@ __device__ __forceinline__ void wave_lower_solve(const T *sd, int ldd, int bs, i
 // takes x_j = v_j / u_jj (one division per step), the lanes above subtract u_ij x_j.
 // Each lane divides once (its own 1/u_jj); step j takes lane j's by v_readlane.  Steps below
 // column 0 (the last group) have u = 0 and 1/u_jj = 0 and change nothing.
-template <typename T>
-__device__ __forceinline__ void wave_upper_solve(const T *sd, int ldd, int bs, int lane, T (&v)[SOLVE_RC])
+template <typename T, int R>
+__device__ __forceinline__ void wave_upper_solve(const T *sd, int ldd, int bs, int lane, T (&v)[R])
 {
     const T rd = lane < bs ? T(1) / sd[lane + lane * ldd] : T(1);
     for (int j1 = bs - 1; j1 >= 0; j1 -= 8) {
@@ -84,7 +85,7 @@ __device__ __forceinline__ void wave_upper_solve(const T *sd, int ldd, int bs, i
         for (int q = 0; q < 8; q++) {
             const int j = j1 - q;
 #pragma unroll
-            for (int r = 0; r < SOLVE_RC; r++) {
+            for (int r = 0; r < R; r++) {
                 const T t = lane_value(v[r], max(j, 0)) * rj[q];
                 v[r] = lane == j ? t : v[r] - u[q] * t;
             }
@@ -185,14 +186,22 @@ __global__ __launch_bounds__(256) void k_fwd_tri(SolveArgs a)
             sd[(e % SOLVE_NB) + (e / SOLVE_NB) * LDD] = d0[h];
         }
         __syncthreads();
-        if (tid < 64) {
-            T v[SOLVE_RC];
+        {
+            // the right-hand sides split over the four waves, RW each
+            const int rw0 = (tid >> 6) * RW;
+            T v[RW];
 #pragma unroll
-            for (int r = 0; r < SOLVE_RC; r++) v[r] = lane < bs0 ? g[r * TRI_NP + lane] : T(0);
-            wave_lower_solve<T>(sd, LDD, bs0, lane, v);
+            for (int rr = 0; rr < RW; rr++) {
+                const int r = rw0 + rr;
+                v[rr] = lane < bs0 ? g[r * TRI_NP + lane] : T(0);
+            }
+            wave_lower_solve<T, RW>(sd, LDD, bs0, lane, v);
             if (lane < bs0) {
 #pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++) g[r * TRI_NP + lane] = v[r];
+                for (int rr = 0; rr < RW; rr++) {
+                    const int r = rw0 + rr;
+                    g[r * TRI_NP + lane] = v[rr];
+                }
             }
         }
         __syncthreads();
@@ -211,18 +220,20 @@ __global__ __launch_bounds__(256) void k_fwd_tri(SolveArgs a)
                 sd[(e % SOLVE_NB) + (e / SOLVE_NB) * LDD] = d1[h];
             }
             __syncthreads();
-            if (tid < 64) {
-                T v[SOLVE_RC];
+            {
+                const int rw0 = (tid >> 6) * RW;
+                T v[RW];
 #pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++) {
+                for (int rr = 0; rr < RW; rr++) {
+                    const int r = rw0 + rr;
                     const T sum = ((red[r * NT + lane] + red[r * NT + 64 + lane]) + red[r * NT + 128 + lane]) +
                                   red[r * NT + 192 + lane];
-                    v[r] = lane < bs1 ? g[r * TRI_NP + SOLVE_NB + lane] - sum : T(0);
+                    v[rr] = lane < bs1 ? g[r * TRI_NP + SOLVE_NB + lane] - sum : T(0);
                 }
-                wave_lower_solve<T>(sd, LDD, bs1, lane, v);
+                wave_lower_solve<T, RW>(sd, LDD, bs1, lane, v);
                 if (lane < bs1) {
 #pragma unroll
-                    for (int r = 0; r < SOLVE_RC; r++) g[r * TRI_NP + SOLVE_NB + lane] = v[r];
+                    for (int rr = 0; rr < RW; rr++) g[(rw0 + rr) * TRI_NP + SOLVE_NB + lane] = v[rr];
                 }
             }
             __syncthreads();
@@ -545,14 +556,22 @@ __global__ __launch_bounds__(256) void k_bwd_tri(SolveArgs a)
                 sd[(e % SOLVE_NB) + (e / SOLVE_NB) * LDD] = d1[h];
             }
             __syncthreads();
-            if (tid < 64) {
-                T v[SOLVE_RC];
+            {
+                // the right-hand sides split over the four waves, RW each
+                const int rw0 = (tid >> 6) * RW;
+                T v[RW];
 #pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++) v[r] = lane < bs1 ? g[r * TRI_NP + SOLVE_NB + lane] : T(0);
-                wave_upper_solve<T>(sd, LDD, bs1, lane, v);
+                for (int rr = 0; rr < RW; rr++) {
+                    const int r = rw0 + rr;
+                    v[rr] = lane < bs1 ? g[r * TRI_NP + SOLVE_NB + lane] : T(0);
+                }
+                wave_upper_solve<T, RW>(sd, LDD, bs1, lane, v);
                 if (lane < bs1) {
 #pragma unroll
-                    for (int r = 0; r < SOLVE_RC; r++) g[r * TRI_NP + SOLVE_NB + lane] = v[r];
+                    for (int rr = 0; rr < RW; rr++) {
+                        const int r = rw0 + rr;
+                        g[r * TRI_NP + SOLVE_NB + lane] = v[rr];
+                    }
                 }
             }
             __syncthreads();
@@ -571,20 +590,22 @@ __global__ __launch_bounds__(256) void k_bwd_tri(SolveArgs a)
             sd[(e % SOLVE_NB) + (e / SOLVE_NB) * LDD] = d0[h];
         }
         __syncthreads();
-        if (tid < 64) {
-            T v[SOLVE_RC];
+        {
+            const int rw0 = (tid >> 6) * RW;
+            T v[RW];
 #pragma unroll
-            for (int r = 0; r < SOLVE_RC; r++) {
+            for (int rr = 0; rr < RW; rr++) {
+                const int r = rw0 + rr;
                 T val = lane < bs0 ? g[r * TRI_NP + lane] : T(0);
                 if (bs1 > 0 && lane < bs0)
                     val -= ((red[r * NT + lane] + red[r * NT + 64 + lane]) + red[r * NT + 128 + lane]) +
                            red[r * NT + 192 + lane];
-                v[r] = val;
+                v[rr] = val;
             }
-            wave_upper_solve<T>(sd, LDD, bs0, lane, v);
+            wave_upper_solve<T, RW>(sd, LDD, bs0, lane, v);
             if (lane < bs0) {
 #pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++) g[r * TRI_NP + lane] = v[r];
+                for (int rr = 0; rr < RW; rr++) g[(rw0 + rr) * TRI_NP + lane] = v[rr];
             }
         }
         __syncthreads();
@@ -862,14 +883,22 @@ __device__ __forceinline__ void fwd_front_b(const SolveArgs &a, const int32_t *_
                 sd[(e % SOLVE_NB) + (e / SOLVE_NB) * LDD] = dpre[h];
             }
             __syncthreads();
-            if (tid < 64) {
-                T v[SOLVE_RC];
+            {
+                // the right-hand sides split over the four waves, RW each
+                const int rw0 = (tid >> 6) * RW;
+                T v[RW];
 #pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++) v[r] = lane < bs ? g[r * MAXNF + b0 + lane] : T(0);
-                wave_lower_solve<T>(sd, LDD, bs, lane, v);
+                for (int rr = 0; rr < RW; rr++) {
+                    const int r = rw0 + rr;
+                    v[rr] = lane < bs ? g[r * MAXNF + b0 + lane] : T(0);
+                }
+                wave_lower_solve<T, RW>(sd, LDD, bs, lane, v);
                 if (lane < bs) {
 #pragma unroll
-                    for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + b0 + lane] = v[r];
+                    for (int rr = 0; rr < RW; rr++) {
+                        const int r = rw0 + rr;
+                        g[r * MAXNF + b0 + lane] = v[rr];
+                    }
                 }
             }
             __syncthreads();
@@ -1007,14 +1036,22 @@ __device__ __forceinline__ void bwd_front_b(const SolveArgs &a, int slot, int ma
                 sd[i + j * LDD] = (i < bs && j < bs) ? F[(b0 + i) + (int64_t)(b0 + j) * nf] : T(0);
             }
             __syncthreads();
-            if (tid < 64) {
-                T v[SOLVE_RC];
+            {
+                // the right-hand sides split over the four waves, RW each
+                const int rw0 = (tid >> 6) * RW;
+                T v[RW];
 #pragma unroll
-                for (int r = 0; r < SOLVE_RC; r++) v[r] = lane < bs ? g[r * MAXNF + b0 + lane] : T(0);
-                wave_upper_solve<T>(sd, LDD, bs, lane, v);
+                for (int rr = 0; rr < RW; rr++) {
+                    const int r = rw0 + rr;
+                    v[rr] = lane < bs ? g[r * MAXNF + b0 + lane] : T(0);
+                }
+                wave_upper_solve<T, RW>(sd, LDD, bs, lane, v);
                 if (lane < bs) {
 #pragma unroll
-                    for (int r = 0; r < SOLVE_RC; r++) g[r * MAXNF + b0 + lane] = v[r];
+                    for (int rr = 0; rr < RW; rr++) {
+                        const int r = rw0 + rr;
+                        g[r * MAXNF + b0 + lane] = v[rr];
+                    }
                 }
             }
             __syncthreads();
