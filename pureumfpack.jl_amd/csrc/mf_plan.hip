@@ -1950,10 +1950,12 @@ static int solve_core(mf_plan *P, int64_t nrhs, int nmat, int32_t refine_steps, 
     int it = 0, conv = 0;
     std::vector<double> h;
     double be = 0, r2 = 0;
+    bool fresh = false;   // h holds the norms of the current solution
     for (; it < refine_steps; it++) {
         if (bwd_tol > 0) {
             if ((rc = norms(h)) != MF_OK) return rc;
             bwd(h, be, r2);
+            fresh = true;
             if (be <= bwd_tol) { conv = 1; break; }
         } else {
             HIPCHK(launch_residual(P->d_csr_rp.as<int32_t>(), P->d_csr_ci.as<int32_t>(),
@@ -1962,10 +1964,13 @@ static int solve_core(mf_plan *P, int64_t nrhs, int nmat, int32_t refine_steps, 
                                    P->stream));
         }
         if ((rc = once(P->sb.R, 1)) != MF_OK) return rc;
+        fresh = false;
     }
     if (rep) {
-        if ((rc = norms(h)) != MF_OK) return rc;
-        bwd(h, be, r2);
+        if (!fresh) {
+            if ((rc = norms(h)) != MF_OK) return rc;
+            bwd(h, be, r2);
+        }
         if (bwd_tol > 0 && be <= bwd_tol) conv = 1;
         rep->iterations = it;
         rep->converged = conv;
