@@ -222,13 +222,21 @@ struct mf_plan {
         int nmat = 0;
         const double *rhs = nullptr;
         const void *w0 = nullptr;
-    } sgraph[2];                        // captured sweeps: [0] solve from B, [1] correction from R
+    } sgraph[3];   // captured sweeps: [0] solve from B, [1] correction from R, [2] [0] with the top inversions
     DevBuf d_Lp, d_Up, d_uoff, d_Li, d_Lx, d_Ui, d_Ux, d_rowfac;
     int64_t solve_nrhs = 0;
     bool factored = false;
     int nmat_factored = 0;
     int rs_given = 0;
     hipStream_t stream = nullptr;
+    // pivot-block inversions of the top level's fronts: run by the first solve after a
+    // factorization on stream `side`, beside the sweeps of the levels below (which do not
+    // read them), joined before the top level's sweep
+    std::vector<Launch> inv_top;
+    bool inv_pending = false;   // factored, top-level inverses not computed yet
+    bool side_busy = false;     // inversions enqueued on `side` that no host sync has covered
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_inv = nullptr;
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
     int graph_nmat[3] = {0, 0, 0};
     int graph_rs[3] = {0, 0, 0};
@@ -243,10 +251,14 @@ struct mf_plan {
 
     ~mf_plan()
     {
+        if (side) (void)hipStreamSynchronize(side);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_inv) (void)hipEventDestroy(ev_inv);
+        if (side) (void)hipStreamDestroy(side);
         for (int g = 0; g < 3; g++)
             if (graph[g]) (void)hipGraphExecDestroy(graph[g]);
-        for (int g = 0; g < 2; g++)
+        for (int g = 0; g < 3; g++)
             if (sgraph[g].exec) (void)hipGraphExecDestroy(sgraph[g].exec);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -1117,11 +1129,16 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
             sched.push_back(L);
         }
     }
-    // recursive inversion of the pivot-block factors of the large fronts (after every level)
-    {
+    // recursive inversion of the pivot-block factors of the large fronts (after every level).
+    // The fronts of the top level are inverted in launches of their own (P.inv_top, built once)
+    // that the first solve runs beside the sweeps of the levels below, which do not need them.
+    for (int part = 0; part < (sp == 0 ? 2 : 1); part++) {
+        const bool top = part == 1;
+        std::vector<Launch> &idst = top ? P.inv_top : sched;
+        gemm_dst = &idst;
         std::vector<int32_t> items;
         for (int64_t s = 0; s < nsuper; s++)
-            if (ioff[s] >= 0)
+            if (ioff[s] >= 0 && (level[s] == nlev - 1) == top)
                 for (int64_t b0 = 0; b0 < np[s]; b0 += INV_LEAF) {
                     items.push_back((int32_t)s); items.push_back((int32_t)b0); items.push_back(0); items.push_back(0);
                 }
@@ -1139,12 +1156,12 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                 Lf.flops += 2.0 * b * b * b / 3.0;
                 Lf.bytes += 3.0 * b * b * elem;
             }
-            sched.push_back(Lf);
+            idst.push_back(Lf);
             for (int64_t half = INV_LEAF;; half *= 2) {
                 std::vector<GemmProb> t1, t2;
                 int64_t toff = pe;   // temps live after the inverses
                 for (int64_t s = 0; s < nsuper; s++) {
-                    if (ioff[s] < 0) continue;
+                    if (ioff[s] < 0 || (level[s] == nlev - 1) != top) continue;
                     const int64_t f = foff[s], Li = ioff[s], Ui = ioff[s] + np[s] * np[s], ld = np[s];
                     for (int64_t a0 = 0; a0 + half < np[s]; a0 += 2 * half) {
                         int64_t m = a0 + half, b = std::min(a0 + 2 * half, np[s]);
@@ -1185,6 +1202,7 @@ static int build_plan(mf_plan &P, int64_t n, const int64_t *qf1, int64_t nsuper,
                 add_gemm(t2, (int)nlev, 2);
             }
         }
+        gemm_dst = &sched;
     }
     }
     P.store_elems = pe + P.tmp_elems;
@@ -1466,6 +1484,11 @@ extern "C" int mf_plan_create(int64_t n, const int64_t *qf, int64_t nsuper, cons
         return fail(MF_OOM, "host allocation failed while building the plan");
     }
     if (rc != MF_OK) return rc;
+    if (!P->inv_top.empty()) {
+        HIPCHK(hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&P->ev_inv, hipEventDisableTiming));
+    }
     *out = P.release();
     return MF_OK;
 }
@@ -1533,8 +1556,50 @@ extern "C" int mf_plan_get_stats(const mf_plan *P, mf_plan_stats *st)
 // pivoted schedule for only the matrices whose interchange-free attempt failed.
 enum { G_SPEC = 0, G_PIV_ALL = 1, G_PIV_REDO = 2 };
 
+static FactorArgs factor_args(mf_plan &P, const double *d_ax, const double *d_rs, bool redo);
+
+// The top level's pivot-block inversions on stream st, for every factored matrix.
+template <typename T>
+static hipError_t launch_inv_top(mf_plan &P, int nmat, hipStream_t st)
+{
+    const FactorArgs a = factor_args(P, nullptr, nullptr, false);
+    for (const Launch &L : P.inv_top) {
+        hipError_t e = launch_factor_step<T>(L, a, P.d_lists.as<int32_t>(), P.d_probs.as<GemmProb>(),
+                                             P.d_tiles.as<GemmTile>(), nmat, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 template <typename T>
 static int run_factor(mf_plan &P, const double *d_ax, const double *d_rs, int nmat, int sp, bool redo)
+{
+    FactorArgs a = factor_args(P, d_ax, d_rs, redo);
+    HIPCHK(launch_fill_i32_masked(P.d_info.as<int32_t>(), INT32_MAX, a.only, nmat, P.stream));
+    if (!redo) HIPCHK(launch_fill_i32(P.d_flag.as<int32_t>(), 0, nmat, P.stream));
+    for (const Launch &L : P.sched[sp])
+        HIPCHK(profiled(P, launch_class(L), L.flops * nmat, L.bytes * nmat, [&] {
+            return launch_factor_step<T>(L, a, P.d_lists.as<int32_t>(), P.d_probs.as<GemmProb>(),
+                                         P.d_tiles.as<GemmTile>(), nmat, P.stream);
+        }));
+    if (P.prof) {
+        // timed launch by launch: the top inversions stay in the factorization
+        for (const Launch &L : P.inv_top)
+            HIPCHK(profiled(P, launch_class(L), L.flops * nmat, L.bytes * nmat, [&] {
+                return launch_factor_step<T>(L, a, P.d_lists.as<int32_t>(), P.d_probs.as<GemmProb>(),
+                                             P.d_tiles.as<GemmTile>(), nmat, P.stream);
+            }));
+        P.inv_pending = false;
+    } else {
+        P.inv_pending = !P.inv_top.empty();
+    }
+    if (!redo)
+        HIPCHK(launch_anorm(P.d_csr_rp.as<int32_t>(), P.d_csr_vmap.as<int32_t>(), d_ax, P.nnz, P.n,
+                            P.d_npart.as<double>(), P.d_anorm.as<double>(), nmat, P.stream));
+    return MF_OK;
+}
+
+static FactorArgs factor_args(mf_plan &P, const double *d_ax, const double *d_rs, bool redo)
 {
     FactorArgs a{};
     a.sn = P.d_sn.as<SnDev>();
@@ -1569,17 +1634,7 @@ static int run_factor(mf_plan &P, const double *d_ax, const double *d_rs, int nm
     a.ax_stride = P.nnz;
     a.rs_stride = P.n;
     a.n_stride = P.n;
-    HIPCHK(launch_fill_i32_masked(P.d_info.as<int32_t>(), INT32_MAX, a.only, nmat, P.stream));
-    if (!redo) HIPCHK(launch_fill_i32(P.d_flag.as<int32_t>(), 0, nmat, P.stream));
-    for (const Launch &L : P.sched[sp])
-        HIPCHK(profiled(P, launch_class(L), L.flops * nmat, L.bytes * nmat, [&] {
-            return launch_factor_step<T>(L, a, P.d_lists.as<int32_t>(), P.d_probs.as<GemmProb>(),
-                                         P.d_tiles.as<GemmTile>(), nmat, P.stream);
-        }));
-    if (!redo)
-        HIPCHK(launch_anorm(P.d_csr_rp.as<int32_t>(), P.d_csr_vmap.as<int32_t>(), d_ax, P.nnz, P.n,
-                            P.d_npart.as<double>(), P.d_anorm.as<double>(), nmat, P.stream));
-    return MF_OK;
+    return a;
 }
 
 // Enqueues schedule sp for every matrix, or (redo) the pivoted schedule for the matrices whose
@@ -1622,6 +1677,10 @@ static int factor_common(mf_plan *P, const double *d_ax, const double *d_rs, int
                          int64_t *info_col)
 {
     int rc;
+    if (P->side_busy) {   // a solve that ended early may have left the top inversions running
+        HIPCHK(hipStreamSynchronize(P->side));
+        P->side_busy = false;
+    }
     std::vector<int32_t> info(nmat), flag(nmat, 0);
     // Every call first tries the interchange-free schedule (when it applies); the matrices whose
     // multiplier check failed are then factored again on the pivoted schedule, from the assembly
@@ -1756,9 +1815,16 @@ static int solve_columns(mf_plan &P, const double *d_rhs, int64_t nrhs, int accu
 }
 
 template <typename T>
-static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumulate, int nmat)
+static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumulate, int nmat, bool inv)
 {
     if (use_columns(P, nrhs, nmat)) return solve_columns<T>(P, d_rhs, nrhs, accumulate, nmat);
+    if (inv) {
+        // the top level's inversions beside the sweeps below it, joined before its level
+        HIPCHK(hipEventRecord(P.ev_fork, P.stream));
+        HIPCHK(hipStreamWaitEvent(P.side, P.ev_fork, 0));
+        HIPCHK(launch_inv_top<T>(P, P.nmat_factored, P.side));
+        HIPCHK(hipEventRecord(P.ev_inv, P.side));
+    }
     // d_rhs: n x nrhs per matrix (ld n); result into P.sb.X (ld n), overwritten or accumulated
     const int64_t n = P.n;
     const double *rs = P.rs_given ? P.d_rs.as<double>() : nullptr;
@@ -1793,6 +1859,7 @@ static int solve_once(mf_plan &P, const double *d_rhs, int64_t nrhs, int accumul
     auto merge_ok = [&](int64_t wcount) { return wcount * nmat * nzc <= 4096; };
     for (int64_t lv = 0; lv < P.nlevels; lv++) {
         const double vb = 3.0 * e * P.lvl_vec[lv] * nrhs;
+        if (inv && lv == P.nlevels - 1) HIPCHK(hipStreamWaitEvent(P.stream, P.ev_inv, 0));
         if (P.lvl_s64c[lv] > 0 && P.lvl_s1kc[lv] > 0 && merge_ok(P.lvl_s64c[lv])) {
             HIPCHK(profiled(P, 5, 0.0, 0.0, [&] {
                 return launch_solve_merged<T>(1, P.lvl_s1k_small[lv], a, sl, P.lvl_s1ko[lv], P.lvl_s1kc[lv],
@@ -1889,20 +1956,35 @@ static int solve_core(mf_plan *P, int64_t nrhs, int nmat, int32_t refine_steps, 
     const int64_t n = P->n;
     auto t0 = std::chrono::steady_clock::now();
     int rc;
-    auto eager = [&](const double *rhs, int acc) {
-        return P->prec == MF_FP64 ? solve_once<double>(*P, rhs, nrhs, acc, nmat)
-                                  : solve_once<float>(*P, rhs, nrhs, acc, nmat);
+    // the first solve after a factorization computes the top level's inverses beside its sweeps
+    // (timed runs and column sweeps, which read no inverse: on the stream first, or not at all)
+    bool inv = P->inv_pending && !use_columns(*P, nrhs, nmat);
+    if (inv && P->prof) {
+        HIPCHK(P->prec == MF_FP64 ? launch_inv_top<double>(*P, P->nmat_factored, P->stream)
+                                  : launch_inv_top<float>(*P, P->nmat_factored, P->stream));
+        P->inv_pending = inv = false;
+    }
+    auto eager = [&](const double *rhs, int acc, bool with_inv) {
+        return P->prec == MF_FP64 ? solve_once<double>(*P, rhs, nrhs, acc, nmat, with_inv)
+                                  : solve_once<float>(*P, rhs, nrhs, acc, nmat, with_inv);
     };
-    // With use_graph, each sweep pair (the solve from B, the correction from R) is captured once
-    // per (nrhs, nmat, buffers) and replayed: the same kernels in the same order, so the same bits.
+    // With use_graph, each sweep pair (the solve from B, the correction from R, the solve from B
+    // with the top inversions beside it) is captured once per (nrhs, nmat, buffers) and
+    // replayed: the same kernels in the same order, so the same bits.
     auto once = [&](const double *rhs, int acc) -> int {
-        if (!P->use_graph || P->prof || P->sb_scratch) return eager(rhs, acc);
-        mf_plan::SolveGraph &G = P->sgraph[acc];
+        const bool with_inv = inv && acc == 0;
+        int r0;
+        if (!P->use_graph || P->prof || P->sb_scratch) {
+            r0 = eager(rhs, acc, with_inv);
+            if (r0 == MF_OK && with_inv) { P->inv_pending = inv = false; P->side_busy = true; }
+            return r0;
+        }
+        mf_plan::SolveGraph &G = P->sgraph[with_inv ? 2 : acc];
         if (!G.exec || G.nrhs != nrhs || G.nmat != nmat || G.rhs != rhs || G.w0 != P->sb.w0) {
             if (G.exec) { (void)hipGraphExecDestroy(G.exec); G.exec = nullptr; }
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
-            const int r = eager(rhs, acc);
+            const int r = eager(rhs, acc, with_inv);
             const hipError_t ec = hipStreamEndCapture(P->stream, &g);
             if (r != MF_OK || ec != hipSuccess) {
                 if (ec == hipSuccess && g) (void)hipGraphDestroy(g);
@@ -1919,6 +2001,7 @@ static int solve_core(mf_plan *P, int64_t nrhs, int nmat, int32_t refine_steps, 
             G.w0 = P->sb.w0;
         }
         HIPCHK(hipGraphLaunch(G.exec, P->stream));
+        if (with_inv) { P->inv_pending = inv = false; P->side_busy = true; }
         return MF_OK;
     };
     auto norms = [&](std::vector<double> &h) -> int {
@@ -1979,6 +2062,7 @@ static int solve_core(mf_plan *P, int64_t nrhs, int nmat, int32_t refine_steps, 
     } else {
         HIPCHK(hipStreamSynchronize(P->stream));
     }
+    P->side_busy = false;   // joined into the stream before the top level's sweep, now synchronized
     auto t1 = std::chrono::steady_clock::now();
     if (rep) rep->seconds = std::chrono::duration<double>(t1 - t0).count();
     if (P->prof) {
