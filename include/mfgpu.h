@@ -132,7 +132,9 @@ int mf_plan_create(int64_t n, const int64_t *qf, int64_t nsuper, const int64_t *
  * pattern given to mf_plan_create), row scaling Rs (n; NULL = no scaling).  Replaces the
  * supernode loop src/multifrontal.jl:198-357.  With check != 0 an exactly-zero block
  * pivot returns MF_SINGULAR and *info_col = its column (1-based, in factor order, as
- * SingularException(c1+info-1) at src/multifrontal.jl:250-251). */
+ * SingularException(c1+info-1) at src/multifrontal.jl:250-251).  The pivot-block inverses of
+ * the top level's fronts, which only the solves use, are computed by the first solve after
+ * the factorization (beside its sweeps of the levels below). */
 int mf_factor(mf_plan *P, const double *Ax, const double *Rs, int32_t check,
               int64_t *info_col);
 /* Same with Ax and Rs already in device memory (HBM-resident inputs). */
