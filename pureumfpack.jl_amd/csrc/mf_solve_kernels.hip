@@ -111,13 +111,30 @@ __global__ __launch_bounds__(256) void k_fwd_gather(SolveArgs a, const int32_t *
     T *w0 = reinterpret_cast<T *>(a.w0) + vb * a.n;
     T *uv = reinterpret_cast<T *>(a.uv) + vb * a.nupd;
     const int q0 = ecolptr[d.fcol + p], q1 = ecolptr[d.fcol + p + 1];
+    // the offsets of the first GQ contributions, then all of their values, are requested together
+    constexpr int GQ = 4;
+    int32_t offs[GQ];
+#pragma unroll
+    for (int u = 0; u < GQ; u++) offs[u] = q0 + u < q1 ? eupd[q0 + u] : 0;
     for (int r0 = blockIdx.z * SOLVE_RC; r0 < (int)a.nrhs; r0 += gridDim.z * SOLVE_RC) {
         const int rc = min(SOLVE_RC, (int)a.nrhs - r0);
         T acc[SOLVE_RC];
 #pragma unroll
         for (int r = 0; r < SOLVE_RC; r++)
             acc[r] = (r < rc && p < d.np) ? w0[(int64_t)(r0 + r) * a.n + d.c1 + p] : T(0);
-        for (int q = q0; q < q1; q++) {
+        T v[GQ][SOLVE_RC];
+#pragma unroll
+        for (int u = 0; u < GQ; u++)
+#pragma unroll
+            for (int r = 0; r < SOLVE_RC; r++)
+                v[u][r] = (q0 + u < q1 && r < rc) ? uv[(int64_t)(r0 + r) * a.nupd + offs[u]] : T(0);
+#pragma unroll
+        for (int u = 0; u < GQ; u++)
+            if (q0 + u < q1) {
+#pragma unroll
+                for (int r = 0; r < SOLVE_RC; r++) acc[r] += v[u][r];
+            }
+        for (int q = q0 + GQ; q < q1; q++) {
             const int32_t off = eupd[q];
 #pragma unroll
             for (int r = 0; r < SOLVE_RC; r++)
