@@ -1976,7 +1976,10 @@ static int solve_core(mf_plan *P, int64_t nrhs, int nmat, int32_t refine_steps, 
         int r0;
         if (!P->use_graph || P->prof || P->sb_scratch) {
             r0 = eager(rhs, acc, with_inv);
-            if (r0 == MF_OK && with_inv) { P->inv_pending = inv = false; P->side_busy = true; }
+            if (with_inv) {
+                P->side_busy = true;   // even on an error part of it may have been enqueued
+                if (r0 == MF_OK) P->inv_pending = inv = false;
+            }
             return r0;
         }
         mf_plan::SolveGraph &G = P->sgraph[with_inv ? 2 : acc];
